@@ -1,0 +1,191 @@
+"""Benchmark: particle-steps/s of the 3D GSPH Sedov step at 1M particles.
+
+One step = TimeStep, predict, tree build, PreInteraction, FluidForce,
+(GravityForce), correct -- the reference's Solver::integrate
+(src/solver.cpp:966-978) -- on a device-resident state.  The timed region is
+exactly K steps of sphgpu_integrate, which returns only when the stream has
+drained; inputs are resident in HBM before it starts.
+
+Prints one JSON line (the driver's contract), with two extra objects:
+  roofline      achieved algorithmic FLOP rate of the dominant kernel (HIP
+                events on the library's stream over the timed region) against
+                the MI355X FP32 peak;
+  cpu_baseline  the oracle (CPU restatement of the reference) on a bounded
+                sample of the same workload on this host's cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+REPO = Path(__file__).resolve().parent
+sys.path.insert(0, str(REPO / "sph-code-simulation_amd"))
+
+import numpy as np  # noqa: E402
+
+import ics  # noqa: E402
+import sphgpu  # noqa: E402
+
+PEAK_FP32_TFLOPS = 157.3
+PEAK_HBM_GBS = 8000.0
+
+# Algorithmic FLOP per accepted GSPH force pair (second order, iterative
+# solver), counted from the pair loop of the force kernel: reconstruction,
+# two kernel gradients, force and energy terms ~300 FLOP, plus ~44 FLOP per
+# Newton iteration of the Riemann solver (FMA = 2, div/sqrt = 1).
+FORCE_PAIR_FLOP = 300.0
+RIEMANN_ITER_FLOP = 44.0
+# PreInteraction per candidate per h-Newton iteration (~85 members, ~25
+# rejected; a mean of ~60 is used), per candidate of the final density pass
+# (~80) and per member of the MUSCL gradient pass (~80).
+PRE_NEWTON_FLOP = 60.0
+PRE_FINAL_FLOP = 80.0
+PRE_GRAD_FLOP = 80.0
+
+
+def make_workload(name: str, form: str):
+    guess = ics.GUESS_LEARNED if form == "learned" else ics.GUESS_PLAIN
+    if name == "sedov_3d_gsph_1m":
+        p, f = ics.sedov_3d(n1=100, riemann_guess=guess)
+    elif name == "sedov_3d_gsph_262k":
+        p, f = ics.sedov_3d(n1=64, riemann_guess=guess)
+    elif name == "sedov_3d_gsph_32k":
+        p, f = ics.sedov_3d(n1=32, riemann_guess=guess)
+    else:
+        raise SystemExit(f"unknown workload {name}")
+    p["pair_precision"] = ics.PAIR_FP64 if form == "fp64" else ics.PAIR_FP32
+    return p, f
+
+
+def run_form(name: str, form: str, steps: int, warmup: int, mlp_params=None):
+    p, f = make_workload(name, form)
+    n = f.shape[1]
+    ctx = sphgpu.Context(p, n)
+    ctx.upload(f)
+    if mlp_params is not None:
+        ctx.mlp_set_params(mlp_params)
+    ctx.initialize()
+    if warmup:
+        ctx.integrate(warmup)
+    e0 = ctx.energy_momentum()
+    ctx.reset_stats()
+    ctx.enable_timing(True)
+    t0 = time.perf_counter()
+    ctx.integrate(steps)
+    t1 = time.perf_counter()
+    times = ctx.kernel_times()
+    ctx.enable_timing(False)
+    stats = ctx.stats()
+    e1 = ctx.energy_momentum()
+    sim_t = ctx.time
+    ctx.close()
+    wall = t1 - t0
+    return dict(n=n, wall=wall, value=n * steps / wall, times=times, stats=stats, e0=e0, e1=e1, time=sim_t)
+
+
+def roofline(res: dict, steps: int) -> dict:
+    st = res["stats"]
+    times = res["times"]
+    pairs = st["force_pairs"] / steps
+    k_r = st["riemann_iters"] / max(1, st["riemann_calls"])
+    force_flop = pairs * (FORCE_PAIR_FLOP + RIEMANN_ITER_FLOP * k_r)
+    cand = st["pre_candidates"] / steps
+    k_h = st["pre_newton_iters"] / max(1, res["n"] * steps)
+    pre_flop = cand * (PRE_NEWTON_FLOP * k_h + PRE_FINAL_FLOP) + res["n"] * 55 * PRE_GRAD_FLOP
+    flops = {"fluid_force": force_flop, "pre_interaction": pre_flop}
+    totals = {k: v[0] * v[1] for k, v in times.items()}
+    dom = max((k for k in flops if k in times), key=lambda k: totals[k])
+    avg_ms = times[dom][0]
+    achieved = flops[dom] / (avg_ms * 1e-3) / 1e12
+    return {
+        "kernel": dom, "bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
+        "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
+        "flop_per_launch": flops[dom], "avg_ms": avg_ms,
+        "note": "FP32 VALU pair loop against the MI355X FP32 peak (157.3 TFLOP/s)",
+    }
+
+
+def cpu_baseline(seconds_hint: float = 15.0) -> dict:
+    """The oracle on a bounded sample: Sedov 3D GSPH 64^3, 4 steps."""
+    sys.path.insert(0, str(REPO / "tests"))
+    from checkers import Oracle, _oracle_lib  # noqa: F401
+
+    import ctypes
+    lib = ctypes.CDLL(str(REPO / "oracle" / "liboracle.so"))
+    lib.orc_num_threads.restype = ctypes.c_int
+    cores = lib.orc_num_threads()
+    p, f = ics.sedov_3d(n1=64)
+    o = Oracle(p, f)
+    o.initialize()
+    steps = 4
+    t0 = time.perf_counter()
+    o.integrate(steps)
+    t1 = time.perf_counter()
+    n = f.shape[1]
+    return {"value": round(n * steps / (t1 - t0), 1), "unit": "particle-steps/s", "cores": cores,
+            "kind": "port", "sample": f"oracle, Sedov 3D GSPH 64^3 (262144 particles), {steps} steps, "
+                                      f"{t1 - t0:.1f} s"}
+
+
+def main() -> None:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="sedov_3d_gsph_1m")
+    ap.add_argument("--form", default="fp32", choices=["fp32", "fp64", "learned"])
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported"}))
+        raise SystemExit(2)
+
+    res = run_form(args.workload, args.form, args.steps, args.warmup)
+    rl = roofline(res, args.steps)
+    out = {
+        "metric": "particle-steps/s at 1M 3D GSPH: learned-FP32 vs plain, energy drift <=1e-4",
+        "value": round(res["value"], 1),
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(res["wall"] / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32" if args.form != "fp64" else "f64",
+        "data": "synthetic",
+        "config": {"workload": args.workload, "particles": res["n"], "form": args.form,
+                   "sph": "gsph-2nd-order", "riemann": "iterative", "kernel": "cubic_spline",
+                   "neighbor_number": 50, "leaf": 32, "periodic": True},
+        "roofline": rl,
+    }
+    st = res["stats"]
+    out["counters"] = {
+        "riemann_iters_per_call": round(st["riemann_iters"] / max(1, st["riemann_calls"]), 4),
+        "riemann_wave_max_per_slot": round(st["riemann_wave_max_iters"] * 64 / max(1, st["riemann_calls"]), 4),
+        "riemann_fallbacks": st["riemann_fallbacks"], "riemann_learned": st["riemann_learned"],
+        "force_pairs_per_particle": round(st["force_pairs"] / (res["n"] * args.steps), 3),
+        "pre_candidates_per_particle": round(st["pre_candidates"] / (res["n"] * args.steps), 3),
+        "h_newton_iters_per_particle": round(st["pre_newton_iters"] / (res["n"] * args.steps), 3),
+        "nodes": st["nodes"], "leaves": st["leaves"],
+    }
+    out["energy_drift_timed"] = float((res["e1"][3] - res["e0"][3]) / abs(res["e0"][3]))
+    out["kernel_ms"] = {k: round(v[0], 4) for k, v in res["times"].items()}
+    if not args.no_fp64 and args.form != "fp64":
+        r64 = run_form(args.workload, "fp64", args.steps, args.warmup)
+        out["plain_fp64"] = {"value": round(r64["value"], 1), "ms_per_step": round(r64["wall"] / args.steps * 1e3, 4),
+                             "ratio": round(res["value"] / r64["value"], 3),
+                             "kernel_ms": {k: round(v[0], 4) for k, v in r64["times"].items()}}
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,242 @@
+/*
+ * sphgpu.h -- C-ABI of the MI355X-native SPH per-step hot path.
+ *
+ * One opaque context owns the device-resident particle state (SoA in HBM),
+ * the octree, the neighbour lists and one HIP stream.  Every entry point is
+ * extern "C", takes plain pointers and sizes, returns 0 on success or a
+ * negative SPHGPU_E* code (text via sphgpu_last_error), never throws, and is
+ * blocking on return (stream-ordered work is complete).  One host thread per
+ * context; a context is not thread-safe.
+ *
+ * The entry points replace, one for one, the calls the reference driver makes
+ * on its Module objects (reference: include/module.hpp:10-14,
+ * src/solver.cpp:941-978):
+ *
+ *   sphgpu_build_tree       <- Simulation::make_tree / BHTree::make
+ *                              (src/simulation.cpp:40-43, src/bhtree.cpp:59-115)
+ *   sphgpu_pre_interaction  <- {,disph::,gsph::}PreInteraction::calculation
+ *                              (src/pre_interaction.cpp:39-169,
+ *                               src/disph/d_pre_interaction.cpp:21-162,
+ *                               src/gsph/g_pre_interaction.cpp:27-144)
+ *   sphgpu_fluid_force      <- {,disph::,gsph::}FluidForce::calculation
+ *                              (src/fluid_force.cpp:26-86,
+ *                               src/disph/d_fluid_force.cpp:26-86,
+ *                               src/gsph/g_fluid_force.cpp:44-171)
+ *   sphgpu_gravity          <- GravityForce::calculation (src/gravity_force.cpp:52-89)
+ *   sphgpu_timestep         <- TimeStep::calculation (src/timestep.cpp:18-38)
+ *   sphgpu_predict/correct  <- Solver::predict / Solver::correct
+ *                              (src/solver.cpp:980-1062, 1064-1113)
+ *   sphgpu_energy_momentum  <- Solver::compute_total_energies (src/solver.cpp:1149-1175)
+ *
+ * Host particle data crosses the boundary as one FP64 block of
+ * SPHGPU_NFIELDS rows of n values (row-major, [field][particle], particle
+ * index == SPHParticle::id), selected by a field bit mask.  The layout is the
+ * reference SPHParticle (include/particle.hpp:8-43) plus the GSPH MUSCL
+ * gradient arrays (src/solver.cpp:707-719) flattened to scalar rows.
+ */
+#ifndef SPHGPU_H
+#define SPHGPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- enums mirroring include/parameters.hpp:8-27 ------------------------ */
+enum { SPHGPU_SSPH = 0, SPHGPU_DISPH = 1, SPHGPU_GSPH = 2 };
+enum { SPHGPU_CUBIC_SPLINE = 0, SPHGPU_WENDLAND = 1, SPHGPU_GAUSSIAN = 2 };
+enum { SPHGPU_RIEMANN_HLL = 0, SPHGPU_RIEMANN_ITERATIVE = 1 };
+/* pair-loop arithmetic: FP32 (compensated sums) or FP64 (reference-equivalent) */
+enum { SPHGPU_PAIR_FP32 = 0, SPHGPU_PAIR_FP64 = 1 };
+/* Riemann starting guess: the reference's linear impedance guess, or the MLP */
+enum { SPHGPU_GUESS_PLAIN = 0, SPHGPU_GUESS_LEARNED = 1 };
+
+/* ---- error codes --------------------------------------------------------- */
+enum {
+    SPHGPU_OK = 0,
+    SPHGPU_E_ARG = -1,          /* bad argument / unsupported parameter      */
+    SPHGPU_E_HIP = -2,          /* HIP runtime error                         */
+    SPHGPU_E_NEIGHBOR_OVERFLOW = -3, /* bhtree.cpp:129-135,300-302           */
+    SPHGPU_E_NO_FREE_NODE = -4, /* bhtree.cpp:222-224                        */
+    SPHGPU_E_STATE = -5,        /* call order violated (e.g. no tree)        */
+    SPHGPU_E_NOMEM = -6
+};
+
+/*
+ * Parameters: the keys and defaults of the reference parameter JSON
+ * (src/solver.cpp:299-438), as consumed by SPHParameters
+ * (include/parameters.hpp:28-97).  sphgpu_params_default() fills the
+ * reference's code defaults.
+ */
+typedef struct sphgpu_params {
+    int32_t dim;                 /* 1, 2 or 3 (SPH_DIM)                         */
+    int32_t sph_type;            /* SPHType: ssph | disph | gsph                */
+    int32_t kernel;              /* KernelType                                  */
+    int32_t riemann_solver;      /* gsph.riemann_solver ("hll" | "iterative")   */
+    int32_t neighbor_number;     /* neighborNumber (50)                         */
+    int32_t leaf_particle_num;   /* leafParticleNumber (1)                      */
+    int32_t max_tree_level;      /* maxTreeLevel (20)                           */
+    int32_t iterative_sml;       /* iterativeSmoothingLength (1)                */
+    int32_t use_balsara_switch;  /* useBalsaraSwitch (1)                        */
+    int32_t use_time_dependent_av; /* useTimeDependentAV (0)                    */
+    int32_t use_ac;              /* useArtificialConductivity (0)               */
+    int32_t periodic;            /* periodic (0)                                */
+    int32_t use_gravity;         /* useGravity (0)                              */
+    int32_t gsph_2nd_order;      /* use2ndOrderGSPH (1)                         */
+    int32_t pair_precision;      /* SPHGPU_PAIR_* (build-specific)              */
+    int32_t riemann_guess;       /* SPHGPU_GUESS_* (build-specific)             */
+    double gamma;                /* gamma (5/3)                                 */
+    double cfl_sound;            /* cflSound (0.3)                              */
+    double cfl_force;            /* cflForce (0.125)                            */
+    double av_alpha;             /* avAlpha (1)                                 */
+    double alpha_max;            /* alphaMax (2)                                */
+    double alpha_min;            /* alphaMin (0.1)                              */
+    double epsilon_av;           /* epsilonAV (0.2)                             */
+    double alpha_ac;             /* alphaAC (1)                                 */
+    double gravity_constant;     /* G (1)                                       */
+    double theta;                /* theta (0.5)                                 */
+    double range_min[3];         /* rangeMin                                    */
+    double range_max[3];         /* rangeMax                                    */
+} sphgpu_params;
+
+/* ---- host field block layout: rows of n doubles ------------------------- */
+enum {
+    SPHGPU_F_POS = 0,       /* 0..2   pos                                     */
+    SPHGPU_F_VEL = 3,       /* 3..5   vel                                     */
+    SPHGPU_F_VEL_P = 6,     /* 6..8   vel_p                                   */
+    SPHGPU_F_ACC = 9,       /* 9..11  acc                                     */
+    SPHGPU_F_MASS = 12,
+    SPHGPU_F_DENS = 13,
+    SPHGPU_F_PRES = 14,
+    SPHGPU_F_ENE = 15,
+    SPHGPU_F_ENE_P = 16,
+    SPHGPU_F_DENE = 17,
+    SPHGPU_F_SML = 18,
+    SPHGPU_F_SOUND = 19,
+    SPHGPU_F_BALSARA = 20,
+    SPHGPU_F_ALPHA = 21,
+    SPHGPU_F_GRADH = 22,
+    SPHGPU_F_PHI = 23,
+    SPHGPU_F_ID = 24,       /* integer-valued                                 */
+    SPHGPU_F_NEIGHBOR = 25, /* integer-valued                                 */
+    SPHGPU_F_GRAD_D = 26,   /* 26..28 grad_density                            */
+    SPHGPU_F_GRAD_P = 29,   /* 29..31 grad_pressure                           */
+    SPHGPU_F_GRAD_V = 32,   /* 32..40 grad_velocity_k[axis] at 32 + 3k + axis */
+    SPHGPU_NFIELDS = 41
+};
+#define SPHGPU_MASK(f) (((uint64_t)1) << (f))
+#define SPHGPU_MASK_ALL ((((uint64_t)1) << SPHGPU_NFIELDS) - 1)
+
+typedef struct sphgpu_ctx sphgpu_ctx;
+
+/* Counters of the last module calls (device-side sums). */
+typedef struct sphgpu_stats {
+    int64_t pre_candidates;      /* sum over i of candidate-list length       */
+    int64_t pre_newton_iters;    /* sum over i of h Newton iterations         */
+    int64_t pre_newton_fail;     /* particles whose h iteration did not converge */
+    int64_t force_pairs;         /* accepted force pairs (directed)           */
+    int64_t force_candidates;    /* force candidate-list entries              */
+    int64_t riemann_iters;       /* sum of Newton iterations in the Riemann solver */
+    int64_t riemann_wave_max_iters; /* sum over wavefronts of max iterations  */
+    int64_t riemann_calls;       /* Riemann problems solved                   */
+    int64_t riemann_fallbacks;   /* learned guesses rejected for the plain one */
+    int64_t riemann_learned;     /* pairs that used the learned guess         */
+    int64_t gravity_interactions;/* particle-particle + particle-node terms    */
+    int64_t nodes;               /* tree nodes (root included)                */
+    int64_t leaves;
+    int32_t max_depth;           /* deepest node level (root = 1)             */
+    int32_t pad;
+} sphgpu_stats;
+
+/* ---- lifecycle ------------------------------------------------------------ */
+void        sphgpu_params_default(sphgpu_params *p, int dim);
+int         sphgpu_create(const sphgpu_params *p, int n, sphgpu_ctx **out);
+int         sphgpu_destroy(sphgpu_ctx *ctx);
+const char *sphgpu_last_error(const sphgpu_ctx *ctx);
+int         sphgpu_device_info(char *name, int name_len, int *cu_count, double *hbm_gb);
+
+/* ---- state exchange (fields: [SPHGPU_NFIELDS][n] FP64, index = id) ------- */
+int sphgpu_upload(sphgpu_ctx *ctx, const double *fields, uint64_t mask);
+int sphgpu_download(sphgpu_ctx *ctx, double *fields, uint64_t mask);
+int sphgpu_set_dt(sphgpu_ctx *ctx, double dt);
+int sphgpu_get_dt(sphgpu_ctx *ctx, double *dt);
+int sphgpu_set_time(sphgpu_ctx *ctx, double t);
+int sphgpu_get_time(sphgpu_ctx *ctx, double *t);
+int sphgpu_get_h_per_v_sig(sphgpu_ctx *ctx, double *h_per_v_sig);
+
+/* ---- modules (drop-ins for Module::calculation) -------------------------- */
+int sphgpu_build_tree(sphgpu_ctx *ctx);
+int sphgpu_pre_interaction(sphgpu_ctx *ctx);   /* includes set_kernel       */
+int sphgpu_fluid_force(sphgpu_ctx *ctx);
+int sphgpu_gravity(sphgpu_ctx *ctx);
+int sphgpu_timestep(sphgpu_ctx *ctx);
+int sphgpu_predict(sphgpu_ctx *ctx);
+int sphgpu_correct(sphgpu_ctx *ctx);
+
+/* Solver::initialize tail (src/solver.cpp:941-963): alpha, balsara, sound,
+ * tree, pre-interaction, fluid force, gravity at dt = 0. */
+int sphgpu_initialize(sphgpu_ctx *ctx);
+/* nsteps x Solver::integrate + Simulation::update_time (src/solver.cpp:966-978) */
+int sphgpu_integrate(sphgpu_ctx *ctx, int nsteps);
+/* out[0..6] = kinetic, thermal, potential, total, P_x, P_y, P_z (FP64) */
+int sphgpu_energy_momentum(sphgpu_ctx *ctx, double out[7]);
+int sphgpu_get_stats(sphgpu_ctx *ctx, sphgpu_stats *out);
+int sphgpu_reset_stats(sphgpu_ctx *ctx);
+
+/* ---- parity hooks -------------------------------------------------------- */
+/* BHTree::neighbor_search for every particle with caller-supplied smoothing
+ * lengths h[n] (index = id).  is_ij selects the FluidForce variant.  The sets
+ * are returned in id space, unsorted: counts[n], lists[n*cap].  For is_ij the
+ * set is the one the force loop accepts (r < max(h_i,h_j), r != 0). */
+int sphgpu_neighbor_sets(sphgpu_ctx *ctx, const double *h, int is_ij, int cap,
+                         int32_t *counts, int32_t *lists);
+/* Tree topology after sphgpu_build_tree: per node (pre-order) level, particle
+ * count, first-member offset into members[], is_leaf, centre[3], edge, mass,
+ * com[3], kernel_size.  members[] holds ids in key order.  Buffers sized by
+ * the caller from n_nodes (call with NULL buffers to query). */
+int sphgpu_tree_dump(sphgpu_ctx *ctx, int64_t *n_nodes, int32_t *level,
+                     int32_t *count, int32_t *first, int32_t *is_leaf,
+                     double *center, double *edge, double *mass, double *com,
+                     double *kernel_size, int32_t *members);
+/* Batched Riemann problems, states [u, rho, p, c] x n for left and right;
+ * solver = SPHGPU_RIEMANN_*; precision = SPHGPU_PAIR_*; iters may be NULL. */
+int sphgpu_riemann_batch(sphgpu_ctx *ctx, int solver, int precision, int n,
+                         const double *left, const double *right,
+                         double *pstar, double *vstar, int32_t *iters);
+
+/* ---- learned Riemann starting guess ------------------------------------- */
+/* Pair-state log written by the GSPH force kernel: rows of 7 floats
+ * (u, rho, p)_L, (u, rho, p)_R, converged p*.  One row in `stride` pairs is
+ * kept, up to `capacity` rows (ring buffer). */
+int sphgpu_riemann_log_enable(sphgpu_ctx *ctx, int64_t capacity, int stride);
+int sphgpu_riemann_log_read(sphgpu_ctx *ctx, float *rows, int64_t max_rows,
+                            int64_t *n_rows);
+/* MLP 6 -> H -> H -> 1 (H = 64); parameters FP32 master copy, bf16 compute. */
+int sphgpu_mlp_init(sphgpu_ctx *ctx, uint64_t seed);
+int sphgpu_mlp_set_params(sphgpu_ctx *ctx, const float *params, int64_t n);
+int sphgpu_mlp_get_params(sphgpu_ctx *ctx, float *params, int64_t n);
+int64_t sphgpu_mlp_param_count(void);
+/* One Adam step on a batch of logged rows (device-resident after log_read or
+ * given here as host rows); returns the batch loss. */
+int sphgpu_mlp_train(sphgpu_ctx *ctx, const float *rows, int64_t n_rows,
+                     int epochs, int batch, float lr, float *final_loss);
+/* Forward and backward of the mean-squared loss on host rows, for gradient
+ * checks: loss and dL/dparams (FP32). */
+int sphgpu_mlp_loss_grad(sphgpu_ctx *ctx, const float *rows, int64_t n_rows,
+                         float *loss, float *grad);
+int sphgpu_mlp_predict(sphgpu_ctx *ctx, const float *rows, int64_t n_rows,
+                       float *pstar_guess);
+
+/* ---- timing ------------------------------------------------------------- */
+/* Per-kernel averages (ms) of launches since the last reset, measured with
+ * HIP events on the context's stream.  name_buf receives "name;name;...". */
+int sphgpu_kernel_times(sphgpu_ctx *ctx, char *name_buf, int buf_len,
+                        double *avg_ms, int64_t *count, int max_kernels,
+                        int *n_kernels);
+int sphgpu_enable_timing(sphgpu_ctx *ctx, int on);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SPHGPU_H */

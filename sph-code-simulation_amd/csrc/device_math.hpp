@@ -1,0 +1,288 @@
+// device_math.hpp -- per-pair arithmetic of the SPH hot path, for gfx950.
+//
+// Everything here is templated on the pair precision T (float or double) and
+// the dimension D.  With T = double the expressions keep the association of
+// the reference code they restate, so the FP64 form differs from the
+// reference only through summation order; with T = float the same formulas
+// run at FP32 rate.  All HIP sources are compiled with -ffp-contract=off;
+// functions that may use FMA say so with a local pragma.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/sphgpu.h"
+
+#define SPH_DEV __device__ __forceinline__
+
+namespace sphgpu {
+
+constexpr int kWave = 64;
+
+template <typename T> SPH_DEV T tabs(T x) { return x < T(0) ? -x : x; }
+SPH_DEV float tsqrt(float x) { return __builtin_sqrtf(x); }
+SPH_DEV double tsqrt(double x) { return __builtin_sqrt(x); }
+SPH_DEV float texp(float x) { return __expf(x); }
+SPH_DEV double texp(double x) { return exp(x); }
+
+// Periodic::calc_r_ij per axis (include/periodic.hpp:34-59): choose among
+// d, d + L, d - L the one of smallest magnitude, ties in the order (d, d+L, d-L).
+template <typename T> SPH_DEV T min_image(T d, T L)
+{
+    const T r1 = d, r2 = d + L, r3 = d - L;
+    const T a1 = tabs(r1), a2 = tabs(r2), a3 = tabs(r3);
+    if (a1 <= a2 && a1 <= a3) return r1;
+    if (a2 <= a3 && a2 <= a1) return r2;
+    if (a3 <= a1 && a3 <= a2) return r3;
+    return r1;
+}
+
+// Box geometry shared by all kernels (FP64 master copy plus an FP32 copy).
+struct Box {
+    int periodic;
+    double range[3];
+    double lo[3];
+    double hi[3];
+    float rangef[3];
+};
+
+template <typename T> SPH_DEV T box_range(const Box &b, int k);
+template <> SPH_DEV double box_range<double>(const Box &b, int k) { return b.range[k]; }
+template <> SPH_DEV float box_range<float>(const Box &b, int k) { return b.rangef[k]; }
+
+// r_i - r_j with the reference's minimum-image choice.
+template <typename T> SPH_DEV T rij_axis(T xi, T xj, const Box &b, int k)
+{
+    const T d = xi - xj;
+    return b.periodic ? min_image(d, box_range<T>(b, k)) : d;
+}
+
+// include/defines.hpp:28-32
+template <typename T> SPH_DEV T sqr(T x) { return x * x; }
+template <typename T> SPH_DEV T pow3(T x) { return x * x * x; }
+template <typename T> SPH_DEV T pow5(T x) { return x * x * x * x * x; }
+template <typename T> SPH_DEV T pow6(T x) { return x * x * x * x * x * x; }
+
+// include/kernel/kernel_function.hpp:8-16 and pre_interaction.cpp:217-225
+template <int D, typename T> SPH_DEV T powh(T h) { return D == 1 ? h : D == 2 ? h * h : h * h * h; }
+template <int D, typename T> SPH_DEV T powh_m1(T h) { return D == 1 ? T(1) : D == 2 ? h : h * h; }
+
+template <int D> struct Sigma;
+template <> struct Sigma<1> {
+    static constexpr double cubic = 2.0 / 3.0, c4 = 0.0, gauss = 0.564189583547756;
+};
+template <> struct Sigma<2> {
+    static constexpr double cubic = 10.0 / (7.0 * 3.1415926535897932384626433832795028841971693993751);
+    static constexpr double c4 = 9.0 / 3.1415926535897932384626433832795028841971693993751;
+    static constexpr double gauss = 1.0 / 3.1415926535897932384626433832795028841971693993751;
+};
+template <> struct Sigma<3> {
+    static constexpr double cubic = 1.0 / 3.1415926535897932384626433832795028841971693993751;
+    static constexpr double c4 = 495. / (32 * 3.1415926535897932384626433832795028841971693993751);
+    static constexpr double gauss = 0.179587122125167;
+};
+
+// SPH kernels, support r < h.  w(r,h); dwc(r,h) with grad W = r_ij * dwc;
+// dhw(r,h) = dW/dh.  Cubic spline: include/kernel/cubic_spline.hpp:27-52.
+// Wendland C4: include/kernel/wendland_kernel.hpp:30-48.  Gaussian:
+// include/kernel/gaussian_kernel.hpp:27-51.
+template <int D, int K, typename T> struct Kern;
+
+template <int D, typename T> struct Kern<D, SPHGPU_CUBIC_SPLINE, T> {
+    static SPH_DEV T w(T r, T h)
+    {
+        const T sig = T(Sigma<D>::cubic);
+        const T h_ = h * T(0.5);
+        const T q = r / h_;
+        return sig / powh<D>(h_) * (T(0.25) * pow3(T(0.5) * (T(2.0) - q + tabs(T(2.0) - q))) -
+                                    pow3(T(0.5) * (T(1.0) - q + tabs(T(1.0) - q))));
+    }
+    static SPH_DEV T dwc(T r, T h)
+    {
+        if (r == T(0)) return T(0);
+        const T sig = T(Sigma<D>::cubic);
+        const T h_ = h * T(0.5);
+        const T q = r / h_;
+        return -sig / (powh<D>(h_) * h_ * r) *
+               (T(0.75) * sqr(T(0.5) * (T(2.0) - q + tabs(T(2.0) - q))) - T(3.0) * sqr(T(0.5) * (T(1.0) - q + tabs(T(1.0) - q))));
+    }
+    static SPH_DEV T dhw(T r, T h)
+    {
+        const T sig = T(Sigma<D>::cubic);
+        const T h_ = h * T(0.5);
+        const T q = r / h_;
+        return T(0.5) * sig / (powh<D>(h_) * h_) *
+               (sqr((tabs(T(2.0) - q) + T(2.0) - q) * T(0.5)) * (T((3. + D) * 0.25) * q - T(0.5 * D)) +
+                sqr((tabs(T(1.0) - q) + T(1.0) - q) * T(0.5)) * (T(-3.0 - D) * q + T(D)));
+    }
+};
+
+template <int D, typename T> struct Kern<D, SPHGPU_WENDLAND, T> {
+    static SPH_DEV T w(T r, T h)
+    {
+        const T sig = T(Sigma<D>::c4);
+        const T q = r / h;
+        return sig / powh<D>(h) * pow6(T(0.5) * (T(1.0) - q + tabs(T(1.0) - q))) * (T(1.0) + T(6.0) * q + T(35.0 / 3.0) * q * q);
+    }
+    static SPH_DEV T dwc(T r, T h)
+    {
+        const T sig = T(Sigma<D>::c4);
+        const T q = r / h;
+        return T(-56.0 / 3.0) * sig / (powh<D>(h) * sqr(h)) * pow5(T(0.5) * (T(1.0) - q + tabs(T(1.0) - q))) * (T(1.0) + T(5.0) * q);
+    }
+    static SPH_DEV T dhw(T r, T h)
+    {
+        const T sig = T(Sigma<D>::c4);
+        const T q = r / h;
+        return -sig / (powh<D>(h) * h * T(3.0)) * pow5(T(0.5) * (T(1.0) - q + tabs(T(1.0) - q))) *
+               (T(3.0 * D) + T(15.0 * D) * q + T(-56.0 + 17.0 * D) * q * q - T(35.0 * (8.0 + D)) * pow3(q));
+    }
+};
+
+template <int D, typename T> struct Kern<D, SPHGPU_GAUSSIAN, T> {
+    static SPH_DEV T w(T r, T h)
+    {
+        const T q = r / h;
+        return T(Sigma<D>::gauss) / powh<D>(h) * texp(-(q * q));
+    }
+    static SPH_DEV T dwc(T r, T h)
+    {
+        if (r == T(0)) return T(0);
+        const T q = r / h;
+        const T w_val = T(Sigma<D>::gauss) / powh<D>(h) * texp(-(q * q));
+        return T(-2.0) * w_val * q / (h * r);
+    }
+    static SPH_DEV T dhw(T r, T h)
+    {
+        const T q = r / h;
+        const T q2 = q * q;
+        const T w_val = T(Sigma<D>::gauss) / powh<D>(h) * texp(-q2);
+        return w_val * (T(2.0) * q2 - T(D)) / h;
+    }
+};
+
+// van Leer limiter, src/gsph/g_fluid_force.cpp:33-41
+template <typename T> SPH_DEV T limiter(T dq1, T dq2)
+{
+    const T p = dq1 * dq2;
+    if (p <= T(0)) return T(0);
+    return T(2.0) * p / (dq1 + dq2);
+}
+
+// HLL solver, src/gsph/g_fluid_force.cpp:173-204.  State = (u, rho, p, c).
+template <typename T> SPH_DEV void hll_solver(const T l[4], const T r[4], T &pstar, T &vstar)
+{
+    const T u_l = l[0], rho_l = l[1], p_l = l[2], c_l = l[3];
+    const T u_r = r[0], rho_r = r[1], p_r = r[2], c_r = r[3];
+    const T roe_l = tsqrt(rho_l);
+    const T roe_r = tsqrt(rho_r);
+    const T roe_inv = T(1.0) / (roe_l + roe_r);
+    const T u_t = (roe_l * u_l + roe_r * u_r) * roe_inv;
+    const T c_t = (roe_l * c_l + roe_r * c_r) * roe_inv;
+    const T a = u_l - c_l, b = u_t - c_t;
+    const T s_l = (b < a) ? b : a;      // std::min
+    const T c = u_r + c_r, d = u_t + c_t;
+    const T s_r = (c < d) ? d : c;      // std::max
+    const T c1 = rho_l * (s_l - u_l);
+    const T c2 = rho_r * (s_r - u_r);
+    const T c3 = T(1.0) / (c1 - c2);
+    const T c4 = p_l - u_l * c1;
+    const T c5 = p_r - u_r * c2;
+    vstar = (c5 - c4) * c3;
+    pstar = (c1 * c5 - c2 * c4) * c3;
+}
+
+// Two-shock Newton iteration of src/gsph/g_fluid_force.cpp:209-304, split so
+// the starting guess can be supplied (the learned guess replaces :244-246).
+template <typename T> struct TwoShock {
+    T u_l, p_l, u_r, p_r, V_l, V_r, cl, cr, gamma1, gamma2;
+    bool bad;
+    SPH_DEV void setup(T gamma, const T l[4], const T r[4])
+    {
+        u_l = l[0]; p_l = l[2]; u_r = r[0]; p_r = r[2];
+        const T rho_l = l[1], rho_r = r[1];
+        bad = (rho_l < T(0) || rho_r < T(0) || p_l < T(0) || p_r < T(0));
+        gamma2 = T(1.0) + gamma;
+        gamma1 = T(0.5) * gamma2 / gamma;
+        V_l = T(1.0) / rho_l;
+        V_r = T(1.0) / rho_r;
+        cl = tsqrt(gamma * p_l * rho_l);
+        cr = tsqrt(gamma * p_r * rho_r);
+    }
+    // the reference's linearised impedance guess, g_fluid_force.cpp:244-246
+    SPH_DEV T plain_guess() const
+    {
+        T g = p_r - p_l - cr * (u_r - u_l);
+        g = p_l + g * cl / (cl + cr);
+        return (g < T(1e-25)) ? T(1e-25) : g;  // std::max(g, smallp)
+    }
+    // one Newton step, g_fluid_force.cpp:257-279
+    SPH_DEV T step(T p) const
+    {
+        T w_l = T(1.0) + gamma1 * (p - p_l) / p_l;
+        w_l = cl * tsqrt(w_l);
+        T w_r = T(1.0) + gamma1 * (p - p_r) / p_r;
+        w_r = cr * tsqrt(w_r);
+        T z_l = T(4.0) * V_l * w_l * w_l;
+        z_l = -z_l * w_l / (z_l - gamma2 * (p - p_l));
+        T z_r = T(4.0) * V_r * w_r * w_r;
+        z_r = z_r * w_r / (z_r - gamma2 * (p - p_r));
+        const T ustar_l = u_l - (p - p_l) / w_l;
+        const T ustar_r = u_r + (p - p_r) / w_r;
+        T pn = p + (ustar_r - ustar_l) * (z_l * z_r) / (z_r - z_l);
+        return (T(1e-25) < pn) ? pn : T(1e-25);  // std::max(smallp, p)
+    }
+    // g_fluid_force.cpp:254-286; returns the iteration count
+    SPH_DEV int iterate(T &p) const
+    {
+        int it = 0;
+        for (int iter = 0; iter < 20; ++iter) {
+            const T p_old = p;
+            p = step(p);
+            it = iter + 1;
+            if (tabs(p - p_old) / p < T(1e-6)) break;
+        }
+        return it;
+    }
+    // g_fluid_force.cpp:288-301
+    SPH_DEV T vstar(T p) const
+    {
+        T w_l = T(1.0) + gamma1 * (p - p_l) / p_l;
+        w_l = cl * tsqrt(w_l);
+        T w_r = T(1.0) + gamma1 * (p - p_r) / p_r;
+        w_r = cr * tsqrt(w_r);
+        const T ustar_l = u_l - (p - p_l) / w_l;
+        const T ustar_r = u_r + (p - p_r) / w_r;
+        return T(0.5) * (ustar_l + ustar_r);
+    }
+};
+
+// Exact inequality of the reference's FP64 neighbour predicates.
+// sqrt(r2) < h, with the correctly rounded FP64 sqrt.
+SPH_DEV bool r_less(double r2, double h) { return __builtin_sqrt(r2) < h; }
+
+// Neumaier-compensated accumulator for FP32 sums (plain sum for FP64).
+template <typename T> struct Acc {
+    T s, c;
+    SPH_DEV Acc() : s(T(0)), c(T(0)) {}
+    SPH_DEV void add(T x)
+    {
+        const T t = s + x;
+        if (tabs(s) >= tabs(x)) c += (s - t) + x;
+        else c += (x - t) + s;
+        s = t;
+    }
+    SPH_DEV T get() const { return s + c; }
+};
+template <> struct Acc<double> {
+    double s;
+    SPH_DEV Acc() : s(0.0) {}
+    SPH_DEV void add(double x) { s += x; }
+    SPH_DEV double get() const { return s; }
+};
+
+// Order-preserving integer image of a non-negative double, for atomicMin/Max.
+SPH_DEV unsigned long long dbits(double x) { return (unsigned long long)__double_as_longlong(x); }
+SPH_DEV double bitsd(unsigned long long b) { return __longlong_as_double((long long)b); }
+
+} // namespace sphgpu

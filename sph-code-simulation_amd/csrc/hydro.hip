@@ -1,0 +1,969 @@
+// hydro.hip -- PreInteraction and FluidForce pair loops for SSPH, DISPH, GSPH.
+//
+// One thread per particle i walks its neighbour list (built by neighbor.hip
+// with the reference's exact FP64 predicates) and reads neighbour data from
+// key-ordered SoA arrays.  Pair arithmetic runs in T: float with
+// Neumaier-compensated sums for densities and forces, or double with the
+// reference's expressions.  Set membership (r < h) is always decided in FP64.
+//
+// References:
+//   PreInteraction (SSPH)   src/pre_interaction.cpp:39-283
+//   PreInteraction (DISPH)  src/disph/d_pre_interaction.cpp:21-230
+//   PreInteraction (GSPH)   src/gsph/g_pre_interaction.cpp:27-144
+//   FluidForce (SSPH)       src/fluid_force.cpp:26-116
+//   FluidForce (DISPH)      src/disph/d_fluid_force.cpp:26-86
+//   FluidForce (GSPH)       src/gsph/g_fluid_force.cpp:44-304
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <vector>
+
+#include "ctx.hpp"
+#include "mlp_infer.hpp"
+
+namespace sphgpu {
+
+namespace {
+
+constexpr double kPi = 3.1415926535897932384626433832795028841971693993751;
+template <int D> constexpr double area_const() { return D == 1 ? 2.0 : D == 2 ? kPi : 4.0 * kPi / 3.0; }
+
+template <typename T> struct PreArgs {
+    int n;
+    int *idx;                     // compacted in place: members first
+    const int *cnt;
+    PairView<T> pv;
+    const double *x64[3];
+    const double *hs;
+    const double *mass64, *ene64, *sound64, *dens64;
+    double *sml, *dens, *pres, *gradh, *balsara, *alpha;
+    int *neighbor;
+    double *grad[kNGrad];
+    float *mf[M_NFIELD];          // FP32 mirrors to refresh (FP32 mode), else null
+    unsigned long long *hvs_min;
+    const double *dscal;
+    long long *stats;
+    Box box;
+    int nn, iterative, balsara_on, td_av, gsph2;
+    double gamma, kappa, alpha_min, alpha_max, eps_av;
+};
+
+template <int D, typename T>
+SPH_DEV T pair_r2(const PairView<T> &pv, const T xi[3], int j, const Box &b, T d[3])
+{
+    T s = T(0);
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        d[k] = rij_axis<T>(xi[k], pv.x[k][j], b, k);
+        s = (k == 0) ? d[k] * d[k] : s + d[k] * d[k];
+    }
+    return s;
+}
+
+SPH_DEV double r2_64(const double *const x[3], const double xi[3], int j, const Box &b, int D)
+{
+    double s = 0.0;
+    for (int k = 0; k < D; ++k) {
+        const double d = rij_axis<double>(xi[k], x[k][j], b, k);
+        s = (k == 0) ? d * d : s + d * d;
+    }
+    return s;
+}
+
+SPH_DEV void wave_min_atomic(unsigned long long *dst, double v)
+{
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+        const double o = __shfl_xor(v, off, kWave);
+        v = o < v ? o : v;
+    }
+    if ((threadIdx.x & (kWave - 1)) == 0) atomicMin(dst, dbits(v));
+}
+
+SPH_DEV void wave_add_atomic(long long *dst, long long v)
+{
+    for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+    if ((threadIdx.x & (kWave - 1)) == 0) atomicAdd((unsigned long long *)dst, (unsigned long long)v);
+}
+
+template <int D, int K, int TYPE, typename T>
+__global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
+{
+    const int i0 = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = i0 < a.n;
+    const int i = valid ? i0 : a.n - 1;
+    const PairView<T> &pv = a.pv;
+    const int nc = valid ? a.cnt[i] : 0;
+    T xi[3] = {0, 0, 0}, vi[3] = {0, 0, 0};
+    double xi64[3] = {0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        xi[k] = pv.x[k][i];
+        vi[k] = pv.v[k][i];
+        xi64[k] = a.x64[k][i];
+    }
+    const T mi = pv.mass[i], ui = pv.ene[i], ci = pv.sound[i];
+    const double hs64 = a.hs[i];
+    const double A = area_const<D>();
+    long long newton_iters = 0, newton_fail = 0;
+
+    // ---- smoothing length: newton_raphson (pre_interaction.cpp:227-283) ----
+    T h;
+    if (a.iterative) {
+        h = T(hs64 / a.kappa);
+        const T b = (TYPE == SPHGPU_DISPH) ? T(a.nn / A) : T(a.mass64[i] * a.nn / A);
+        bool conv = false;
+        int it = 0;
+        for (; it < 10;) {
+            const T hb = h;
+            Acc<T> dn, ddn;
+            for (int q = 0; q < nc; ++q) {
+                const int j = a.idx[(size_t)q * a.n + i];
+                T d[3];
+                const T r = tsqrt(pair_r2<D, T>(pv, xi, j, a.box, d));
+                if (r >= h) continue;
+                const T wt = (TYPE == SPHGPU_DISPH) ? T(1) : pv.mass[j];
+                dn.add(wt * Kern<D, K, T>::w(r, h));
+                ddn.add(wt * Kern<D, K, T>::dhw(r, h));
+            }
+            const T dens = dn.get(), ddens = ddn.get();
+            const T f = dens * powh<D>(h) - b;
+            const T df = ddens * powh<D>(h) + T(D) * dens * powh_m1<D>(h);
+            h -= f / df;
+            ++it;
+            if (tabs(h - hb) < (h + hb) * T(1e-4)) { conv = true; break; }
+        }
+        newton_iters = it;
+        if (!conv) {
+            h = T(hs64 / a.kappa);
+            newton_fail = 1;
+        }
+    } else {
+        h = T(hs64);
+    }
+    const double h64 = (double)h;
+
+    // ---- density, pressure, grad-h, v_sig (members r < h, decided in FP64) ----
+    Acc<T> dens_a, dhd_a, n_a, pres_a, dhp_a, dhn_a;
+    T vsig = ci * T(2.0);
+    int w = 0;
+    for (int q = 0; q < nc; ++q) {
+        const int j = a.idx[(size_t)q * a.n + i];
+        T d[3];
+        const T r2 = pair_r2<D, T>(pv, xi, j, a.box, d);
+        bool in;
+        if (sizeof(T) == 8) in = tsqrt(r2) < h;
+        else in = r_less(r2_64(a.x64, xi64, j, a.box, D), h64);
+        if (!in) continue;
+        a.idx[(size_t)w * a.n + i] = j;  // compact: members first
+        ++w;
+        const T r = tsqrt(r2);
+        const T wk = Kern<D, K, T>::w(r, h);
+        const T mj = pv.mass[j];
+        if (TYPE == SPHGPU_DISPH) {
+            const T dh = Kern<D, K, T>::dhw(r, h);
+            const T uj = pv.ene[j];
+            dens_a.add(mj * wk);
+            n_a.add(wk);
+            pres_a.add(mj * uj * wk);
+            dhp_a.add(mj * uj * dh);
+            dhn_a.add(dh);
+        } else if (TYPE == SPHGPU_SSPH) {
+            dens_a.add(mj * wk);
+            dhd_a.add(mj * Kern<D, K, T>::dhw(r, h));
+        } else {
+            dens_a.add(mj * wk);
+        }
+        if (j != i) {
+            T dot = T(0);
+#pragma unroll
+            for (int k = 0; k < D; ++k) dot = (k == 0) ? d[k] * (vi[k] - pv.v[k][j]) : dot + d[k] * (vi[k] - pv.v[k][j]);
+            const T vs = ci + pv.sound[j] - T(3.0) * dot / r;
+            if (vs > vsig) vsig = vs;
+        }
+    }
+    const T dens = dens_a.get();
+    const T gm1 = T(a.gamma - 1.0);
+    T pres, gradh = T(0);
+    if (TYPE == SPHGPU_DISPH) {
+        const T ni = n_a.get();
+        pres = gm1 * pres_a.get();
+        gradh = h / (T(D) * ni) * dhp_a.get() / (T(1.0) + h / (T(D) * ni) * dhn_a.get());
+    } else if (TYPE == SPHGPU_SSPH) {
+        pres = gm1 * dens * ui;
+        gradh = T(1.0) / (T(1.0) + h / (T(D) * dens) * dhd_a.get());
+    } else {
+        pres = gm1 * dens * ui;
+    }
+    if (valid) {
+        a.sml[i] = h64;
+        a.dens[i] = (double)dens;
+        a.pres[i] = (double)pres;
+        if (TYPE != SPHGPU_GSPH) a.gradh[i] = (double)gradh;
+        a.neighbor[i] = w;
+        if (a.mf[M_SML]) {
+            a.mf[M_SML][i] = (float)h;
+            a.mf[M_DENS][i] = (float)dens;
+            a.mf[M_PRES][i] = (float)pres;
+            if (TYPE != SPHGPU_GSPH) a.mf[M_GRADH][i] = (float)gradh;
+        }
+    }
+    wave_min_atomic(a.hvs_min, valid ? (double)(h / vsig) : 1.7976931348623157e308);
+    wave_add_atomic(&a.stats[1], valid ? newton_iters : 0);
+    wave_add_atomic(&a.stats[2], valid ? newton_fail : 0);
+    if (!valid) return;
+
+    if (TYPE == SPHGPU_GSPH) {
+        // MUSCL gradients, g_pre_interaction.cpp:112-136
+        if (!a.gsph2) return;
+        T dd[3] = {0, 0, 0}, du[3] = {0, 0, 0}, dv[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+        for (int q = 0; q < w; ++q) {
+            const int j = a.idx[(size_t)q * a.n + i];
+            T d[3];
+            const T r = tsqrt(pair_r2<D, T>(pv, xi, j, a.box, d));
+            const T cw = Kern<D, K, T>::dwc(r, h);
+            const T mj = pv.mass[j];
+            const T mdu = mj * (pv.ene[j] - ui);
+            T mdv[3];
+#pragma unroll
+            for (int b2 = 0; b2 < D; ++b2) mdv[b2] = mj * (pv.v[b2][j] - vi[b2]);
+#pragma unroll
+            for (int k = 0; k < D; ++k) {
+                const T dwk = d[k] * cw;
+                dd[k] += dwk * mj;
+                du[k] += dwk * mdu;
+#pragma unroll
+                for (int b2 = 0; b2 < D; ++b2) dv[b2][k] += dwk * mdv[b2];
+            }
+        }
+        const T rho_inv = T(1.0) / dens;
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            const T gp = (dd[k] * ui + du[k]) * gm1;
+            a.grad[k][i] = (double)dd[k];
+            a.grad[3 + k][i] = (double)gp;
+            if (a.mf[M_SML]) {
+                a.mf[M_GD0 + k][i] = (float)dd[k];
+                a.mf[M_GP0 + k][i] = (float)gp;
+            }
+#pragma unroll
+            for (int b2 = 0; b2 < D; ++b2) {
+                const T gv = dv[b2][k] * rho_inv;
+                a.grad[6 + 3 * b2 + k][i] = (double)gv;
+                if (a.mf[M_SML]) a.mf[M_GV0 + 3 * b2 + k][i] = (float)gv;
+            }
+        }
+        return;
+    }
+
+    // ---- Balsara switch and time-dependent AV (pre_interaction.cpp:115-161) ----
+    const T dt = T(a.dscal[0]);
+    T alpha = pv.alpha[i];
+    const bool do_bal = a.balsara_on && D != 1;
+    if (do_bal || a.td_av) {
+        T div = T(0), rot[3] = {0, 0, 0};
+        for (int q = 0; q < w; ++q) {
+            const int j = a.idx[(size_t)q * a.n + i];
+            T d[3];
+            const T r = tsqrt(pair_r2<D, T>(pv, xi, j, a.box, d));
+            const T cw = Kern<D, K, T>::dwc(r, h);
+            T dw[3], vij[3];
+#pragma unroll
+            for (int k = 0; k < D; ++k) {
+                dw[k] = d[k] * cw;
+                vij[k] = vi[k] - pv.v[k][j];
+            }
+            const T wt = (TYPE == SPHGPU_DISPH) ? pv.mass[j] * pv.ene[j] : pv.mass[j];
+            T dot = T(0);
+#pragma unroll
+            for (int k = 0; k < D; ++k) dot = (k == 0) ? vij[k] * dw[k] : dot + vij[k] * dw[k];
+            div -= wt * dot;
+            if (D == 3) {
+                rot[0] += (vij[1] * dw[2 % D] - vij[2 % D] * dw[1 % D]) * wt;
+                rot[1] += (vij[2 % D] * dw[0] - vij[0] * dw[2 % D]) * wt;
+                rot[2] += (vij[0] * dw[1 % D] - vij[1 % D] * dw[0]) * wt;
+            } else if (D == 2) {
+                rot[0] += (vij[0] * dw[1 % D] - vij[1 % D] * dw[0]) * wt;
+            }
+        }
+        T rot_abs;
+        if (TYPE == SPHGPU_DISPH) {
+            const T p_inv = gm1 / pres;
+            div *= p_inv;
+            for (int k = 0; k < 3; ++k) rot[k] *= p_inv;
+        } else {
+            div /= dens;
+            for (int k = 0; k < 3; ++k) rot[k] /= dens;
+        }
+        if (D == 3) rot_abs = tsqrt(rot[0] * rot[0] + rot[1] * rot[1] + rot[2] * rot[2]);
+        else rot_abs = tabs(rot[0]);
+        const T tau_inv = T(a.eps_av) * ci / h;
+        if (do_bal) {
+            const T bal = tabs(div) / (tabs(div) + rot_abs + T(1e-4) * ci / h);
+            a.balsara[i] = (double)bal;
+            if (a.mf[M_SML]) a.mf[M_BALSARA][i] = (float)bal;
+            if (a.td_av) {
+                const T mdiv = -div > T(0) ? -div : T(0);
+                const T dalpha = (-(alpha - T(a.alpha_min)) * tau_inv + mdiv * (T(a.alpha_max) - alpha)) * dt;
+                alpha += dalpha;
+            }
+        } else {
+            const T s_i = -div > T(0) ? -div : T(0);
+            alpha = (alpha + dt * tau_inv * T(a.alpha_min) + s_i * dt * T(a.alpha_max)) /
+                    (T(1.0) + dt * tau_inv + s_i * dt);
+        }
+        if (a.td_av) {
+            a.alpha[i] = (double)alpha;
+            if (a.mf[M_SML]) a.mf[M_ALPHA][i] = (float)alpha;
+        }
+    }
+}
+
+// ---- initial_smoothing (pre_interaction.cpp:171-215) ----------------------------
+template <int D, int K>
+__global__ __launch_bounds__(256) void k_initial_density(int n, const int *__restrict__ idx, const int *__restrict__ cnt,
+                                                         const double *const *x, const double *__restrict__ mass,
+                                                         const double *__restrict__ sml, double *__restrict__ dens,
+                                                         Box box)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double xi[3] = {0, 0, 0};
+    for (int k = 0; k < D; ++k) xi[k] = x[k][i];
+    const double h = sml[i];
+    double s = 0.0;
+    const int nc = cnt[i];
+    for (int q = 0; q < nc; ++q) {
+        const int j = idx[(size_t)q * n + i];
+        const double r = __builtin_sqrt(r2_64(x, xi, j, box, D));
+        if (r >= h) continue;
+        s += mass[j] * Kern<D, K, double>::w(r, h);
+    }
+    dens[i] = s;
+}
+
+// smoothing-length guess (pre_interaction.cpp:61-64, 186-189)
+template <int D>
+__global__ void k_hguess(int n, const double *__restrict__ mass, const double *__restrict__ dens, int nn,
+                         double kappa, double *__restrict__ hs)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double A = area_const<D>();
+    hs[i] = pow(nn * mass[i] / (dens[i] * A), 1.0 / D) * kappa;
+}
+
+__global__ void k_minbits_init(unsigned long long *m)
+{
+    m[threadIdx.x] = 0x7fefffffffffffffull;  // DBL_MAX
+}
+
+__global__ void k_store_hvs(const unsigned long long *m, double *dscal) { dscal[2] = bitsd(m[0]); }
+
+// ---- FluidForce --------------------------------------------------------------------
+template <typename T> struct ForceArgs {
+    int n;
+    const int *idx, *cnt;
+    PairView<T> pv;
+    double *acc[3];
+    double *dene;
+    const double *dscal;
+    long long *stats;
+    Box box;
+    double gamma, alpha_ac;
+    int use_ac, use_gravity, gsph2, refine64, guess;
+    MLPDev mlp;
+    float *rlog;
+    long long rlog_cap;
+    int rlog_stride;
+    unsigned long long *rlog_count;
+};
+
+template <int D, typename T>
+SPH_DEV T av_pi(const T vij[3], const T d[3], T r, T ai, T aj, T bi, T bj, T ci, T cj, T rhoi, T rhoj)
+{
+    // FluidForce::artificial_viscosity, src/fluid_force.cpp:89-106
+    T vr = T(0);
+#pragma unroll
+    for (int k = 0; k < D; ++k) vr = (k == 0) ? vij[k] * d[k] : vr + vij[k] * d[k];
+    if (vr < T(0)) {
+        const T alpha = T(0.5) * (ai + aj);
+        const T balsara = T(0.5) * (bi + bj);
+        const T w_ij = vr / r;
+        const T v_sig = ci + cj - T(3.0) * w_ij;
+        const T rho_ij_inv = T(2.0) / (rhoi + rhoj);
+        return T(-0.5) * balsara * alpha * v_sig * w_ij * rho_ij_inv;
+    }
+    return T(0);
+}
+
+template <int D, int K, int TYPE, typename T>
+__global__ __launch_bounds__(256) void k_force_sph(ForceArgs<T> a)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    const PairView<T> &pv = a.pv;
+    T xi[3] = {0, 0, 0}, vi[3] = {0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < D; ++k) { xi[k] = pv.x[k][i]; vi[k] = pv.v[k][i]; }
+    const T hi = pv.sml[i], rhoi = pv.dens[i], presi = pv.pres[i], gradhi = pv.gradh[i];
+    const T ci = pv.sound[i], ai = pv.alpha[i], bi = pv.balsara[i], mi = pv.mass[i], ui = pv.ene[i];
+    const T gm1 = T(a.gamma - 1.0);
+    const T p_per_rho2_i = presi / sqr(rhoi);
+    const T g2u_i = sqr(gm1) * ui;
+    const T g2upp_i = g2u_i / presi;
+    const T m_u_inv = T(1.0) / (mi * ui);
+    Acc<T> ac[3], de;
+    const int nc = a.cnt[i];
+    for (int q = 0; q < nc; ++q) {
+        const int j = a.idx[(size_t)q * a.n + i];
+        T d[3];
+        const T r = tsqrt(pair_r2<D, T>(pv, xi, j, a.box, d));
+        const T hj = pv.sml[j], rhoj = pv.dens[j], presj = pv.pres[j], mj = pv.mass[j], uj = pv.ene[j];
+        const T cwi = Kern<D, K, T>::dwc(r, hi);
+        const T cwj = Kern<D, K, T>::dwc(r, hj);
+        T dwi[3], dwj[3], dwij[3], vij[3];
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            dwi[k] = d[k] * cwi;
+            dwj[k] = d[k] * cwj;
+            dwij[k] = (dwi[k] + dwj[k]) * T(0.5);
+            vij[k] = vi[k] - pv.v[k][j];
+        }
+        const T pi_ij = av_pi<D, T>(vij, d, r, ai, pv.alpha[j], bi, pv.balsara[j], ci, pv.sound[j], rhoi, rhoj);
+        T dene_ac = T(0);
+        if (a.use_ac) {
+            // FluidForce::artificial_conductivity, src/fluid_force.cpp:108-116
+            T vsig;
+            if (a.use_gravity) {
+                T vr = T(0);
+#pragma unroll
+                for (int k = 0; k < D; ++k) vr = (k == 0) ? vij[k] * d[k] : vr + vij[k] * d[k];
+                vsig = tabs(vr / r);
+            } else {
+                vsig = tsqrt(T(2.0) * tabs(presi - presj) / (rhoi + rhoj));
+            }
+            T dr = T(0);
+#pragma unroll
+            for (int k = 0; k < D; ++k) dr = (k == 0) ? dwij[k] * d[k] : dr + dwij[k] * d[k];
+            dene_ac = T(a.alpha_ac) * mj * vsig * (ui - uj) * dr / r;
+        }
+        T s1, s2, e1;
+        T vdwi = T(0), vdwij = T(0);
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            vdwi = (k == 0) ? vij[k] * dwi[k] : vdwi + vij[k] * dwi[k];
+            vdwij = (k == 0) ? vij[k] * dwij[k] : vdwij + vij[k] * dwij[k];
+        }
+        if (TYPE == SPHGPU_SSPH) {
+            s1 = mj * (p_per_rho2_i * gradhi + T(0.5) * pi_ij);
+            s2 = mj * (presj / sqr(rhoj) * pv.gradh[j] + T(0.5) * pi_ij);
+            e1 = mj * p_per_rho2_i * gradhi * vdwi + T(0.5) * mj * pi_ij * vdwij + dene_ac;
+        } else {
+            const T f_ij = T(1.0) - gradhi / (mj * uj);
+            const T f_ji = T(1.0) - pv.gradh[j] * m_u_inv;
+            const T u_per_pres_j = uj / presj;
+            s1 = mj * (g2upp_i * uj * f_ij + T(0.5) * pi_ij);
+            s2 = mj * (g2u_i * u_per_pres_j * f_ji + T(0.5) * pi_ij);
+            e1 = mj * g2upp_i * uj * f_ij * vdwi + T(0.5) * mj * pi_ij * vdwij + dene_ac;
+        }
+#pragma unroll
+        for (int k = 0; k < D; ++k) ac[k].add(-(dwi[k] * s1 + dwj[k] * s2));
+        de.add(e1);
+    }
+#pragma unroll
+    for (int k = 0; k < D; ++k) a.acc[k][i] = (double)ac[k].get();
+    a.dene[i] = (double)de.get();
+}
+
+// Riemann problem of one pair: iterative two-shock solver or HLL.  In the
+// FP32 form the Newton loop runs in FP32 and, with refine64, one final Newton
+// step and v* are evaluated in FP64.
+template <typename T, int SOLVER>
+SPH_DEV int solve_pair(const T l[4], const T r[4], T gamma, int refine64, const T *guess, T &pstar, T &vstar,
+                       int &fallback)
+{
+    if (SOLVER == SPHGPU_RIEMANN_HLL) {
+        hll_solver<T>(l, r, pstar, vstar);
+        return 0;
+    }
+    TwoShock<T> ts;
+    ts.setup(gamma, l, r);
+    if (ts.bad) {
+        pstar = T(0);
+        vstar = T(0);
+        return 0;
+    }
+    T p;
+    int it;
+    fallback = 0;
+    if (guess) {
+        // learned starting point: Newton from it for at most 4 steps; if that
+        // does not meet the solver's tolerance, restart from the plain guess
+        const T g = *guess;
+        bool conv = false;
+        it = 0;
+        if (g > T(1e-25) && g < T(1e30)) {
+            p = g;
+            for (int k = 0; k < 4; ++k) {
+                const T p_old = p;
+                p = ts.step(p);
+                ++it;
+                if (tabs(p - p_old) / p < T(1e-6)) { conv = true; break; }
+            }
+        }
+        if (!conv) {
+            fallback = 1;
+            p = ts.plain_guess();
+            it += ts.iterate(p);
+        }
+    } else {
+        p = ts.plain_guess();
+        it = ts.iterate(p);
+    }
+    if (sizeof(T) == 4 && refine64) {
+        TwoShock<double> td;
+        const double ld[4] = {(double)l[0], (double)l[1], (double)l[2], (double)l[3]};
+        const double rd[4] = {(double)r[0], (double)r[1], (double)r[2], (double)r[3]};
+        td.setup((double)gamma, ld, rd);
+        const double pd = td.step((double)p);
+        pstar = (T)pd;
+        vstar = (T)td.vstar(pd);
+    } else {
+        pstar = p;
+        vstar = ts.vstar(p);
+    }
+    return it;
+}
+
+template <int D, int K, typename T, int SOLVER, int LEARNED>
+__global__ __launch_bounds__(256) void k_force_gsph(ForceArgs<T> a)
+{
+    const int i0 = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = i0 < a.n;
+    const int i = valid ? i0 : a.n - 1;
+    const PairView<T> &pv = a.pv;
+    T xi[3] = {0, 0, 0}, vi[3] = {0, 0, 0}, gdi[3] = {0, 0, 0}, gpi[3] = {0, 0, 0}, gvi[3][3] = {};
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        xi[k] = pv.x[k][i];
+        vi[k] = pv.v[k][i];
+        if (a.gsph2) {
+            gdi[k] = pv.gd[k][i];
+            gpi[k] = pv.gp[k][i];
+#pragma unroll
+            for (int b2 = 0; b2 < D; ++b2) gvi[b2][k] = pv.gv[3 * b2 + k][i];
+        }
+    }
+    const T hi = pv.sml[i], rhoi = pv.dens[i], presi = pv.pres[i], ci = pv.sound[i];
+    const T gamma = T(a.gamma);
+    const T rho2_inv_i = T(1.0) / sqr(rhoi);
+    const T dt = T(a.dscal[0]);
+    Acc<T> ac[3], de;
+    const int nc = valid ? a.cnt[i] : 0;
+    // the wave runs as many pair slots as its longest list
+    int nmax = nc;
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+        const int o = __shfl_xor(nmax, off, kWave);
+        nmax = o > nmax ? o : nmax;
+    }
+    long long iters = 0, wave_max_sum = 0, calls = 0, fallbacks = 0, learned = 0;
+    for (int q = 0; q < nmax; ++q) {
+        const bool act = q < nc;
+        int it = 0;
+        if (act) {
+            const int j = a.idx[(size_t)q * a.n + i];
+            T d[3];
+            const T r = tsqrt(pair_r2<D, T>(pv, xi, j, a.box, d));
+            const T r_inv = T(1.0) / r;
+            T e[3], vj[3];
+            T ve_i = T(0), ve_j = T(0);
+#pragma unroll
+            for (int k = 0; k < D; ++k) {
+                e[k] = d[k] * r_inv;
+                vj[k] = pv.v[k][j];
+                ve_i = (k == 0) ? vi[k] * e[k] : ve_i + vi[k] * e[k];
+                ve_j = (k == 0) ? vj[k] * e[k] : ve_j + vj[k] * e[k];
+            }
+            const T rhoj = pv.dens[j], presj = pv.pres[j], cj = pv.sound[j];
+            T left[4], right[4];
+            if (a.gsph2) {
+                // MUSCL reconstruction, g_fluid_force.cpp:103-140
+                const T delta_i = T(0.5) * (T(1.0) - ci * dt * r_inv);
+                const T delta_j = T(0.5) * (T(1.0) - cj * dt * r_inv);
+                const T dv_ij = ve_i - ve_j;
+                T dvi[3], dvj[3];
+#pragma unroll
+                for (int b2 = 0; b2 < D; ++b2) {
+                    T si = T(0), sj = T(0);
+#pragma unroll
+                    for (int k = 0; k < D; ++k) {
+                        si = (k == 0) ? gvi[b2][k] * e[k] : si + gvi[b2][k] * e[k];
+                        const T gvj = pv.gv[3 * b2 + k][j];
+                        sj = (k == 0) ? gvj * e[k] : sj + gvj * e[k];
+                    }
+                    dvi[b2] = si;
+                    dvj[b2] = sj;
+                }
+                T dve_i = T(0), dve_j = T(0), dd_i = T(0), dd_j = T(0), dp_i = T(0), dp_j = T(0);
+#pragma unroll
+                for (int k = 0; k < D; ++k) {
+                    dve_i = (k == 0) ? dvi[k] * e[k] : dve_i + dvi[k] * e[k];
+                    dve_j = (k == 0) ? dvj[k] * e[k] : dve_j + dvj[k] * e[k];
+                    dd_i = (k == 0) ? gdi[k] * e[k] : dd_i + gdi[k] * e[k];
+                    const T gdj = pv.gd[k][j];
+                    dd_j = (k == 0) ? gdj * e[k] : dd_j + gdj * e[k];
+                    dp_i = (k == 0) ? gpi[k] * e[k] : dp_i + gpi[k] * e[k];
+                    const T gpj = pv.gp[k][j];
+                    dp_j = (k == 0) ? gpj * e[k] : dp_j + gpj * e[k];
+                }
+                dve_i = dve_i * r;
+                dve_j = dve_j * r;
+                dd_i = dd_i * r;
+                dd_j = dd_j * r;
+                dp_i = dp_i * r;
+                dp_j = dp_j * r;
+                right[0] = ve_i - limiter(dv_ij, dve_i) * delta_i;
+                left[0] = ve_j + limiter(dv_ij, dve_j) * delta_j;
+                const T dd_ij = rhoi - rhoj;
+                right[1] = rhoi - limiter(dd_ij, dd_i) * delta_i;
+                left[1] = rhoj + limiter(dd_ij, dd_j) * delta_j;
+                const T dp_ij = presi - presj;
+                right[2] = presi - limiter(dp_ij, dp_i) * delta_i;
+                left[2] = presj + limiter(dp_ij, dp_j) * delta_j;
+                right[3] = tsqrt(gamma * right[2] / right[1]);
+                left[3] = tsqrt(gamma * left[2] / left[1]);
+            } else {
+                right[0] = ve_i; right[1] = rhoi; right[2] = presi; right[3] = ci;
+                left[0] = ve_j; left[1] = rhoj; left[2] = presj; left[3] = cj;
+            }
+            T pstar, vstar;
+            int fb = 0;
+            T guess;
+            const T *gp = nullptr;
+            if (LEARNED) {
+                guess = (T)mlp_guess(a.mlp, (float)left[0], (float)left[1], (float)left[2], (float)right[0],
+                                     (float)right[1], (float)right[2]);
+                gp = &guess;
+            }
+            it = solve_pair<T, SOLVER>(left, right, gamma, a.refine64, gp, pstar, vstar, fb);
+            if (SOLVER == SPHGPU_RIEMANN_ITERATIVE) {
+                ++calls;
+                fallbacks += fb;
+                learned += (LEARNED && !fb) ? 1 : 0;
+                if (a.rlog) {
+                    const unsigned long long slot = atomicAdd(a.rlog_count, 1ull);
+                    if ((long long)(slot % (unsigned long long)a.rlog_stride) == 0) {
+                        const long long row = (long long)(slot / (unsigned long long)a.rlog_stride) % a.rlog_cap;
+                        float *o = a.rlog + row * 7;
+                        o[0] = (float)left[0]; o[1] = (float)left[1]; o[2] = (float)left[2];
+                        o[3] = (float)right[0]; o[4] = (float)right[1]; o[5] = (float)right[2];
+                        o[6] = (float)pstar;
+                    }
+                }
+            }
+            // Godunov force, g_fluid_force.cpp:158-165
+            const T hj = pv.sml[j], mj = pv.mass[j];
+            const T cwi = Kern<D, K, T>::dwc(r, hi);
+            const T cwj = Kern<D, K, T>::dwc(r, hj);
+            const T rho2_inv_j = T(1.0) / sqr(rhoj);
+            const T s1 = mj * pstar * rho2_inv_i;
+            const T s2 = mj * pstar * rho2_inv_j;
+            T fv = T(0);
+#pragma unroll
+            for (int k = 0; k < D; ++k) {
+                const T f = d[k] * cwi * s1 + d[k] * cwj * s2;
+                ac[k].add(-f);
+                const T dvk = e[k] * vstar - vi[k];
+                fv = (k == 0) ? f * dvk : fv + f * dvk;
+            }
+            de.add(-fv);
+        }
+        if (SOLVER == SPHGPU_RIEMANN_ITERATIVE) {
+            iters += it;
+            int m = it;
+            for (int off = kWave / 2; off > 0; off >>= 1) {
+                const int o = __shfl_xor(m, off, kWave);
+                m = o > m ? o : m;
+            }
+            wave_max_sum += m;
+        }
+    }
+    if (valid) {
+#pragma unroll
+        for (int k = 0; k < D; ++k) a.acc[k][i] = (double)ac[k].get();
+        a.dene[i] = (double)de.get();
+    }
+    wave_add_atomic(&a.stats[3], valid ? nc : 0);
+    if (SOLVER == SPHGPU_RIEMANN_ITERATIVE) {
+        wave_add_atomic(&a.stats[5], iters);
+        if ((threadIdx.x & (kWave - 1)) == 0) atomicAdd((unsigned long long *)&a.stats[6], (unsigned long long)wave_max_sum);
+        wave_add_atomic(&a.stats[7], calls);
+        wave_add_atomic(&a.stats[8], fallbacks);
+        wave_add_atomic(&a.stats[9], learned);
+    }
+}
+
+// batched Riemann problems (parity hook)
+template <typename T, int SOLVER>
+__global__ void k_riemann_batch(int n, const double *__restrict__ L, const double *__restrict__ R, double gamma,
+                                int refine64, double *__restrict__ ps, double *__restrict__ vs, int *__restrict__ it)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    T l[4], r[4];
+    for (int q = 0; q < 4; ++q) { l[q] = (T)L[4 * k + q]; r[q] = (T)R[4 * k + q]; }
+    T p, v;
+    int fb = 0;
+    const int iters = solve_pair<T, SOLVER>(l, r, (T)gamma, refine64, nullptr, p, v, fb);
+    ps[k] = (double)p;
+    vs[k] = (double)v;
+    it[k] = iters;
+}
+
+// ---------------------------------------------------------------------------------
+template <typename T> void fill_mirror_ptrs(sphgpu_ctx *c, PreArgs<T> &a)
+{
+    for (int m = 0; m < M_NFIELD; ++m) a.mf[m] = (sizeof(T) == 4) ? c->mf[m] : nullptr;
+}
+
+template <int D, int K, int TYPE, typename T> void launch_pre(sphgpu_ctx *c)
+{
+    PreArgs<T> a;
+    a.n = c->n;
+    a.idx = c->lpre.idx;
+    a.cnt = c->lpre.cnt;
+    a.pv = pair_view<T>(c);
+    for (int k = 0; k < 3; ++k) a.x64[k] = c->pos[k < c->dim ? k : 0];
+    a.hs = c->hs;
+    a.mass64 = c->sc[S_MASS]; a.ene64 = c->sc[S_ENE]; a.sound64 = c->sc[S_SOUND]; a.dens64 = c->sc[S_DENS];
+    a.sml = c->sc[S_SML]; a.dens = c->sc[S_DENS]; a.pres = c->sc[S_PRES]; a.gradh = c->sc[S_GRADH];
+    a.balsara = c->sc[S_BALSARA]; a.alpha = c->sc[S_ALPHA];
+    a.neighbor = c->neighbor;
+    for (int g = 0; g < kNGrad; ++g) a.grad[g] = c->grad[g];
+    fill_mirror_ptrs<T>(c, a);
+    a.hvs_min = c->dmin;
+    a.dscal = c->dscal;
+    a.stats = c->dstats;
+    a.box = c->box;
+    a.nn = c->prm.neighbor_number;
+    a.iterative = c->prm.iterative_sml;
+    a.balsara_on = c->prm.use_balsara_switch;
+    a.td_av = c->prm.use_time_dependent_av;
+    a.gsph2 = c->prm.gsph_2nd_order;
+    a.gamma = c->prm.gamma;
+    a.kappa = c->prm.iterative_sml ? 1.2 : 1.0;
+    a.alpha_min = c->prm.alpha_min;
+    a.alpha_max = c->prm.alpha_max;
+    a.eps_av = c->prm.epsilon_av;
+    hipLaunchKernelGGL((k_pre<D, K, TYPE, T>), dim3(grid_for(c->n, 256)), dim3(256), 0, c->stream, a);
+}
+
+template <int D, int K, typename T> void dispatch_pre_type(sphgpu_ctx *c)
+{
+    switch (c->prm.sph_type) {
+    case SPHGPU_SSPH: launch_pre<D, K, SPHGPU_SSPH, T>(c); break;
+    case SPHGPU_DISPH: launch_pre<D, K, SPHGPU_DISPH, T>(c); break;
+    default: launch_pre<D, K, SPHGPU_GSPH, T>(c); break;
+    }
+}
+
+template <int D, typename T> void dispatch_pre_kernel(sphgpu_ctx *c)
+{
+    switch (c->prm.kernel) {
+    case SPHGPU_CUBIC_SPLINE: dispatch_pre_type<D, SPHGPU_CUBIC_SPLINE, T>(c); break;
+    case SPHGPU_WENDLAND: if (D > 1) dispatch_pre_type<D, SPHGPU_WENDLAND, T>(c); break;
+    default: dispatch_pre_type<D, SPHGPU_GAUSSIAN, T>(c); break;
+    }
+}
+
+template <typename T> void dispatch_pre(sphgpu_ctx *c)
+{
+    if (c->dim == 1) dispatch_pre_kernel<1, T>(c);
+    else if (c->dim == 2) dispatch_pre_kernel<2, T>(c);
+    else dispatch_pre_kernel<3, T>(c);
+}
+
+template <int D> void launch_initial_density(sphgpu_ctx *c, const double *const *xdev)
+{
+    const dim3 g(grid_for(c->n, 256)), b(256);
+    switch (c->prm.kernel) {
+    case SPHGPU_CUBIC_SPLINE:
+        hipLaunchKernelGGL((k_initial_density<D, SPHGPU_CUBIC_SPLINE>), g, b, 0, c->stream, c->n, c->lpre.idx,
+                           c->lpre.cnt, xdev, c->sc[S_MASS], c->sc[S_SML], c->sc[S_DENS], c->box);
+        break;
+    case SPHGPU_WENDLAND:
+        if (D > 1)
+            hipLaunchKernelGGL((k_initial_density<D, SPHGPU_WENDLAND>), g, b, 0, c->stream, c->n, c->lpre.idx,
+                               c->lpre.cnt, xdev, c->sc[S_MASS], c->sc[S_SML], c->sc[S_DENS], c->box);
+        break;
+    default:
+        hipLaunchKernelGGL((k_initial_density<D, SPHGPU_GAUSSIAN>), g, b, 0, c->stream, c->n, c->lpre.idx,
+                           c->lpre.cnt, xdev, c->sc[S_MASS], c->sc[S_SML], c->sc[S_DENS], c->box);
+    }
+}
+
+template <int D> void launch_hguess(sphgpu_ctx *c, double kappa, double *out)
+{
+    hipLaunchKernelGGL((k_hguess<D>), dim3(grid_for(c->n, 256)), dim3(256), 0, c->stream, c->n,
+                       (const double *)c->sc[S_MASS], (const double *)c->sc[S_DENS], c->prm.neighbor_number, kappa, out);
+}
+
+static void hguess(sphgpu_ctx *c, double kappa, double *out)
+{
+    if (c->dim == 1) launch_hguess<1>(c, kappa, out);
+    else if (c->dim == 2) launch_hguess<2>(c, kappa, out);
+    else launch_hguess<3>(c, kappa, out);
+}
+
+// ---- force dispatch ----
+template <typename T> ForceArgs<T> force_args(sphgpu_ctx *c)
+{
+    ForceArgs<T> a;
+    a.n = c->n;
+    a.idx = c->lforce.idx;
+    a.cnt = c->lforce.cnt;
+    a.pv = pair_view<T>(c);
+    for (int k = 0; k < 3; ++k) a.acc[k] = c->acc[k];
+    a.dene = c->sc[S_DENE];
+    a.dscal = c->dscal;
+    a.stats = c->dstats;
+    a.box = c->box;
+    a.gamma = c->prm.gamma;
+    a.alpha_ac = c->prm.alpha_ac;
+    a.use_ac = c->prm.use_ac;
+    a.use_gravity = c->prm.use_gravity;
+    a.gsph2 = c->prm.gsph_2nd_order;
+    a.refine64 = riemann_refine64(c);
+    a.guess = c->prm.riemann_guess;
+    a.mlp = mlp_dev(c);
+    a.rlog = c->rlog_on ? c->rlog : nullptr;
+    a.rlog_cap = c->rlog_cap;
+    a.rlog_stride = c->rlog_stride;
+    a.rlog_count = c->rlog_count;
+    return a;
+}
+
+template <int D, int K, typename T> void launch_force(sphgpu_ctx *c)
+{
+    ForceArgs<T> a = force_args<T>(c);
+    const dim3 g(grid_for(c->n, 256)), b(256);
+    if (c->prm.sph_type == SPHGPU_SSPH) {
+        hipLaunchKernelGGL((k_force_sph<D, K, SPHGPU_SSPH, T>), g, b, 0, c->stream, a);
+    } else if (c->prm.sph_type == SPHGPU_DISPH) {
+        hipLaunchKernelGGL((k_force_sph<D, K, SPHGPU_DISPH, T>), g, b, 0, c->stream, a);
+    } else if (c->prm.riemann_solver == SPHGPU_RIEMANN_HLL) {
+        hipLaunchKernelGGL((k_force_gsph<D, K, T, SPHGPU_RIEMANN_HLL, 0>), g, b, 0, c->stream, a);
+    } else if (c->prm.riemann_guess == SPHGPU_GUESS_LEARNED && a.mlp.w) {
+        hipLaunchKernelGGL((k_force_gsph<D, K, T, SPHGPU_RIEMANN_ITERATIVE, 1>), g, b, 0, c->stream, a);
+    } else {
+        hipLaunchKernelGGL((k_force_gsph<D, K, T, SPHGPU_RIEMANN_ITERATIVE, 0>), g, b, 0, c->stream, a);
+    }
+}
+
+template <int D, typename T> void dispatch_force_kernel(sphgpu_ctx *c)
+{
+    switch (c->prm.kernel) {
+    case SPHGPU_CUBIC_SPLINE: launch_force<D, SPHGPU_CUBIC_SPLINE, T>(c); break;
+    case SPHGPU_WENDLAND: if (D > 1) launch_force<D, SPHGPU_WENDLAND, T>(c); break;
+    default: launch_force<D, SPHGPU_GAUSSIAN, T>(c); break;
+    }
+}
+
+template <typename T> void dispatch_force(sphgpu_ctx *c)
+{
+    if (c->dim == 1) dispatch_force_kernel<1, T>(c);
+    else if (c->dim == 2) dispatch_force_kernel<2, T>(c);
+    else dispatch_force_kernel<3, T>(c);
+}
+
+} // namespace
+
+int pre_interaction(sphgpu_ctx *c)
+{
+    if (!c->tree.valid) return fail(c, SPHGPU_E_STATE, "pre_interaction needs a built tree");
+    int rc;
+    if (c->mirror_dirty && (rc = refresh_mirrors(c))) return rc;
+    const double kappa = c->prm.iterative_sml ? 1.2 : 1.0;
+    if (c->pre_first) {
+        // PreInteraction::initial_smoothing: h = (nn m / (rho A))^(1/D), density
+        hguess(c, 1.0, c->sc[S_SML]);
+        if ((rc = search(c, c->lpre, 0, c->sc[S_SML]))) return rc;
+        const double *xd[3];
+        for (int k = 0; k < 3; ++k) xd[k] = c->pos[k < c->dim ? k : 0];
+        double **xdev = nullptr;
+        // the pointer table lives in the reduction scratch (3 entries)
+        xdev = (double **)c->red;
+        HIPCHK(c, hipMemcpyAsync(xdev, xd, sizeof(xd), hipMemcpyHostToDevice, c->stream));
+        time_begin(c, "initial_smoothing");
+        if (c->dim == 1) launch_initial_density<1>(c, (const double *const *)xdev);
+        else if (c->dim == 2) launch_initial_density<2>(c, (const double *const *)xdev);
+        else launch_initial_density<3>(c, (const double *const *)xdev);
+        time_end(c);
+        HIPCHK(c, hipGetLastError());
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        c->pre_first = false;
+    }
+    time_begin(c, "pre_hguess");
+    hguess(c, kappa, c->hs);
+    time_end(c);
+    if ((rc = search(c, c->lpre, 0, c->hs))) return rc;
+    hipLaunchKernelGGL(k_minbits_init, dim3(1), dim3(8), 0, c->stream, c->dmin);
+    time_begin(c, "pre_interaction");
+    if (c->prm.pair_precision == SPHGPU_PAIR_FP64) dispatch_pre<double>(c);
+    else dispatch_pre<float>(c);
+    time_end(c);
+    hipLaunchKernelGGL(k_store_hvs, dim3(1), dim3(1), 0, c->stream, (const unsigned long long *)c->dmin, c->dscal);
+    HIPCHK(c, hipGetLastError());
+    c->have_pre = true;
+    return tree_set_kernel(c);
+}
+
+int fluid_force(sphgpu_ctx *c)
+{
+    if (!c->tree.valid) return fail(c, SPHGPU_E_STATE, "fluid_force needs a built tree");
+    int rc;
+    if (c->mirror_dirty && (rc = refresh_mirrors(c))) return rc;
+    if ((rc = search(c, c->lforce, 1, c->sc[S_SML]))) return rc;
+    time_begin(c, "fluid_force");
+    if (c->prm.pair_precision == SPHGPU_PAIR_FP64) dispatch_force<double>(c);
+    else dispatch_force<float>(c);
+    time_end(c);
+    HIPCHK(c, hipGetLastError());
+    return 0;
+}
+
+int riemann_batch(sphgpu_ctx *c, int solver, int precision, int n, const double *left, const double *right,
+                  double *pstar, double *vstar, int32_t *iters)
+{
+    double *d = nullptr;
+    int *di = nullptr;
+    const size_t b = sizeof(double) * (size_t)n;
+    HIPCHK(c, hipMalloc((void **)&d, b * 12));
+    HIPCHK(c, hipMalloc((void **)&di, sizeof(int) * (size_t)n));
+    double *L = d, *R = d + 4ull * n, *P = d + 8ull * n, *V = d + 9ull * n;
+    HIPCHK(c, hipMemcpyAsync(L, left, b * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(R, right, b * 4, hipMemcpyHostToDevice, c->stream));
+    const dim3 g(grid_for(n, 256)), bl(256);
+    const int ref = riemann_refine64(c);
+    if (precision == SPHGPU_PAIR_FP64) {
+        if (solver == SPHGPU_RIEMANN_HLL)
+            hipLaunchKernelGGL((k_riemann_batch<double, SPHGPU_RIEMANN_HLL>), g, bl, 0, c->stream, n, L, R, c->prm.gamma, ref, P, V, di);
+        else
+            hipLaunchKernelGGL((k_riemann_batch<double, SPHGPU_RIEMANN_ITERATIVE>), g, bl, 0, c->stream, n, L, R, c->prm.gamma, ref, P, V, di);
+    } else {
+        if (solver == SPHGPU_RIEMANN_HLL)
+            hipLaunchKernelGGL((k_riemann_batch<float, SPHGPU_RIEMANN_HLL>), g, bl, 0, c->stream, n, L, R, c->prm.gamma, ref, P, V, di);
+        else
+            hipLaunchKernelGGL((k_riemann_batch<float, SPHGPU_RIEMANN_ITERATIVE>), g, bl, 0, c->stream, n, L, R, c->prm.gamma, ref, P, V, di);
+    }
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(pstar, P, b, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(vstar, V, b, hipMemcpyDeviceToHost, c->stream));
+    if (iters) HIPCHK(c, hipMemcpyAsync(iters, di, sizeof(int) * (size_t)n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    hipFree(d);
+    hipFree(di);
+    return 0;
+}
+
+} // namespace sphgpu

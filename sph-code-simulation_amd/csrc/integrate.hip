@@ -1,0 +1,291 @@
+// integrate.hip -- FP64 integrator, timestep and global reductions.
+//
+// References: Solver::predict / Solver::correct, standard branch
+// (src/solver.cpp:1047-1060, 1106-1111); TimeStep::calculation
+// (src/timestep.cpp:18-38); Solver::compute_total_energies
+// (src/solver.cpp:1149-1175); Solver::initialize tail (src/solver.cpp:941-953).
+// All arithmetic here is FP64 with the reference's expressions.  Reductions
+// are two-pass with a fixed combination order, so they are deterministic.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+
+#include "ctx.hpp"
+
+namespace sphgpu {
+
+namespace {
+
+struct IntArgs {
+    int n, dim;
+    double *pos[3], *vel[3], *vel_p[3];
+    const double *acc[3];
+    double *ene, *ene_p, *sound;
+    const double *dene;
+    const double *dscal;
+    double c_sound;
+    Box box;
+};
+
+__global__ void k_predict(IntArgs a)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    const double dt = a.dscal[0];
+    for (int k = 0; k < a.dim; ++k) a.vel_p[k][i] = a.vel[k][i] + a.acc[k][i] * (0.5 * dt);
+    a.ene_p[i] = a.ene[i] + a.dene[i] * (0.5 * dt);
+    for (int k = 0; k < a.dim; ++k) a.pos[k][i] += a.vel_p[k][i] * dt;
+    for (int k = 0; k < a.dim; ++k) a.vel[k][i] += a.acc[k][i] * dt;
+    a.ene[i] += a.dene[i] * dt;
+    a.sound[i] = __builtin_sqrt(a.c_sound * a.ene[i]);
+    if (a.box.periodic) {
+        // Periodic::apply, include/periodic.hpp:61-72
+        for (int k = 0; k < a.dim; ++k) {
+            double r = a.pos[k][i];
+            if (r < a.box.lo[k]) r += a.box.range[k];
+            else if (r > a.box.hi[k]) r -= a.box.range[k];
+            a.pos[k][i] = r;
+        }
+    }
+}
+
+__global__ void k_correct(IntArgs a)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    const double dt = a.dscal[0];
+    for (int k = 0; k < a.dim; ++k) a.vel[k][i] = a.vel_p[k][i] + a.acc[k][i] * (0.5 * dt);
+    a.ene[i] = a.ene_p[i] + a.dene[i] * (0.5 * dt);
+    a.sound[i] = __builtin_sqrt(a.c_sound * a.ene[i]);
+}
+
+__global__ void k_init_fields(int n, double alpha0, double c_sound, const double *__restrict__ ene,
+                              double *__restrict__ alpha, double *__restrict__ balsara, double *__restrict__ sound)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    alpha[i] = alpha0;
+    balsara[i] = 1.0;
+    sound[i] = __builtin_sqrt(c_sound * ene[i]);
+}
+
+// dt_force_i = c_force * sqrt(h / |a|) over |a| > 0, block minima
+__global__ void k_dt_partial(int n, int dim, const double *ax, const double *ay, const double *az,
+                             const double *__restrict__ sml, double c_force, double *__restrict__ part)
+{
+    __shared__ double s[256];
+    double m = 1.7976931348623157e308;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        double a2 = ax[i] * ax[i];
+        if (dim > 1) a2 = a2 + ay[i] * ay[i];
+        if (dim > 2) a2 = a2 + az[i] * az[i];
+        const double acc_abs = __builtin_sqrt(a2);
+        if (acc_abs > 0.0) {
+            const double d = c_force * __builtin_sqrt(sml[i] / acc_abs);
+            if (d < m) m = d;
+        }
+    }
+    s[threadIdx.x] = m;
+    __syncthreads();
+    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o && s[threadIdx.x + o] < s[threadIdx.x]) s[threadIdx.x] = s[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = s[0];
+}
+
+__global__ void k_dt_final(const double *__restrict__ part, int nb, double c_sound, double *dscal)
+{
+    if (threadIdx.x != 0) return;
+    double m = 1.7976931348623157e308;
+    for (int b = 0; b < nb; ++b)
+        if (part[b] < m) m = part[b];
+    const double dt_sound = c_sound * dscal[2];
+    dscal[0] = (m < dt_sound) ? m : dt_sound;  // std::min(dt_sound, dt_force)
+}
+
+__global__ void k_advance_time(double *dscal) { dscal[1] = dscal[1] + dscal[0]; }
+
+// kinetic, thermal, potential, P_x, P_y, P_z partial sums per block
+__global__ void k_energy_partial(int n, int dim, int grav, const double *vx, const double *vy, const double *vz,
+                                 const double *__restrict__ m, const double *__restrict__ u,
+                                 const double *__restrict__ phi, double *__restrict__ part)
+{
+    __shared__ double s[6][256];
+    double a[6] = {0, 0, 0, 0, 0, 0};
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const double v0 = vx[i], v1 = dim > 1 ? vy[i] : 0.0, v2 = dim > 2 ? vz[i] : 0.0;
+        double vsq = v0 * v0;
+        if (dim > 1) vsq = vsq + v1 * v1;
+        if (dim > 2) vsq = vsq + v2 * v2;
+        a[0] += 0.5 * m[i] * vsq;
+        a[1] += m[i] * u[i];
+        if (grav) a[2] += 0.5 * m[i] * phi[i];
+        a[3] += m[i] * v0;
+        a[4] += m[i] * v1;
+        a[5] += m[i] * v2;
+    }
+    for (int q = 0; q < 6; ++q) s[q][threadIdx.x] = a[q];
+    __syncthreads();
+    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o)
+            for (int q = 0; q < 6; ++q) s[q][threadIdx.x] += s[q][threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int q = 0; q < 6; ++q) part[blockIdx.x * 6 + q] = s[q][0];
+}
+
+__global__ void k_energy_final(const double *__restrict__ part, int nb, double *__restrict__ out)
+{
+    if (threadIdx.x != 0) return;
+    double a[6] = {0, 0, 0, 0, 0, 0};
+    for (int b = 0; b < nb; ++b)
+        for (int q = 0; q < 6; ++q) a[q] += part[b * 6 + q];
+    out[0] = a[0];
+    out[1] = a[1];
+    out[2] = a[2];
+    out[3] = a[0] + a[1] + a[2];
+    out[4] = a[3];
+    out[5] = a[4];
+    out[6] = a[5];
+}
+
+struct MirrorArgs {
+    int n;
+    const double *src[M_NFIELD];
+    float *dst[M_NFIELD];
+};
+
+__global__ void k_mirrors(MirrorArgs a)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    for (int f = 0; f < M_NFIELD; ++f)
+        if (a.src[f]) a.dst[f][i] = (float)a.src[f][i];
+}
+
+IntArgs int_args(sphgpu_ctx *c)
+{
+    IntArgs a;
+    a.n = c->n;
+    a.dim = c->dim;
+    for (int k = 0; k < 3; ++k) {
+        a.pos[k] = c->pos[k];
+        a.vel[k] = c->vel[k];
+        a.vel_p[k] = c->vel_p[k];
+        a.acc[k] = c->acc[k];
+    }
+    a.ene = c->sc[S_ENE];
+    a.ene_p = c->sc[S_ENE_P];
+    a.sound = c->sc[S_SOUND];
+    a.dene = c->sc[S_DENE];
+    a.dscal = c->dscal;
+    a.c_sound = c->prm.gamma * (c->prm.gamma - 1.0);
+    a.box = c->box;
+    return a;
+}
+
+} // namespace
+
+int predict(sphgpu_ctx *c)
+{
+    time_begin(c, "predict");
+    hipLaunchKernelGGL(k_predict, dim3(grid_for(c->n, 256)), dim3(256), 0, c->stream, int_args(c));
+    time_end(c);
+    HIPCHK(c, hipGetLastError());
+    c->tree.valid = false;  // positions moved: the tree is stale
+    c->mirror_dirty = true;
+    return 0;
+}
+
+int correct(sphgpu_ctx *c)
+{
+    time_begin(c, "correct");
+    hipLaunchKernelGGL(k_correct, dim3(grid_for(c->n, 256)), dim3(256), 0, c->stream, int_args(c));
+    time_end(c);
+    HIPCHK(c, hipGetLastError());
+    c->mirror_dirty = true;
+    return 0;
+}
+
+int init_fields(sphgpu_ctx *c)
+{
+    hipLaunchKernelGGL(k_init_fields, dim3(grid_for(c->n, 256)), dim3(256), 0, c->stream, c->n, c->prm.av_alpha,
+                       c->prm.gamma * (c->prm.gamma - 1.0), (const double *)c->sc[S_ENE], c->sc[S_ALPHA],
+                       c->sc[S_BALSARA], c->sc[S_SOUND]);
+    HIPCHK(c, hipGetLastError());
+    c->mirror_dirty = true;
+    return 0;
+}
+
+int timestep(sphgpu_ctx *c)
+{
+    const int nb = std::min(c->red_blocks, grid_for(c->n, 256));
+    const double *ax = c->acc[0], *ay = c->acc[c->dim > 1 ? 1 : 0], *az = c->acc[c->dim > 2 ? 2 : 0];
+    time_begin(c, "timestep");
+    hipLaunchKernelGGL(k_dt_partial, dim3(nb), dim3(256), 0, c->stream, c->n, c->dim, ax, ay, az,
+                       (const double *)c->sc[S_SML], c->prm.cfl_force, c->red);
+    hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(64), 0, c->stream, (const double *)c->red, nb, c->prm.cfl_sound,
+                       c->dscal);
+    time_end(c);
+    HIPCHK(c, hipGetLastError());
+    return 0;
+}
+
+int advance_time(sphgpu_ctx *c)
+{
+    hipLaunchKernelGGL(k_advance_time, dim3(1), dim3(1), 0, c->stream, c->dscal);
+    HIPCHK(c, hipGetLastError());
+    return 0;
+}
+
+int energy_momentum(sphgpu_ctx *c, double out[7])
+{
+    const int nb = std::min(c->red_blocks, grid_for(c->n, 256));
+    const double *vx = c->vel[0], *vy = c->vel[c->dim > 1 ? 1 : 0], *vz = c->vel[c->dim > 2 ? 2 : 0];
+    hipLaunchKernelGGL(k_energy_partial, dim3(nb), dim3(256), 0, c->stream, c->n, c->dim, c->prm.use_gravity, vx, vy,
+                       vz, (const double *)c->sc[S_MASS], (const double *)c->sc[S_ENE], (const double *)c->sc[S_PHI],
+                       c->red);
+    hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(64), 0, c->stream, (const double *)c->red, nb, c->dscal + 8);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(c->hscal + 8, c->dscal + 8, sizeof(double) * 7, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    std::memcpy(out, c->hscal + 8, sizeof(double) * 7);
+    return 0;
+}
+
+int refresh_mirrors(sphgpu_ctx *c)
+{
+    if (c->prm.pair_precision == SPHGPU_PAIR_FP64) {
+        c->mirror_dirty = false;
+        return 0;
+    }
+    MirrorArgs a;
+    a.n = c->n;
+    for (int f = 0; f < M_NFIELD; ++f) { a.src[f] = nullptr; a.dst[f] = c->mf[f]; }
+    for (int k = 0; k < c->dim; ++k) {
+        a.src[M_X + k] = c->pos[k];
+        a.src[M_VX + k] = c->vel[k];
+        a.src[M_GD0 + k] = c->grad[k];
+        a.src[M_GP0 + k] = c->grad[3 + k];
+        for (int b = 0; b < c->dim; ++b) a.src[M_GV0 + 3 * b + k] = c->grad[6 + 3 * b + k];
+    }
+    a.src[M_MASS] = c->sc[S_MASS];
+    a.src[M_ENE] = c->sc[S_ENE];
+    a.src[M_SOUND] = c->sc[S_SOUND];
+    a.src[M_SML] = c->sc[S_SML];
+    a.src[M_DENS] = c->sc[S_DENS];
+    a.src[M_PRES] = c->sc[S_PRES];
+    a.src[M_GRADH] = c->sc[S_GRADH];
+    a.src[M_BALSARA] = c->sc[S_BALSARA];
+    a.src[M_ALPHA] = c->sc[S_ALPHA];
+    time_begin(c, "mirrors");
+    hipLaunchKernelGGL(k_mirrors, dim3(grid_for(c->n, 256)), dim3(256), 0, c->stream, a);
+    time_end(c);
+    HIPCHK(c, hipGetLastError());
+    c->mirror_dirty = false;
+    return 0;
+}
+
+} // namespace sphgpu

@@ -1,0 +1,750 @@
+// tree.hip -- octree build on the GPU with the reference's exact topology.
+//
+// Reference: BHTree::make / BHNode::create_tree / assign / set_kernel
+// (src/bhtree.cpp:59-115, 154-247, 249-275).  The reference inserts particles
+// one by one into a pointer octree: the child of a node is chosen by the FP64
+// comparisons pos[k] > center[k] (ties go to the lower child), child centres
+// are centre + (+-1) * edge * 0.25, and a child with more than
+// leafParticleNumber particles is split again while the parent's level is below
+// maxTreeLevel.
+//
+// Here the same decisions are made for every particle independently:
+//  1. key: the particle's octant digit at every level, from replaying the
+//     FP64 centre recursion (bits = dim per level, most significant = level 1);
+//  2. stable LSD radix sort of the keys (8-bit digits, hand-written);
+//  3. the particle state is permuted into key order;
+//  4. kdig[i] = first level at which key[i-1] and key[i] differ; a node at
+//     level L is a maximal run of positions without a boundary of kdig <= L-1;
+//  5. lam[i] = level of the leaf that holds particle i, from a scan of the
+//     kdig window of +-leafParticleNumber positions (a node is a leaf when it
+//     holds <= leaf particles or sits at level maxTreeLevel+1);
+//  6. the nodes starting at position s are the levels max(2, kdig[s]+1) ..
+//     lam[s]; an exclusive scan of their counts gives every node its index in
+//     pre-order (sorted by start, then level), which is the reference's
+//     depth-first child order;
+//  7. per node: level, [start, end), skip (index of the next node outside its
+//     subtree, for stackless walks), centre and edge replayed in FP64, and mass,
+//     centre of mass and kernel_size from block-hierarchy range reductions.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "ctx.hpp"
+
+namespace sphgpu {
+
+namespace {
+
+constexpr int kSortThreads = 256;
+constexpr int kSortItems = 16;
+constexpr int kSortTile = kSortThreads * kSortItems;
+
+struct TreeShape {
+    int dim;
+    int bits;        // bits per level = dim
+    int ml;          // maxTreeLevel
+    int leaf;        // leafParticleNumber
+    int n;
+};
+
+// ---- root geometry (src/bhtree.cpp:20-32 periodic, :67-103 bounding box) --
+__global__ void k_bbox_partial(const double *__restrict__ x, const double *__restrict__ y,
+                               const double *__restrict__ z, int n, int dim, double *__restrict__ part)
+{
+    __shared__ double smin[3][256], smax[3][256];
+    double mn[3] = {1.7976931348623157e308, 1.7976931348623157e308, 1.7976931348623157e308};
+    double mx[3] = {-1.7976931348623157e308, -1.7976931348623157e308, -1.7976931348623157e308};
+    const double *p[3] = {x, y, z};
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        for (int k = 0; k < dim; ++k) {
+            const double v = p[k][i];
+            mn[k] = v < mn[k] ? v : mn[k];
+            mx[k] = v > mx[k] ? v : mx[k];
+        }
+    }
+    for (int k = 0; k < 3; ++k) { smin[k][threadIdx.x] = mn[k]; smax[k][threadIdx.x] = mx[k]; }
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            for (int k = 0; k < 3; ++k) {
+                const double a = smin[k][threadIdx.x + s], b = smax[k][threadIdx.x + s];
+                if (a < smin[k][threadIdx.x]) smin[k][threadIdx.x] = a;
+                if (b > smax[k][threadIdx.x]) smax[k][threadIdx.x] = b;
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 3; ++k) {
+            part[blockIdx.x * 6 + k] = smin[k][0];
+            part[blockIdx.x * 6 + 3 + k] = smax[k][0];
+        }
+}
+
+__global__ void k_bbox_final(const double *__restrict__ part, int nblocks, int dim, double *__restrict__ root)
+{
+    if (threadIdx.x != 0) return;
+    double mn[3] = {1.7976931348623157e308, 1.7976931348623157e308, 1.7976931348623157e308};
+    double mx[3] = {-1.7976931348623157e308, -1.7976931348623157e308, -1.7976931348623157e308};
+    for (int b = 0; b < nblocks; ++b)
+        for (int k = 0; k < 3; ++k) {
+            if (part[b * 6 + k] < mn[k]) mn[k] = part[b * 6 + k];
+            if (part[b * 6 + 3 + k] > mx[k]) mx[k] = part[b * 6 + 3 + k];
+        }
+    double l = 0.0;
+    for (int k = 0; k < dim; ++k) {
+        root[k] = (mx[k] + mn[k]) * 0.5;
+        const double range = mx[k] - mn[k];
+        if (l < range) l = range;
+    }
+    for (int k = dim; k < 3; ++k) root[k] = 0.0;
+    root[3] = l;
+}
+
+// ---- 1. keys ----------------------------------------------------------------
+__global__ void k_keys(const double *__restrict__ x, const double *__restrict__ y, const double *__restrict__ z,
+                       const double *__restrict__ root, TreeShape sh, unsigned long long *__restrict__ key,
+                       int *__restrict__ perm)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= sh.n) return;
+    double pos[3] = {x[i], sh.dim > 1 ? y[i] : 0.0, sh.dim > 2 ? z[i] : 0.0};
+    double c[3] = {root[0], root[1], root[2]};
+    double e = root[3];
+    unsigned long long kk = 0;
+    for (int lev = 0; lev < sh.ml; ++lev) {
+        unsigned dig = 0;
+        for (int k = 0; k < sh.dim; ++k) {
+            const bool up = pos[k] > c[k];  // src/bhtree.cpp:211-218
+            dig |= (up ? 1u : 0u) << k;
+            // child centre, src/bhtree.cpp:233-239: c + (+-1 * edge) * 0.25
+            const double s = up ? 1.0 : -1.0;
+            c[k] = c[k] + s * e * 0.25;
+        }
+        e = e * 0.5;
+        kk = (kk << sh.bits) | dig;
+    }
+    key[i] = kk;
+    perm[i] = i;
+}
+
+// ---- 2. radix sort -------------------------------------------------------------
+__global__ __launch_bounds__(kSortThreads) void k_radix_hist(const unsigned long long *__restrict__ key, int n,
+                                                            int shift, unsigned int *__restrict__ hist, int nblocks)
+{
+    __shared__ unsigned int cnt[256];
+    cnt[threadIdx.x] = 0;
+    __syncthreads();
+    const int base = blockIdx.x * kSortTile;
+    for (int r = 0; r < kSortItems; ++r) {
+        const int e = base + r * kSortThreads + threadIdx.x;
+        if (e < n) atomicAdd(&cnt[(unsigned)(key[e] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    hist[threadIdx.x * nblocks + blockIdx.x] = cnt[threadIdx.x];
+}
+
+// exclusive scan of hist[256 * nblocks] (digit-major), single workgroup
+__global__ __launch_bounds__(1024) void k_radix_scan(unsigned int *__restrict__ hist, int total)
+{
+    __shared__ unsigned int part[1024];
+    const int t = threadIdx.x;
+    const int per = (total + 1023) / 1024;
+    const int lo = t * per, hi = min(total, lo + per);
+    unsigned int s = 0;
+    for (int i = lo; i < hi; ++i) s += hist[i];
+    part[t] = s;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        unsigned int v = (t >= off) ? part[t - off] : 0u;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    unsigned int run = (t == 0) ? 0u : part[t - 1];
+    for (int i = lo; i < hi; ++i) {
+        const unsigned int v = hist[i];
+        hist[i] = run;
+        run += v;
+    }
+}
+
+__global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
+    const unsigned long long *__restrict__ key_in, const int *__restrict__ val_in, unsigned long long *__restrict__ key_out,
+    int *__restrict__ val_out, int n, int shift, const unsigned int *__restrict__ hist, int nblocks)
+{
+    constexpr int kWaves = kSortThreads / kWave;
+    __shared__ unsigned int base[256];
+    __shared__ unsigned int running[256];
+    __shared__ unsigned int wcnt[kWaves][256];
+    const int tid = threadIdx.x;
+    const int lane = tid & (kWave - 1);
+    const int w = tid / kWave;
+    base[tid] = hist[tid * nblocks + blockIdx.x];
+    running[tid] = 0;
+    const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const int tile = blockIdx.x * kSortTile;
+    for (int r = 0; r < kSortItems; ++r) {
+        for (int q = 0; q < kWaves; ++q) wcnt[q][tid] = 0;
+        __syncthreads();
+        const int e = tile + r * kSortThreads + tid;
+        const bool valid = e < n;
+        unsigned long long k = 0;
+        int v = 0;
+        unsigned d = 256;  // invalid lanes take a digit no valid lane has
+        if (valid) {
+            k = key_in[e];
+            v = val_in[e];
+            d = (unsigned)(k >> shift) & 255u;
+        }
+        // lanes holding the same digit (9 ballots: 8 digit bits + validity)
+        unsigned long long same = __ballot(1);
+        for (int b = 0; b < 9; ++b) {
+            const bool bit = (d >> b) & 1u;
+            const unsigned long long bal = __ballot(bit);
+            same &= bit ? bal : ~bal;
+        }
+        const int rank = __popcll(same & lt);
+        if (valid && rank == 0) wcnt[w][d] = __popcll(same);
+        __syncthreads();
+        if (valid) {
+            unsigned int pre = 0;
+            for (int q = 0; q < w; ++q) pre += wcnt[q][d];
+            const unsigned int dst = base[d] + running[d] + pre + rank;
+            key_out[dst] = k;
+            val_out[dst] = v;
+        }
+        __syncthreads();
+        unsigned int tot = 0;
+        for (int q = 0; q < kWaves; ++q) tot += wcnt[q][tid];
+        running[tid] += tot;
+        __syncthreads();
+    }
+}
+
+// ---- 3. permutation of the particle state -----------------------------------
+struct GatherBatch {
+    const double *src[16];
+    double *dst[16];
+    int count;
+};
+
+__global__ void k_gather_batch(GatherBatch b, const int *__restrict__ perm, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int s = perm[i];
+    for (int f = 0; f < b.count; ++f) b.dst[f][i] = b.src[f][s];
+}
+
+__global__ void k_gather_int2(const int *__restrict__ a, int *__restrict__ ad, const int *__restrict__ b,
+                              int *__restrict__ bd, const int *__restrict__ perm, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int s = perm[i];
+    ad[i] = a[s];
+    bd[i] = b[s];
+}
+
+// ---- 4. first differing level ---------------------------------------------------
+__global__ void k_kdig(const unsigned long long *__restrict__ key, TreeShape sh, unsigned char *__restrict__ kdig)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= sh.n) return;
+    if (i == 0) { kdig[0] = 0; return; }
+    const unsigned long long x = key[i - 1] ^ key[i];
+    if (x == 0) { kdig[i] = (unsigned char)(sh.ml + 1); return; }
+    const int msb = 63 - __clzll((long long)x);
+    kdig[i] = (unsigned char)(sh.ml - msb / sh.bits);
+}
+
+// ---- 5. leaf level and node counts ----------------------------------------------
+constexpr int kMaxLevels = 64;
+
+__global__ void k_lambda(const unsigned char *__restrict__ kdig, TreeShape sh, unsigned char *__restrict__ lam,
+                         int *__restrict__ leaf_end, int *__restrict__ ncount)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= sh.n) return;
+    const int ML = sh.ml;
+    int nl[kMaxLevels + 2], nr[kMaxLevels + 2];
+    for (int t = 0; t <= ML; ++t) { nl[t] = -1; nr[t] = 0x7fffffff; }
+    // nearest boundary to the left with kdig <= t (position of the group start)
+    {
+        int m = ML + 2;
+        const int lo = max(0, i - sh.leaf + 1);
+        for (int j = i; j >= lo && m > 0; --j) {
+            const int kj = kdig[j];
+            if (kj < m) {
+                for (int t = kj; t < m && t <= ML; ++t) nl[t] = j;
+                m = kj;
+            }
+        }
+    }
+    // nearest boundary to the right (position of the group end); N counts as one
+    {
+        int m = ML + 2;
+        const int hi = min(sh.n, i + sh.leaf);
+        for (int j = i + 1; j <= hi && m > 0; ++j) {
+            const int kj = (j == sh.n) ? 0 : (int)kdig[j];
+            if (kj < m) {
+                for (int t = kj; t < m && t <= ML; ++t) nr[t] = j;
+                m = kj;
+            }
+        }
+    }
+    int L = ML + 1;
+    int lend = -1;
+    for (int l = 2; l <= ML; ++l) {
+        const int t = l - 1;
+        if (nl[t] >= 0 && nr[t] != 0x7fffffff && nr[t] - nl[t] <= sh.leaf) {
+            L = l;
+            lend = nr[t];
+            break;
+        }
+    }
+    lam[i] = (unsigned char)L;
+    leaf_end[i] = lend;
+    const int kd = kdig[i];
+    const int lmin = max(2, kd + 1);
+    ncount[i] = (i == 0 ? 1 : 0) + max(0, L - lmin + 1);
+}
+
+// exclusive scan of ncount -> nbase (n+1 entries), single workgroup
+__global__ __launch_bounds__(1024) void k_scan_int(const int *__restrict__ in, int *__restrict__ out, int n)
+{
+    __shared__ long long part[1024];
+    const int t = threadIdx.x;
+    const int per = (n + 1023) / 1024;
+    const int lo = t * per, hi = min(n, lo + per);
+    long long s = 0;
+    for (int i = lo; i < hi; ++i) s += in[i];
+    part[t] = s;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        long long v = (t >= off) ? part[t - off] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    long long run = (t == 0) ? 0 : part[t - 1];
+    for (int i = lo; i < hi; ++i) {
+        out[i] = (int)min(run, (long long)0x7fffffff);
+        run += in[i];
+    }
+    if (t == 1023) out[n] = (int)min(part[1023], (long long)0x7fffffff);
+}
+
+// ---- 6. node fill ------------------------------------------------------------------
+__device__ int prefix_end(const unsigned long long *__restrict__ key, int n, int s, int shift)
+{
+    // first j > s whose key prefix (key >> shift) differs from key[s]'s
+    const unsigned long long p = (shift >= 64) ? 0ull : (key[s] >> shift);
+    int lo = s + 1, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        const unsigned long long q = (shift >= 64) ? 0ull : (key[mid] >> shift);
+        if (q == p) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__global__ void k_fill(const unsigned long long *__restrict__ key, const unsigned char *__restrict__ kdig,
+                       const unsigned char *__restrict__ lam, const int *__restrict__ leaf_end,
+                       const int *__restrict__ nbase, const double *__restrict__ root, TreeShape sh, int total,
+                       int *__restrict__ nstart, int *__restrict__ nend, int *__restrict__ nskip,
+                       int *__restrict__ nlevel, double *__restrict__ ncenter, double *__restrict__ nedge, int cap,
+                       long long *__restrict__ stats)
+{
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= sh.n) return;
+    int idx = nbase[s];
+    const int cnt = nbase[s + 1] - idx;
+    if (cnt == 0) return;
+    const int L1 = lam[s];
+    const int ML = sh.ml;
+    if (s == 0) {
+        nstart[idx] = 0;
+        nend[idx] = sh.n;
+        nskip[idx] = total;
+        nlevel[idx] = 1;
+        for (int k = 0; k < 3; ++k) ncenter[(size_t)k * cap + idx] = root[k];
+        nedge[idx] = root[3];
+        ++idx;
+    }
+    const int lmin = max(2, (int)kdig[s] + 1);
+    const unsigned long long kk = key[s];
+    // replay the centre recursion down to level lmin
+    double c[3] = {root[0], root[1], root[2]};
+    double e = root[3];
+    for (int d = 1; d < lmin; ++d) {
+        const unsigned dig = (unsigned)(kk >> (sh.bits * (ML - d))) & ((1u << sh.bits) - 1u);
+        for (int k = 0; k < sh.dim; ++k) {
+            const double sg = ((dig >> k) & 1u) ? 1.0 : -1.0;
+            c[k] = c[k] + sg * e * 0.25;
+        }
+        e = e * 0.5;
+    }
+    int leaves = 0;
+    for (int L = lmin; L <= L1; ++L, ++idx) {
+        const bool leaf = (L == L1);
+        int end;
+        if (leaf && L1 <= ML) end = leaf_end[s];
+        else end = prefix_end(key, sh.n, s, sh.bits * (ML - (L - 1)));
+        nstart[idx] = s;
+        nend[idx] = end;
+        nskip[idx] = (end < sh.n) ? nbase[end] : total;
+        nlevel[idx] = leaf ? -L : L;
+        for (int k = 0; k < 3; ++k) ncenter[(size_t)k * cap + idx] = c[k];
+        nedge[idx] = e;
+        leaves += leaf ? 1 : 0;
+        // centre of the child on this particle's path
+        const unsigned dig = (unsigned)(kk >> (sh.bits * (ML - L))) & ((1u << sh.bits) - 1u);
+        for (int k = 0; k < sh.dim; ++k) {
+            const double sg = ((dig >> k) & 1u) ? 1.0 : -1.0;
+            c[k] = c[k] + sg * e * 0.25;
+        }
+        e = e * 0.5;
+    }
+    if (leaves) atomicAdd((unsigned long long *)&stats[11], (unsigned long long)leaves);
+    atomicMax((unsigned long long *)&stats[12], (unsigned long long)L1);
+}
+
+// ---- 7. block hierarchy and node reductions ------------------------------------------
+__global__ void k_block_max(const double *__restrict__ in, int n, double *__restrict__ out)
+{
+    // one 64-lane wave per block of 64 values
+    const int b = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int i = b * kWave + lane;
+    if ((long long)b * kWave >= n) return;
+    double v = (i < n) ? in[i] : 0.0;
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+        const double o = __shfl_down(v, off, kWave);
+        v = o > v ? o : v;
+    }
+    if (lane == 0) out[b] = v;
+}
+
+__global__ void k_block_sum4(const double *__restrict__ m, const double *__restrict__ x, const double *__restrict__ y,
+                             const double *__restrict__ z, int n, int nb, double *__restrict__ out, int stride_in,
+                             int level)
+{
+    // level 0: sums of m, m x, m y, m z over 64 particles
+    // level 1: sums of the level-0 partials over 64 blocks (inputs in out-layout)
+    const int b = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+    const int lane = threadIdx.x & (kWave - 1);
+    if (b >= nb) return;
+    const int i = b * kWave + lane;
+    double s[4] = {0, 0, 0, 0};
+    if (i < n) {
+        if (level == 0) {
+            const double mm = m[i];
+            s[0] = mm;
+            s[1] = mm * x[i];
+            s[2] = y ? mm * y[i] : 0.0;
+            s[3] = z ? mm * z[i] : 0.0;
+        } else {
+            for (int q = 0; q < 4; ++q) s[q] = m[(size_t)q * stride_in + i];
+        }
+    }
+    for (int off = kWave / 2; off > 0; off >>= 1)
+        for (int q = 0; q < 4; ++q) s[q] += __shfl_down(s[q], off, kWave);
+    if (lane == 0)
+        for (int q = 0; q < 4; ++q) out[(size_t)q * nb + b] = s[q];
+}
+
+struct RangeTables {
+    const double *h, *b1, *b2;
+};
+
+__device__ double range_max(RangeTables t, int s, int e)
+{
+    double m = 0.0;
+    while (s < e && (s & 63)) { m = fmax(m, t.h[s]); ++s; }
+    while (e > s && (e & 63)) { --e; m = fmax(m, t.h[e]); }
+    int b0 = s >> 6, b1 = e >> 6;
+    while (b0 < b1 && (b0 & 63)) { m = fmax(m, t.b1[b0]); ++b0; }
+    while (b1 > b0 && (b1 & 63)) { --b1; m = fmax(m, t.b1[b1]); }
+    for (int B = b0 >> 6; B < (b1 >> 6); ++B) m = fmax(m, t.b2[B]);
+    return m;
+}
+
+__global__ void k_node_ks(const int *__restrict__ nstart, const int *__restrict__ nend, int total, RangeTables t,
+                          double *__restrict__ ks)
+{
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= total) return;
+    ks[idx] = range_max(t, nstart[idx], nend[idx]);
+}
+
+struct SumTables {
+    const double *m, *x, *y, *z;   // particles
+    const double *b1, *b2;         // [4][nb1], [4][nb2]
+    int nb1, nb2;
+};
+
+__device__ void range_sum4(const SumTables &t, int s, int e, double out[4])
+{
+    double a[4] = {0, 0, 0, 0};
+    auto addp = [&](int i) {
+        const double mm = t.m[i];
+        a[0] += mm;
+        a[1] += mm * t.x[i];
+        a[2] += t.y ? mm * t.y[i] : 0.0;
+        a[3] += t.z ? mm * t.z[i] : 0.0;
+    };
+    while (s < e && (s & 63)) { addp(s); ++s; }
+    while (e > s && (e & 63)) { --e; addp(e); }
+    int b0 = s >> 6, b1 = e >> 6;
+    while (b0 < b1 && (b0 & 63)) { for (int q = 0; q < 4; ++q) a[q] += t.b1[(size_t)q * t.nb1 + b0]; ++b0; }
+    while (b1 > b0 && (b1 & 63)) { --b1; for (int q = 0; q < 4; ++q) a[q] += t.b1[(size_t)q * t.nb1 + b1]; }
+    for (int B = b0 >> 6; B < (b1 >> 6); ++B)
+        for (int q = 0; q < 4; ++q) a[q] += t.b2[(size_t)q * t.nb2 + B];
+    for (int q = 0; q < 4; ++q) out[q] = a[q];
+}
+
+__global__ void k_node_mass(const int *__restrict__ nstart, const int *__restrict__ nend, int total, SumTables t,
+                            double *__restrict__ mass, double *__restrict__ com, int cap)
+{
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= total) return;
+    double a[4];
+    range_sum4(t, nstart[idx], nend[idx], a);
+    mass[idx] = a[0];
+    for (int k = 0; k < 3; ++k) com[(size_t)k * cap + idx] = (a[0] > 0.0) ? a[1 + k] / a[0] : 0.0;
+}
+
+} // namespace
+
+// ------------------------------------------------------------------------------
+int tree_alloc(sphgpu_ctx *c)
+{
+    Tree &t = c->tree;
+    const int n = c->n;
+    t.node_cap = 5 * n + 1;  // BHTree::resize pool of 5N nodes plus the root
+    auto al = [&](void **p, size_t b) { return hipMalloc(p, b) == hipSuccess && hipMemset(*p, 0, b) == hipSuccess; };
+    const int nb1 = (n + 63) / 64, nb2 = (nb1 + 63) / 64;
+    t.hist_blocks = (n + kSortTile - 1) / kSortTile;
+    bool ok = al((void **)&t.key, 8ull * n) && al((void **)&t.key_alt, 8ull * n) && al((void **)&t.perm, 4ull * n) &&
+              al((void **)&t.perm_alt, 4ull * n) && al((void **)&t.kdig, n) && al((void **)&t.lam, n) &&
+              al((void **)&t.leaf_end, 4ull * n) && al((void **)&t.ncount, 4ull * (n + 1)) &&
+              al((void **)&t.nbase, 4ull * (n + 1)) && al((void **)&t.start, 4ull * t.node_cap) &&
+              al((void **)&t.end, 4ull * t.node_cap) && al((void **)&t.skip, 4ull * t.node_cap) &&
+              al((void **)&t.level, 4ull * t.node_cap) && al((void **)&t.center, 24ull * t.node_cap) &&
+              al((void **)&t.edge, 8ull * t.node_cap) && al((void **)&t.ks, 8ull * t.node_cap) &&
+              al((void **)&t.mass, 8ull * t.node_cap) && al((void **)&t.com, 24ull * t.node_cap) &&
+              al((void **)&t.bmax1, 8ull * nb1) && al((void **)&t.bmax2, 8ull * nb2) &&
+              al((void **)&t.bsum1, 32ull * nb1) && al((void **)&t.bsum2, 32ull * nb2) &&
+              al((void **)&t.root, 8ull * 8) && al((void **)&t.hist, 4ull * 256 * t.hist_blocks);
+    if (!ok) return SPHGPU_E_NOMEM;
+    if (c->prm.periodic) {
+        double r[8] = {0};
+        double l = 0.0;
+        for (int k = 0; k < c->dim; ++k) {
+            r[k] = (c->prm.range_max[k] + c->prm.range_min[k]) * 0.5;
+            const double range = c->prm.range_max[k] - c->prm.range_min[k];
+            if (l < range) l = range;
+        }
+        r[3] = l;
+        if (hipMemcpy(t.root, r, sizeof(r), hipMemcpyHostToDevice) != hipSuccess) return SPHGPU_E_HIP;
+    }
+    return 0;
+}
+
+void tree_free(sphgpu_ctx *c)
+{
+    Tree &t = c->tree;
+    void *ps[] = {t.key, t.key_alt, t.perm, t.perm_alt, t.kdig, t.lam, t.leaf_end, t.ncount, t.nbase, t.start,
+                  t.end, t.skip, t.level, t.center, t.edge, t.ks, t.mass, t.com, t.bmax1, t.bmax2, t.bsum1,
+                  t.bsum2, t.root, t.hist};
+    for (void *p : ps)
+        if (p) hipFree(p);
+    t = Tree();
+}
+
+static int permute_state(sphgpu_ctx *c)
+{
+    const int n = c->n;
+    const int *perm = c->tree.perm;
+    std::vector<double **> fields;
+    for (int k = 0; k < c->dim; ++k) {
+        fields.push_back(&c->pos[k]);
+        fields.push_back(&c->vel[k]);
+        fields.push_back(&c->vel_p[k]);
+        fields.push_back(&c->acc[k]);
+    }
+    for (int s = 0; s < S_NSCALAR; ++s) fields.push_back(&c->sc[s]);
+    if (c->prm.sph_type == SPHGPU_GSPH) {
+        for (int k = 0; k < c->dim; ++k) {
+            fields.push_back(&c->grad[k]);
+            fields.push_back(&c->grad[3 + k]);
+            for (int a = 0; a < c->dim; ++a) fields.push_back(&c->grad[6 + 3 * a + k]);
+        }
+    }
+    for (size_t off = 0; off < fields.size(); off += 16) {
+        GatherBatch b;
+        b.count = (int)std::min<size_t>(16, fields.size() - off);
+        for (int f = 0; f < b.count; ++f) {
+            b.src[f] = *fields[off + f];
+            b.dst[f] = c->alt[f];
+        }
+        hipLaunchKernelGGL(k_gather_batch, dim3(grid_for(n, 256)), dim3(256), 0, c->stream, b, perm, n);
+        for (int f = 0; f < b.count; ++f) std::swap(*fields[off + f], c->alt[f]);
+    }
+    hipLaunchKernelGGL(k_gather_int2, dim3(grid_for(n, 256)), dim3(256), 0, c->stream, (const int *)c->id, c->id_alt,
+                       (const int *)c->neighbor, c->tree.perm_alt, perm, n);
+    std::swap(c->id, c->id_alt);
+    std::swap(c->neighbor, c->tree.perm_alt);
+    HIPCHK(c, hipGetLastError());
+    return 0;
+}
+
+int tree_build(sphgpu_ctx *c)
+{
+    Tree &t = c->tree;
+    const int n = c->n;
+    TreeShape sh{c->dim, c->dim, c->prm.max_tree_level, c->prm.leaf_particle_num, n};
+    const double *px = c->pos[0];
+    const double *py = c->dim > 1 ? c->pos[1] : c->pos[0];
+    const double *pz = c->dim > 2 ? c->pos[2] : c->pos[0];
+
+    time_begin(c, "tree_keys");
+    if (!c->prm.periodic) {
+        const int nbb = std::min(c->red_blocks, grid_for(n, 256));
+        hipLaunchKernelGGL(k_bbox_partial, dim3(nbb), dim3(256), 0, c->stream, px, py, pz, n, c->dim, c->red);
+        hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(64), 0, c->stream, (const double *)c->red, nbb, c->dim, t.root);
+    }
+    hipLaunchKernelGGL(k_keys, dim3(grid_for(n, 256)), dim3(256), 0, c->stream, px, py, pz, (const double *)t.root, sh,
+                       t.key, t.perm);
+    time_end(c);
+
+    // stable LSD radix sort over the dim*maxTreeLevel key bits
+    time_begin(c, "tree_sort");
+    const int key_bits = sh.bits * sh.ml;
+    const int passes = (key_bits + 7) / 8;
+    const int nb = t.hist_blocks;
+    for (int p = 0; p < passes; ++p) {
+        const int shift = 8 * p;
+        hipLaunchKernelGGL(k_radix_hist, dim3(nb), dim3(kSortThreads), 0, c->stream, (const unsigned long long *)t.key, n,
+                           shift, t.hist, nb);
+        hipLaunchKernelGGL(k_radix_scan, dim3(1), dim3(1024), 0, c->stream, t.hist, 256 * nb);
+        hipLaunchKernelGGL(k_radix_scatter, dim3(nb), dim3(kSortThreads), 0, c->stream,
+                           (const unsigned long long *)t.key, (const int *)t.perm, t.key_alt, t.perm_alt, n, shift,
+                           (const unsigned int *)t.hist, nb);
+        std::swap(t.key, t.key_alt);
+        std::swap(t.perm, t.perm_alt);
+    }
+    time_end(c);
+    HIPCHK(c, hipGetLastError());
+
+    time_begin(c, "tree_permute");
+    int rc = permute_state(c);
+    if (rc) return rc;
+    time_end(c);
+
+    time_begin(c, "tree_nodes");
+    hipLaunchKernelGGL(k_kdig, dim3(grid_for(n, 256)), dim3(256), 0, c->stream, (const unsigned long long *)t.key, sh,
+                       t.kdig);
+    hipLaunchKernelGGL(k_lambda, dim3(grid_for(n, 128)), dim3(128), 0, c->stream, (const unsigned char *)t.kdig, sh,
+                       t.lam, t.leaf_end, t.ncount);
+    hipLaunchKernelGGL(k_scan_int, dim3(1), dim3(1024), 0, c->stream, (const int *)t.ncount, t.nbase, n);
+    HIPCHK(c, hipGetLastError());
+    int total = 0;
+    HIPCHK(c, hipMemcpyAsync(&total, t.nbase + n, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (total > t.node_cap) {
+        t.valid = false;
+        return fail(c, SPHGPU_E_NO_FREE_NODE, "There is no free node.");
+    }
+    t.n_nodes = total;
+    hipLaunchKernelGGL(k_fill, dim3(grid_for(n, 128)), dim3(128), 0, c->stream, (const unsigned long long *)t.key,
+                       (const unsigned char *)t.kdig, (const unsigned char *)t.lam, (const int *)t.leaf_end,
+                       (const int *)t.nbase, (const double *)t.root, sh, total, t.start, t.end, t.skip, t.level,
+                       t.center, t.edge, t.node_cap, c->dstats);
+    HIPCHK(c, hipGetLastError());
+    // mass and centre of mass per node (needed by the gravity walk)
+    if (c->prm.use_gravity) {
+        const int nb1 = (n + 63) / 64, nb2 = (nb1 + 63) / 64;
+        hipLaunchKernelGGL(k_block_sum4, dim3(grid_for((long long)nb1 * 64, 256)), dim3(256), 0, c->stream,
+                           (const double *)c->sc[S_MASS], px, c->dim > 1 ? py : nullptr, c->dim > 2 ? pz : nullptr, n,
+                           nb1, t.bsum1, 0, 0);
+        hipLaunchKernelGGL(k_block_sum4, dim3(grid_for((long long)nb2 * 64, 256)), dim3(256), 0, c->stream,
+                           (const double *)t.bsum1, nullptr, nullptr, nullptr, nb1, nb2, t.bsum2, nb1, 1);
+        SumTables st{c->sc[S_MASS], px, c->dim > 1 ? py : nullptr, c->dim > 2 ? pz : nullptr, t.bsum1, t.bsum2, nb1, nb2};
+        hipLaunchKernelGGL(k_node_mass, dim3(grid_for(total, 128)), dim3(128), 0, c->stream, (const int *)t.start,
+                           (const int *)t.end, total, st, t.mass, t.com, t.node_cap);
+    }
+    time_end(c);
+    HIPCHK(c, hipGetLastError());
+    t.valid = true;
+    c->sorted_valid = true;
+    // the inputs of the pair loops moved: refresh their FP32 copies
+    return refresh_mirrors(c);
+}
+
+// BHTree::set_kernel: kernel_size = max smoothing length in each subtree
+int tree_set_kernel(sphgpu_ctx *c)
+{
+    Tree &t = c->tree;
+    if (!t.valid) return fail(c, SPHGPU_E_STATE, "set_kernel without a tree");
+    const int n = c->n;
+    const int nb1 = (n + 63) / 64, nb2 = (nb1 + 63) / 64;
+    time_begin(c, "tree_set_kernel");
+    hipLaunchKernelGGL(k_block_max, dim3(grid_for((long long)nb1 * 64, 256)), dim3(256), 0, c->stream,
+                       (const double *)c->sc[S_SML], n, t.bmax1);
+    hipLaunchKernelGGL(k_block_max, dim3(grid_for((long long)nb2 * 64, 256)), dim3(256), 0, c->stream,
+                       (const double *)t.bmax1, nb1, t.bmax2);
+    RangeTables rt{c->sc[S_SML], t.bmax1, t.bmax2};
+    hipLaunchKernelGGL(k_node_ks, dim3(grid_for(t.n_nodes, 128)), dim3(128), 0, c->stream, (const int *)t.start,
+                       (const int *)t.end, t.n_nodes, rt, t.ks);
+    time_end(c);
+    HIPCHK(c, hipGetLastError());
+    return 0;
+}
+
+int tree_dump(sphgpu_ctx *c, int64_t *n_nodes, int32_t *level, int32_t *count, int32_t *first, int32_t *is_leaf,
+              double *center, double *edge, double *mass, double *com, double *kernel_size, int32_t *members)
+{
+    Tree &t = c->tree;
+    if (!t.valid) return fail(c, SPHGPU_E_STATE, "tree_dump without a tree");
+    const int m = t.n_nodes;
+    *n_nodes = m;
+    if (!level) return 0;
+    std::vector<int> st(m), en(m), lv(m);
+    std::vector<double> ce(3ull * m), ed(m), ma(m), co(3ull * m), ks(m);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(st.data(), t.start, 4ull * m, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(en.data(), t.end, 4ull * m, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(lv.data(), t.level, 4ull * m, hipMemcpyDeviceToHost));
+    for (int k = 0; k < 3; ++k) {
+        HIPCHK(c, hipMemcpy(ce.data() + (size_t)k * m, t.center + (size_t)k * t.node_cap, 8ull * m, hipMemcpyDeviceToHost));
+        HIPCHK(c, hipMemcpy(co.data() + (size_t)k * m, t.com + (size_t)k * t.node_cap, 8ull * m, hipMemcpyDeviceToHost));
+    }
+    HIPCHK(c, hipMemcpy(ed.data(), t.edge, 8ull * m, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(ma.data(), t.mass, 8ull * m, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(ks.data(), t.ks, 8ull * m, hipMemcpyDeviceToHost));
+    std::vector<int> ids(c->n);
+    HIPCHK(c, hipMemcpy(ids.data(), c->id, 4ull * c->n, hipMemcpyDeviceToHost));
+    for (int i = 0; i < m; ++i) {
+        level[i] = lv[i] < 0 ? -lv[i] : lv[i];
+        is_leaf[i] = lv[i] < 0 ? 1 : 0;
+        count[i] = en[i] - st[i];
+        first[i] = st[i];
+        for (int k = 0; k < 3; ++k) {
+            center[3 * i + k] = ce[(size_t)k * m + i];
+            com[3 * i + k] = co[(size_t)k * m + i];
+        }
+        edge[i] = ed[i];
+        mass[i] = ma[i];
+        kernel_size[i] = ks[i];
+    }
+    for (int k = 0; k < c->n; ++k) members[k] = ids[k];
+    return 0;
+}
+
+} // namespace sphgpu

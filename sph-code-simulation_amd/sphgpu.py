@@ -1,0 +1,346 @@
+"""ctypes binding of libsphgpu.so, the C-ABI declared in include/sphgpu.h.
+
+This is plumbing for Python callers (tests, bench, smoke): every method calls
+straight into the HIP library, and loading fails loudly when the library has
+not been built.  There is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+PKG = Path(__file__).resolve().parent
+LIB_PATH = PKG / "libsphgpu.so"
+NFIELDS = 41
+
+
+class Params(C.Structure):
+    """sphgpu_params (include/sphgpu.h)."""
+    _fields_ = [
+        ("dim", C.c_int32), ("sph_type", C.c_int32), ("kernel", C.c_int32),
+        ("riemann_solver", C.c_int32), ("neighbor_number", C.c_int32),
+        ("leaf_particle_num", C.c_int32), ("max_tree_level", C.c_int32),
+        ("iterative_sml", C.c_int32), ("use_balsara_switch", C.c_int32),
+        ("use_time_dependent_av", C.c_int32), ("use_ac", C.c_int32),
+        ("periodic", C.c_int32), ("use_gravity", C.c_int32),
+        ("gsph_2nd_order", C.c_int32), ("pair_precision", C.c_int32),
+        ("riemann_guess", C.c_int32),
+        ("gamma", C.c_double), ("cfl_sound", C.c_double), ("cfl_force", C.c_double),
+        ("av_alpha", C.c_double), ("alpha_max", C.c_double), ("alpha_min", C.c_double),
+        ("epsilon_av", C.c_double), ("alpha_ac", C.c_double),
+        ("gravity_constant", C.c_double), ("theta", C.c_double),
+        ("range_min", C.c_double * 3), ("range_max", C.c_double * 3),
+    ]
+
+
+class Stats(C.Structure):
+    """sphgpu_stats (include/sphgpu.h)."""
+    _fields_ = [(n, C.c_int64) for n in (
+        "pre_candidates", "pre_newton_iters", "pre_newton_fail", "force_pairs",
+        "force_candidates", "riemann_iters", "riemann_wave_max_iters",
+        "riemann_calls", "riemann_fallbacks", "riemann_learned",
+        "gravity_interactions", "nodes", "leaves")] + [
+        ("max_depth", C.c_int32), ("pad", C.c_int32)]
+
+    def as_dict(self) -> dict:
+        return {n: getattr(self, n) for n, _ in self._fields_ if n != "pad"}
+
+
+def params_struct(d: dict) -> Params:
+    p = Params()
+    for name, _ in Params._fields_:
+        if name in ("range_min", "range_max"):
+            v = list(d.get(name, [0.0, 0.0, 0.0])) + [0.0, 0.0, 0.0]
+            setattr(p, name, (C.c_double * 3)(*v[:3]))
+        else:
+            setattr(p, name, type(getattr(p, name))(d[name]))
+    return p
+
+
+class SphGpuError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def load_library() -> C.CDLL:
+    """Load libsphgpu.so from the package directory (built in-tree)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise SphGpuError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+    lib = C.CDLL(str(LIB_PATH), mode=os.RTLD_GLOBAL)
+    vp, ip, dp, i32p, i64p = C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int32), C.POINTER(C.c_int64)
+    fp = C.POINTER(C.c_float)
+    sigs = {
+        "sphgpu_params_default": (None, [C.POINTER(Params), ip]),
+        "sphgpu_create": (ip, [C.POINTER(Params), ip, C.POINTER(vp)]),
+        "sphgpu_destroy": (ip, [vp]),
+        "sphgpu_last_error": (C.c_char_p, [vp]),
+        "sphgpu_device_info": (ip, [C.c_char_p, ip, C.POINTER(C.c_int), dp]),
+        "sphgpu_upload": (ip, [vp, dp, C.c_uint64]),
+        "sphgpu_download": (ip, [vp, dp, C.c_uint64]),
+        "sphgpu_set_dt": (ip, [vp, C.c_double]),
+        "sphgpu_get_dt": (ip, [vp, dp]),
+        "sphgpu_set_time": (ip, [vp, C.c_double]),
+        "sphgpu_get_time": (ip, [vp, dp]),
+        "sphgpu_get_h_per_v_sig": (ip, [vp, dp]),
+        "sphgpu_build_tree": (ip, [vp]),
+        "sphgpu_pre_interaction": (ip, [vp]),
+        "sphgpu_fluid_force": (ip, [vp]),
+        "sphgpu_gravity": (ip, [vp]),
+        "sphgpu_timestep": (ip, [vp]),
+        "sphgpu_predict": (ip, [vp]),
+        "sphgpu_correct": (ip, [vp]),
+        "sphgpu_initialize": (ip, [vp]),
+        "sphgpu_integrate": (ip, [vp, ip]),
+        "sphgpu_energy_momentum": (ip, [vp, dp]),
+        "sphgpu_get_stats": (ip, [vp, C.POINTER(Stats)]),
+        "sphgpu_reset_stats": (ip, [vp]),
+        "sphgpu_neighbor_sets": (ip, [vp, dp, ip, ip, i32p, i32p]),
+        "sphgpu_tree_dump": (ip, [vp, i64p, i32p, i32p, i32p, i32p, dp, dp, dp, dp, dp, i32p]),
+        "sphgpu_riemann_batch": (ip, [vp, ip, ip, ip, dp, dp, dp, dp, i32p]),
+        "sphgpu_riemann_log_enable": (ip, [vp, C.c_int64, ip]),
+        "sphgpu_riemann_log_read": (ip, [vp, fp, C.c_int64, i64p]),
+        "sphgpu_mlp_init": (ip, [vp, C.c_uint64]),
+        "sphgpu_mlp_set_params": (ip, [vp, fp, C.c_int64]),
+        "sphgpu_mlp_get_params": (ip, [vp, fp, C.c_int64]),
+        "sphgpu_mlp_param_count": (C.c_int64, []),
+        "sphgpu_mlp_train": (ip, [vp, fp, C.c_int64, ip, ip, C.c_float, fp]),
+        "sphgpu_mlp_loss_grad": (ip, [vp, fp, C.c_int64, fp, fp]),
+        "sphgpu_mlp_predict": (ip, [vp, fp, C.c_int64, fp]),
+        "sphgpu_kernel_times": (ip, [vp, C.c_char_p, ip, dp, i64p, ip, C.POINTER(C.c_int)]),
+        "sphgpu_enable_timing": (ip, [vp, ip]),
+    }
+    for name, (res, args) in sigs.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+EXPORTED = (
+    "sphgpu_params_default", "sphgpu_create", "sphgpu_destroy", "sphgpu_last_error",
+    "sphgpu_device_info", "sphgpu_upload", "sphgpu_download", "sphgpu_set_dt",
+    "sphgpu_get_dt", "sphgpu_set_time", "sphgpu_get_time", "sphgpu_get_h_per_v_sig",
+    "sphgpu_build_tree", "sphgpu_pre_interaction", "sphgpu_fluid_force",
+    "sphgpu_gravity", "sphgpu_timestep", "sphgpu_predict", "sphgpu_correct",
+    "sphgpu_initialize", "sphgpu_integrate", "sphgpu_energy_momentum",
+    "sphgpu_get_stats", "sphgpu_reset_stats", "sphgpu_neighbor_sets",
+    "sphgpu_tree_dump", "sphgpu_riemann_batch", "sphgpu_riemann_log_enable",
+    "sphgpu_riemann_log_read", "sphgpu_mlp_init", "sphgpu_mlp_set_params",
+    "sphgpu_mlp_get_params", "sphgpu_mlp_param_count", "sphgpu_mlp_train",
+    "sphgpu_mlp_loss_grad", "sphgpu_mlp_predict", "sphgpu_kernel_times",
+    "sphgpu_enable_timing",
+)
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class Context:
+    """One device-resident simulation (sphgpu_ctx)."""
+
+    def __init__(self, params: dict, n: int):
+        self.lib = load_library()
+        self.params = dict(params)
+        self.n = int(n)
+        self._p = params_struct(params)
+        h = C.c_void_p()
+        rc = self.lib.sphgpu_create(C.byref(self._p), self.n, C.byref(h))
+        if rc != 0:
+            raise SphGpuError(f"sphgpu_create failed ({rc})")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.sphgpu_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, what):
+        if rc != 0:
+            msg = self.lib.sphgpu_last_error(self.h)
+            raise SphGpuError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+
+    def upload(self, fields: np.ndarray, mask: int = (1 << NFIELDS) - 1):
+        f = np.ascontiguousarray(fields, dtype=np.float64)
+        assert f.shape == (NFIELDS, self.n), f.shape
+        self._check(self.lib.sphgpu_upload(self.h, _dp(f), mask), "upload")
+
+    def download(self, mask: int = (1 << NFIELDS) - 1) -> np.ndarray:
+        f = np.zeros((NFIELDS, self.n), dtype=np.float64)
+        self._check(self.lib.sphgpu_download(self.h, _dp(f), mask), "download")
+        return f
+
+    def _scalar(self, fn):
+        v = C.c_double(0)
+        self._check(fn(self.h, C.byref(v)), fn.__name__)
+        return v.value
+
+    @property
+    def dt(self): return self._scalar(self.lib.sphgpu_get_dt)
+
+    @dt.setter
+    def dt(self, v): self._check(self.lib.sphgpu_set_dt(self.h, float(v)), "set_dt")
+
+    @property
+    def time(self): return self._scalar(self.lib.sphgpu_get_time)
+
+    @time.setter
+    def time(self, v): self._check(self.lib.sphgpu_set_time(self.h, float(v)), "set_time")
+
+    @property
+    def h_per_v_sig(self): return self._scalar(self.lib.sphgpu_get_h_per_v_sig)
+
+    def build_tree(self): self._check(self.lib.sphgpu_build_tree(self.h), "build_tree")
+    def pre_interaction(self): self._check(self.lib.sphgpu_pre_interaction(self.h), "pre_interaction")
+    def fluid_force(self): self._check(self.lib.sphgpu_fluid_force(self.h), "fluid_force")
+    def gravity(self): self._check(self.lib.sphgpu_gravity(self.h), "gravity")
+    def timestep(self): self._check(self.lib.sphgpu_timestep(self.h), "timestep")
+    def predict(self): self._check(self.lib.sphgpu_predict(self.h), "predict")
+    def correct(self): self._check(self.lib.sphgpu_correct(self.h), "correct")
+    def initialize(self): self._check(self.lib.sphgpu_initialize(self.h), "initialize")
+    def integrate(self, n: int = 1): self._check(self.lib.sphgpu_integrate(self.h, int(n)), "integrate")
+
+    def energy_momentum(self) -> np.ndarray:
+        out = np.zeros(7)
+        self._check(self.lib.sphgpu_energy_momentum(self.h, _dp(out)), "energy_momentum")
+        return out
+
+    def stats(self) -> dict:
+        s = Stats()
+        self._check(self.lib.sphgpu_get_stats(self.h, C.byref(s)), "get_stats")
+        return s.as_dict()
+
+    def reset_stats(self): self._check(self.lib.sphgpu_reset_stats(self.h), "reset_stats")
+
+    def neighbor_sets(self, h: np.ndarray, is_ij: bool, cap: int = 512):
+        h = np.ascontiguousarray(h, dtype=np.float64)
+        counts = np.zeros(self.n, dtype=np.int32)
+        lists = np.zeros(self.n * cap, dtype=np.int32)
+        self._check(self.lib.sphgpu_neighbor_sets(
+            self.h, _dp(h), int(is_ij), int(cap),
+            counts.ctypes.data_as(C.POINTER(C.c_int32)),
+            lists.ctypes.data_as(C.POINTER(C.c_int32))), "neighbor_sets")
+        return counts, lists.reshape(self.n, cap)
+
+    def tree_dump(self) -> dict:
+        nn = C.c_int64(0)
+        self._check(self.lib.sphgpu_tree_dump(self.h, C.byref(nn), *([None] * 11)), "tree_dump")
+        m = nn.value
+        i32 = lambda k: np.zeros(k, dtype=np.int32)  # noqa: E731
+        out = dict(level=i32(m), count=i32(m), first=i32(m), is_leaf=i32(m),
+                   center=np.zeros(3 * m), edge=np.zeros(m), mass=np.zeros(m),
+                   com=np.zeros(3 * m), kernel_size=np.zeros(m), members=i32(self.n))
+        ip = lambda a: a.ctypes.data_as(C.POINTER(C.c_int32))  # noqa: E731
+        self._check(self.lib.sphgpu_tree_dump(
+            self.h, C.byref(nn), ip(out["level"]), ip(out["count"]), ip(out["first"]),
+            ip(out["is_leaf"]), _dp(out["center"]), _dp(out["edge"]), _dp(out["mass"]),
+            _dp(out["com"]), _dp(out["kernel_size"]), ip(out["members"])), "tree_dump")
+        out["center"] = out["center"].reshape(m, 3)
+        out["com"] = out["com"].reshape(m, 3)
+        return out
+
+    def riemann_batch(self, solver: int, precision: int, left: np.ndarray, right: np.ndarray):
+        left = np.ascontiguousarray(left, dtype=np.float64)
+        right = np.ascontiguousarray(right, dtype=np.float64)
+        n = left.shape[0]
+        assert left.shape == (n, 4) and right.shape == (n, 4)
+        ps = np.zeros(n)
+        vs = np.zeros(n)
+        it = np.zeros(n, dtype=np.int32)
+        self._check(self.lib.sphgpu_riemann_batch(
+            self.h, int(solver), int(precision), n, _dp(left), _dp(right), _dp(ps), _dp(vs),
+            it.ctypes.data_as(C.POINTER(C.c_int32))), "riemann_batch")
+        return ps, vs, it
+
+    # ---- learned Riemann guess -------------------------------------------
+    def riemann_log_enable(self, capacity: int, stride: int = 1):
+        self._check(self.lib.sphgpu_riemann_log_enable(self.h, int(capacity), int(stride)), "log_enable")
+
+    def riemann_log_read(self, max_rows: int) -> np.ndarray:
+        rows = np.zeros((max_rows, 7), dtype=np.float32)
+        n = C.c_int64(0)
+        self._check(self.lib.sphgpu_riemann_log_read(
+            self.h, rows.ctypes.data_as(C.POINTER(C.c_float)), int(max_rows), C.byref(n)), "log_read")
+        return rows[:n.value].copy()
+
+    def mlp_init(self, seed: int = 1):
+        self._check(self.lib.sphgpu_mlp_init(self.h, int(seed)), "mlp_init")
+
+    def mlp_param_count(self) -> int:
+        return int(self.lib.sphgpu_mlp_param_count())
+
+    def mlp_get_params(self) -> np.ndarray:
+        n = self.mlp_param_count()
+        a = np.zeros(n, dtype=np.float32)
+        self._check(self.lib.sphgpu_mlp_get_params(self.h, a.ctypes.data_as(C.POINTER(C.c_float)), n), "mlp_get")
+        return a
+
+    def mlp_set_params(self, a: np.ndarray):
+        a = np.ascontiguousarray(a, dtype=np.float32)
+        self._check(self.lib.sphgpu_mlp_set_params(self.h, a.ctypes.data_as(C.POINTER(C.c_float)), a.size), "mlp_set")
+
+    def mlp_train(self, rows: np.ndarray, epochs: int, batch: int, lr: float) -> float:
+        rows = np.ascontiguousarray(rows, dtype=np.float32)
+        loss = C.c_float(0)
+        self._check(self.lib.sphgpu_mlp_train(
+            self.h, rows.ctypes.data_as(C.POINTER(C.c_float)), rows.shape[0], int(epochs),
+            int(batch), float(lr), C.byref(loss)), "mlp_train")
+        return loss.value
+
+    def mlp_loss_grad(self, rows: np.ndarray):
+        rows = np.ascontiguousarray(rows, dtype=np.float32)
+        loss = C.c_float(0)
+        g = np.zeros(self.mlp_param_count(), dtype=np.float32)
+        self._check(self.lib.sphgpu_mlp_loss_grad(
+            self.h, rows.ctypes.data_as(C.POINTER(C.c_float)), rows.shape[0], C.byref(loss),
+            g.ctypes.data_as(C.POINTER(C.c_float))), "mlp_loss_grad")
+        return loss.value, g
+
+    def mlp_predict(self, rows: np.ndarray) -> np.ndarray:
+        rows = np.ascontiguousarray(rows, dtype=np.float32)
+        out = np.zeros(rows.shape[0], dtype=np.float32)
+        self._check(self.lib.sphgpu_mlp_predict(
+            self.h, rows.ctypes.data_as(C.POINTER(C.c_float)), rows.shape[0],
+            out.ctypes.data_as(C.POINTER(C.c_float))), "mlp_predict")
+        return out
+
+    # ---- timing ------------------------------------------------------------
+    def enable_timing(self, on: bool = True):
+        self._check(self.lib.sphgpu_enable_timing(self.h, int(on)), "enable_timing")
+
+    def kernel_times(self) -> dict:
+        buf = C.create_string_buffer(8192)
+        avg = np.zeros(128)
+        cnt = np.zeros(128, dtype=np.int64)
+        nk = C.c_int(0)
+        self._check(self.lib.sphgpu_kernel_times(
+            self.h, buf, 8192, _dp(avg), cnt.ctypes.data_as(C.POINTER(C.c_int64)), 128, C.byref(nk)),
+            "kernel_times")
+        names = buf.value.decode().split(";")[:nk.value]
+        return {nm: (float(avg[i]), int(cnt[i])) for i, nm in enumerate(names)}
+
+
+def device_info() -> dict:
+    lib = load_library()
+    name = C.create_string_buffer(256)
+    cu = C.c_int(0)
+    hbm = C.c_double(0)
+    rc = lib.sphgpu_device_info(name, 256, C.byref(cu), C.byref(hbm))
+    if rc != 0:
+        raise SphGpuError(f"device_info failed ({rc})")
+    return {"name": name.value.decode(), "cu": cu.value, "hbm_gb": hbm.value}

@@ -1,0 +1,213 @@
+"""GPU parity tests: the HIP library through its C-ABI against the oracle.
+
+Every test runs the same seeded input through the oracle (the CPU
+restatement, itself pinned to the reference) and through libsphgpu.so.
+
+Bars:
+  * tree topology, leaf membership, node centres/edges, kernel_size and
+    neighbour sets: bit-exact;
+  * FP64 pair mode: fields within 1e-9 relative (summation order and the
+    device pow differ from the reference; nothing else does);
+  * FP32 pair mode (compensated sums): densities and smoothing lengths within
+    5e-5 relative, accelerations and du/dt within 5e-4 of the field's largest
+    magnitude, per workload below.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import ics
+from checkers import Oracle
+from parity_util import SMALL, force_pairs, rel_err, scale_err, vec_err
+
+pytestmark = pytest.mark.gpu
+
+sphgpu = pytest.importorskip("sphgpu")
+
+
+def gpu_ctx(params, fields, **over):
+    p = dict(params)
+    p.update(over)
+    c = sphgpu.Context(p, fields.shape[1])
+    c.upload(fields)
+    return c
+
+
+def guess_h(params, fields):
+    """A smoothing length per particle of the right size (for tree tests)."""
+    d = params["dim"]
+    a = {1: 2.0, 2: np.pi, 3: 4.0 * np.pi / 3.0}[d]
+    h = (params["neighbor_number"] * fields[ics.F_MASS] / (fields[ics.F_DENS] * a)) ** (1.0 / d)
+    rng = np.random.default_rng(7)
+    return h * (1.0 + 0.1 * rng.random(h.shape))
+
+
+@pytest.mark.parametrize("case", list(SMALL))
+def test_tree_topology(case):
+    p, f = SMALL[case]()
+    f = f.copy()
+    f[ics.F_SML] = guess_h(p, f)
+    o = Oracle(p, f)
+    o.make_tree()
+    o.set_kernel()
+    rec, mem = o.tree_dump()
+    g = gpu_ctx(p, f, pair_precision=ics.PAIR_FP64)
+    g.build_tree()
+    g.neighbor_sets(f[ics.F_SML], False)  # runs set_kernel with these h
+    d = g.tree_dump()
+    print(f"{case}: oracle nodes {len(rec)}, gpu nodes {len(d['level'])}")
+    assert len(rec) == len(d["level"])
+    np.testing.assert_array_equal(rec[:, 0].astype(int), d["level"])
+    np.testing.assert_array_equal(rec[:, 1].astype(int), d["count"])
+    np.testing.assert_array_equal(rec[:, 2].astype(int), d["is_leaf"])
+    np.testing.assert_array_equal(rec[:, 3:6], d["center"])
+    np.testing.assert_array_equal(rec[:, 6], d["edge"])
+    # oracle record: level, num, is_leaf, centre[3], edge, mass, com[3],
+    # kernel_size, member offset
+    np.testing.assert_array_equal(rec[:, 11], d["kernel_size"])
+    if p["use_gravity"]:
+        np.testing.assert_allclose(d["mass"], rec[:, 7], rtol=1e-12, atol=0)
+        np.testing.assert_allclose(d["com"], rec[:, 8:11], rtol=0, atol=1e-12)
+
+
+@pytest.mark.parametrize("case", list(SMALL))
+def test_leaf_members(case):
+    p, f = SMALL[case]()
+    f = f.copy()
+    f[ics.F_SML] = guess_h(p, f)
+    o = Oracle(p, f)
+    o.make_tree()
+    rec, mem = o.tree_dump()
+    g = gpu_ctx(p, f, pair_precision=ics.PAIR_FP64)
+    g.build_tree()
+    d = g.tree_dump()
+    o_leaf = np.nonzero(rec[:, 2] == 1)[0]
+    g_leaf = np.nonzero(d["is_leaf"] == 1)[0]
+    np.testing.assert_array_equal(o_leaf, g_leaf)
+    for k in o_leaf:
+        cnt = int(rec[k, 1])
+        off = int(rec[k, 12])
+        om = np.sort(mem[off:off + cnt])
+        s = d["first"][k]
+        gm = np.sort(d["members"][s:s + cnt])
+        np.testing.assert_array_equal(om, gm)
+
+
+@pytest.mark.parametrize("case", list(SMALL))
+@pytest.mark.parametrize("is_ij", [False, True])
+def test_neighbor_sets_exact(case, is_ij):
+    p, f = SMALL[case]()
+    f = f.copy()
+    f[ics.F_SML] = guess_h(p, f)
+    o = Oracle(p, f)
+    o.make_tree()
+    o.set_kernel()
+    g = gpu_ctx(p, f, pair_precision=ics.PAIR_FP64)
+    g.build_tree()
+    counts, lists = g.neighbor_sets(f[ics.F_SML], is_ij, cap=1024)
+    n = f.shape[1]
+    idx = np.arange(n) if n <= 5000 else np.random.default_rng(1).choice(n, 5000, replace=False)
+    total = 0
+    for i in idx:
+        ref = o.neighbor_search(int(i), is_ij)
+        if is_ij:
+            ref = force_pairs(p, f, int(i), ref)
+        got = lists[i, :counts[i]]
+        assert counts[i] == len(ref), (case, i, counts[i], len(ref))
+        np.testing.assert_array_equal(np.sort(got), np.sort(ref))
+        total += len(ref)
+    print(f"{case} is_ij={is_ij}: {len(idx)} particles, {total} neighbours, all sets identical")
+
+
+def _compare_fields(case, p, gf, of, prec):
+    d = p["dim"]
+    t = p["sph_type"]
+    if prec == ics.PAIR_FP64:
+        tol_rho, tol_vec = 1e-9, 1e-8
+    else:
+        tol_rho, tol_vec = 5e-5, 5e-4
+    e_sml = rel_err(gf[ics.F_SML], of[ics.F_SML])
+    e_rho = rel_err(gf[ics.F_DENS], of[ics.F_DENS])
+    e_acc = vec_err(gf, of, ics.F_ACC, d)
+    e_dene = scale_err(gf[ics.F_DENE], of[ics.F_DENE])
+    e_pres = rel_err(gf[ics.F_PRES], of[ics.F_PRES])
+    print(f"{case} prec={prec}: sml {e_sml.max():.3e} (p99.9 {np.quantile(e_sml, 0.999):.3e}) "
+          f"dens {e_rho.max():.3e} pres {e_pres.max():.3e} acc {e_acc:.3e} dene {e_dene:.3e}")
+    nbad = int((e_sml > tol_rho).sum())
+    assert nbad <= max(1, int(1e-3 * len(e_sml))), f"{nbad} smoothing lengths off"
+    assert np.quantile(e_rho, 0.999) < tol_rho
+    assert np.quantile(e_pres, 0.999) < tol_rho
+    assert e_acc < tol_vec * 10 if prec == ics.PAIR_FP32 else e_acc < tol_vec
+    assert e_dene < tol_vec * 10 if prec == ics.PAIR_FP32 else e_dene < tol_vec
+    if t != ics.GSPH:
+        e_gh = rel_err(gf[ics.F_GRADH], of[ics.F_GRADH])
+        assert np.quantile(e_gh, 0.999) < tol_rho * 10
+    np.testing.assert_array_equal(gf[ics.F_ID], of[ics.F_ID])
+
+
+@pytest.mark.parametrize("case", list(SMALL))
+@pytest.mark.parametrize("prec", [ics.PAIR_FP64, ics.PAIR_FP32])
+def test_initialize_fields(case, prec):
+    p, f = SMALL[case]()
+    o = Oracle(p, f)
+    o.initialize()
+    of = o.get_fields()
+    g = gpu_ctx(p, f, pair_precision=prec)
+    g.initialize()
+    gf = g.download()
+    _compare_fields(case, p, gf, of, prec)
+    # neighbour counts are exact whenever h is (FP64 mode)
+    if prec == ics.PAIR_FP64:
+        same_h = gf[ics.F_SML] == of[ics.F_SML]
+        mism = (gf[ics.F_NEIGHBOR] != of[ics.F_NEIGHBOR]) & same_h
+        assert mism.sum() == 0
+
+
+@pytest.mark.parametrize("case", list(SMALL))
+@pytest.mark.parametrize("prec", [ics.PAIR_FP64, ics.PAIR_FP32])
+def test_short_run(case, prec):
+    p, f = SMALL[case]()
+    steps = 5
+    o = Oracle(p, f)
+    o.initialize()
+    e0 = o.energy()
+    o.integrate(steps)
+    of = o.get_fields()
+    g = gpu_ctx(p, f, pair_precision=prec)
+    g.initialize()
+    g.integrate(steps)
+    gf = g.download()
+    eo = o.energy()
+    eg = g.energy_momentum()
+    print(f"{case} prec={prec}: dt oracle {o.dt:.6e} gpu {g.dt:.6e}; t {o.time:.6e} {g.time:.6e}")
+    print(f"  E oracle {eo[3]:.12e} gpu {eg[3]:.12e} (E0 {e0[3]:.12e})")
+    assert abs(g.dt - o.dt) <= (1e-9 if prec == ics.PAIR_FP64 else 1e-4) * o.dt
+    _compare_fields(case, p, gf, of, prec)
+    drift_o = (eo[3] - e0[3]) / abs(e0[3])
+    drift_g = (eg[3] - e0[3]) / abs(e0[3])
+    assert abs(drift_g - drift_o) < (1e-10 if prec == ics.PAIR_FP64 else 1e-5)
+
+
+def test_riemann_batch_matches_oracle():
+    from checkers import oracle_riemann
+    rng = np.random.default_rng(3)
+    n = 2000
+    L = np.stack([rng.normal(0, 1, n), rng.uniform(0.1, 2, n), 10 ** rng.uniform(-3, 1, n), np.zeros(n)], 1)
+    R = np.stack([rng.normal(0, 1, n), rng.uniform(0.1, 2, n), 10 ** rng.uniform(-3, 1, n), np.zeros(n)], 1)
+    gam = 5.0 / 3.0
+    L[:, 3] = np.sqrt(gam * L[:, 2] / L[:, 1])
+    R[:, 3] = np.sqrt(gam * R[:, 2] / R[:, 1])
+    p, f = ics.sedov_3d(n1=4)
+    g = gpu_ctx(p, f)
+    for solver in (ics.RIEMANN_HLL, ics.RIEMANN_ITERATIVE):
+        ps64, vs64, it64 = g.riemann_batch(solver, ics.PAIR_FP64, L, R)
+        ps32, vs32, it32 = g.riemann_batch(solver, ics.PAIR_FP32, L, R)
+        ref = np.array([oracle_riemann(solver, gam, L[k], R[k])[:2] for k in range(n)])
+        e64 = rel_err(ps64, ref[:, 0]).max()
+        e32 = rel_err(ps32, ref[:, 0]).max()
+        ev = np.abs(vs64 - ref[:, 1]).max()
+        print(f"solver {solver}: p* fp64 {e64:.3e} fp32 {e32:.3e} v* fp64 {ev:.3e}")
+        assert e64 < 1e-13
+        assert ev < 1e-12
+        assert e32 < 1e-4
