@@ -145,6 +145,8 @@ typedef struct sphgpu_stats {
     int64_t gravity_interactions;/* particle-particle + particle-node terms    */
     int64_t nodes;               /* tree nodes (root included)                */
     int64_t leaves;
+    int64_t search_node_visits;  /* tree nodes visited by neighbour-search wavefronts */
+    int64_t search_member_tests; /* leaf members tested per wavefront (summed) */
     int32_t max_depth;           /* deepest node level (root = 1)             */
     int32_t pad;
 } sphgpu_stats;
@@ -164,9 +166,15 @@ int sphgpu_get_dt(sphgpu_ctx *ctx, double *dt);
 int sphgpu_set_time(sphgpu_ctx *ctx, double t);
 int sphgpu_get_time(sphgpu_ctx *ctx, double *t);
 int sphgpu_get_h_per_v_sig(sphgpu_ctx *ctx, double *h_per_v_sig);
+/* Simulation::set_h_per_v_sig (include/simulation.hpp:31), for drivers that
+ * run PreInteraction elsewhere and TimeStep here. */
+int sphgpu_set_h_per_v_sig(sphgpu_ctx *ctx, double h_per_v_sig);
 
 /* ---- modules (drop-ins for Module::calculation) -------------------------- */
 int sphgpu_build_tree(sphgpu_ctx *ctx);
+/* BHTree::set_kernel (src/bhtree.cpp:117-120): kernel_size of every node from
+ * the current smoothing lengths; PreInteraction calls it itself. */
+int sphgpu_set_kernel(sphgpu_ctx *ctx);
 int sphgpu_pre_interaction(sphgpu_ctx *ctx);   /* includes set_kernel       */
 int sphgpu_fluid_force(sphgpu_ctx *ctx);
 int sphgpu_gravity(sphgpu_ctx *ctx);

@@ -365,6 +365,7 @@ int sphgpu_get_dt(sphgpu_ctx *c, double *dt) { return get_scalar(c, 0, dt); }
 int sphgpu_set_time(sphgpu_ctx *c, double t) { return set_scalar(c, 1, t); }
 int sphgpu_get_time(sphgpu_ctx *c, double *t) { return get_scalar(c, 1, t); }
 int sphgpu_get_h_per_v_sig(sphgpu_ctx *c, double *v) { return get_scalar(c, 2, v); }
+int sphgpu_set_h_per_v_sig(sphgpu_ctx *c, double v) { return set_scalar(c, 2, v); }
 
 #define CTX_CALL(expr)                                   \
     do {                                                 \
@@ -377,6 +378,7 @@ int sphgpu_get_h_per_v_sig(sphgpu_ctx *c, double *v) { return get_scalar(c, 2, v
     } while (0)
 
 int sphgpu_build_tree(sphgpu_ctx *c) { CTX_CALL(tree_build(c)); }
+int sphgpu_set_kernel(sphgpu_ctx *c) { CTX_CALL(tree_set_kernel(c)); }
 int sphgpu_pre_interaction(sphgpu_ctx *c) { CTX_CALL(pre_interaction(c)); }
 int sphgpu_fluid_force(sphgpu_ctx *c) { CTX_CALL(fluid_force(c)); }
 int sphgpu_gravity(sphgpu_ctx *c) { CTX_CALL(gravity(c)); }
@@ -441,6 +443,8 @@ int sphgpu_get_stats(sphgpu_ctx *c, sphgpu_stats *out)
     out->nodes = c->tree.n_nodes;
     out->leaves = h[11];
     out->max_depth = (int32_t)h[12];
+    out->search_node_visits = h[13];
+    out->search_member_tests = h[14];
     return 0;
 }
 

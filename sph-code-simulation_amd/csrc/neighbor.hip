@@ -101,7 +101,9 @@ __global__ __launch_bounds__(256) void k_search(SearchArgs a)
     int count = 0;
     bool over = false;
     int node = 0;
+    long long visits = 0, tests = 0;
     while (node < a.n_nodes) {
+        ++visits;
         const double e = a.nedge[node];
         double rn = Rg;
         if (IS_IJ) {
@@ -127,6 +129,7 @@ __global__ __launch_bounds__(256) void k_search(SearchArgs a)
             continue;
         }
         const int s = a.nstart[node], en = a.nend[node];
+        tests += en - s;
         for (int j = s; j < en; ++j) {
             double xj[3] = {0, 0, 0};
 #pragma unroll
@@ -153,7 +156,11 @@ __global__ __launch_bounds__(256) void k_search(SearchArgs a)
     // candidate statistics
     long long c = valid ? count : 0;
     for (int off = kWave / 2; off > 0; off >>= 1) c += __shfl_xor(c, off, kWave);
-    if (lane == 0) atomicAdd((unsigned long long *)&a.stats[IS_IJ ? 4 : 0], (unsigned long long)c);
+    if (lane == 0) {
+        atomicAdd((unsigned long long *)&a.stats[IS_IJ ? 4 : 0], (unsigned long long)c);
+        atomicAdd((unsigned long long *)&a.stats[13], (unsigned long long)visits);
+        atomicAdd((unsigned long long *)&a.stats[14], (unsigned long long)tests);
+    }
 }
 
 } // namespace

@@ -667,7 +667,11 @@ int tree_build(sphgpu_ctx *c)
                        (const int *)t.nbase, (const double *)t.root, sh, total, t.start, t.end, t.skip, t.level,
                        t.center, t.edge, t.node_cap, c->dstats);
     HIPCHK(c, hipGetLastError());
-    // mass and centre of mass per node (needed by the gravity walk)
+    // mass and centre of mass per node (needed by the gravity walk); the
+    // positions are read from the permuted (key-ordered) arrays
+    px = c->pos[0];
+    py = c->dim > 1 ? c->pos[1] : c->pos[0];
+    pz = c->dim > 2 ? c->pos[2] : c->pos[0];
     if (c->prm.use_gravity) {
         const int nb1 = (n + 63) / 64, nb2 = (nb1 + 63) / 64;
         hipLaunchKernelGGL(k_block_sum4, dim3(grid_for((long long)nb1 * 64, 256)), dim3(256), 0, c->stream,

