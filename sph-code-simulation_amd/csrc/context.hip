@@ -58,35 +58,17 @@ void collect_times(sphgpu_ctx *c)
     c->pending.clear();
 }
 
-template <> PairView<float> pair_view<float>(sphgpu_ctx *c)
+template <> RecView<float> rec_view<float>(sphgpu_ctx *c)
 {
-    PairView<float> v;
-    for (int k = 0; k < 3; ++k) {
-        v.x[k] = c->mf[M_X + k];
-        v.v[k] = c->mf[M_VX + k];
-        v.gd[k] = c->mf[M_GD0 + k];
-        v.gp[k] = c->mf[M_GP0 + k];
-    }
-    for (int k = 0; k < 9; ++k) v.gv[k] = c->mf[M_GV0 + k];
-    v.mass = c->mf[M_MASS]; v.ene = c->mf[M_ENE]; v.sound = c->mf[M_SOUND];
-    v.sml = c->mf[M_SML]; v.dens = c->mf[M_DENS]; v.pres = c->mf[M_PRES];
-    v.gradh = c->mf[M_GRADH]; v.balsara = c->mf[M_BALSARA]; v.alpha = c->mf[M_ALPHA];
+    RecView<float> v;
+    for (int r = 0; r < R_NREC; ++r) v.r[r] = c->rec32[r];
     return v;
 }
 
-template <> PairView<double> pair_view<double>(sphgpu_ctx *c)
+template <> RecView<double> rec_view<double>(sphgpu_ctx *c)
 {
-    PairView<double> v;
-    for (int k = 0; k < 3; ++k) {
-        v.x[k] = c->pos[k];
-        v.v[k] = c->vel[k];
-        v.gd[k] = c->grad[k];
-        v.gp[k] = c->grad[3 + k];
-    }
-    for (int k = 0; k < 9; ++k) v.gv[k] = c->grad[6 + k];
-    v.mass = c->sc[S_MASS]; v.ene = c->sc[S_ENE]; v.sound = c->sc[S_SOUND];
-    v.sml = c->sc[S_SML]; v.dens = c->sc[S_DENS]; v.pres = c->sc[S_PRES];
-    v.gradh = c->sc[S_GRADH]; v.balsara = c->sc[S_BALSARA]; v.alpha = c->sc[S_ALPHA];
+    RecView<double> v;
+    for (int r = 0; r < R_NREC; ++r) v.r[r] = c->rec64[r];
     return v;
 }
 
@@ -248,7 +230,9 @@ int sphgpu_create(const sphgpu_params *p, int n, sphgpu_ctx **out)
     for (int a = 0; a < 16 && ok; ++a) ok = alloc((void **)&c->alt[a], dn);
     ok = ok && alloc((void **)&c->id, sizeof(int) * n) && alloc((void **)&c->id_alt, sizeof(int) * n) &&
          alloc((void **)&c->neighbor, sizeof(int) * n);
-    for (int m = 0; m < M_NFIELD && ok; ++m) ok = alloc((void **)&c->mf[m], sizeof(float) * (size_t)n);
+    for (int r = 0; r < R_NREC && ok; ++r) ok = alloc((void **)&c->rec32[r], sizeof(float4) * (size_t)n);
+    if (p->pair_precision == SPHGPU_PAIR_FP64)
+        for (int r = 0; r < R_NREC && ok; ++r) ok = alloc((void **)&c->rec64[r], sizeof(double4) * (size_t)n);
     ok = ok && alloc((void **)&c->hs, dn) && alloc((void **)&c->stage, dn * SPHGPU_NFIELDS);
     c->red_blocks = 1024;
     ok = ok && alloc((void **)&c->red, sizeof(double) * 8 * c->red_blocks);
@@ -284,7 +268,10 @@ int sphgpu_destroy(sphgpu_ctx *c)
     for (int g = 0; g < kNGrad; ++g) hipFree(c->grad[g]);
     for (int a = 0; a < 16; ++a) hipFree(c->alt[a]);
     hipFree(c->id); hipFree(c->id_alt); hipFree(c->neighbor);
-    for (int m = 0; m < M_NFIELD; ++m) hipFree(c->mf[m]);
+    for (int r = 0; r < R_NREC; ++r) {
+        hipFree(c->rec32[r]);
+        hipFree(c->rec64[r]);
+    }
     hipFree(c->hs); hipFree(c->stage); hipFree(c->red);
     hipFree(c->dscal); hipFree(c->dmin); hipFree(c->dstats); hipFree(c->dflag);
     if (c->hscal) hipHostFree(c->hscal);

@@ -5,9 +5,10 @@
 // memory.  `id[k]` is the reference particle index (SPHParticle::id) of the
 // particle stored at sorted position k; host exchange goes through it.
 //
-// FP64 master arrays hold everything the integrator and the host see.  FP32
-// mirrors of the per-pair inputs feed the FP32 pair loops; in FP64 pair mode
-// the pair loops read the masters directly.
+// FP64 master arrays hold everything the integrator and the host see.  The
+// pair loops read packed per-particle records (FP32, or FP64 in the FP64 pair
+// form) refreshed from the masters after each integrator stage and updated
+// in place by PreInteraction.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -27,24 +28,24 @@ enum SField {
     S_ALPHA, S_GRADH, S_PHI, S_NSCALAR
 };
 
-// FP32 mirror fields used by pair loops
-enum MField {
-    M_X, M_Y, M_Z, M_VX, M_VY, M_VZ, M_MASS, M_ENE, M_SOUND, M_SML, M_DENS,
-    M_PRES, M_GRADH, M_BALSARA, M_ALPHA,
-    M_GD0, M_GD1, M_GD2, M_GP0, M_GP1, M_GP2,
-    M_GV0, M_GV1, M_GV2, M_GV3, M_GV4, M_GV5, M_GV6, M_GV7, M_GV8,
-    M_NFIELD
-};
+// Pair records: per particle, 16-byte (FP32) or 32-byte (FP64) 4-vectors in
+// key order, so one neighbour costs a few wide loads:
+//   R_POSH   x, y, z, h          R_VELM   vx, vy, vz, m
+//   R_THERMO dens, pres, sound, u R_AUX   gradh, balsara, alpha, 0
+//   R_G0..3  GSPH gradients: (gd0 gd1 gd2 gp0) (gp1 gp2 gv0 gv1)
+//            (gv2 gv3 gv4 gv5) (gv6 gv7 gv8 0), gv index = 3*component + axis
+enum RecField { R_POSH, R_VELM, R_THERMO, R_AUX, R_G0, R_G1, R_G2, R_G3, R_NREC };
 
 // GSPH gradient rows in FP64 (grad_d[3], grad_p[3], grad_v[9])
 constexpr int kNGrad = 15;
 
-// Pointers a pair kernel reads for neighbour j, in precision T.
-template <typename T> struct PairView {
-    const T *x[3];
-    const T *v[3];
-    const T *mass, *ene, *sound, *sml, *dens, *pres, *gradh, *balsara, *alpha;
-    const T *gd[3], *gp[3], *gv[9];
+template <typename T> struct V4T;
+template <> struct V4T<float> { typedef float4 type; };
+template <> struct V4T<double> { typedef double4 type; };
+
+template <typename T> struct RecView {
+    typedef typename V4T<T>::type V4;
+    const V4 *r[R_NREC];
 };
 
 struct Tree {
@@ -108,8 +109,10 @@ struct sphgpu_ctx {
     // permutation double-buffers for the persistent fields
     double *alt[16] = {};
     int *id_alt = nullptr;
-    // FP32 mirrors
-    float *mf[sphgpu::M_NFIELD] = {};
+    // pair records: FP32 always (they also feed the search prefilter), FP64
+    // in FP64 pair mode
+    float4 *rec32[sphgpu::R_NREC] = {};
+    double4 *rec64[sphgpu::R_NREC] = {};
     // per-particle scratch
     double *hs = nullptr;        // search radius of the pre-interaction
     double *stage = nullptr;     // host-exchange staging, [NFIELDS][n]
@@ -127,7 +130,7 @@ struct sphgpu_ctx {
     sphgpu::Tree tree;
     sphgpu::Lists lpre, lforce;
     bool sorted_valid = false;   // state is in current tree order
-    bool mirror_dirty = true;    // FP32 mirrors lag the FP64 masters
+    bool mirror_dirty = true;    // pair records lag the FP64 masters
     bool pre_first = true;       // PreInteraction::m_first
     bool have_pre = false;
 
@@ -203,6 +206,6 @@ int mlp_create(sphgpu_ctx *c);
 void mlp_destroy(sphgpu_ctx *c);
 const float *mlp_device_params(sphgpu_ctx *c);
 
-template <typename T> PairView<T> pair_view(sphgpu_ctx *c);
+template <typename T> RecView<T> rec_view(sphgpu_ctx *c);
 
 } // namespace sphgpu

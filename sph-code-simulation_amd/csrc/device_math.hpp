@@ -261,6 +261,43 @@ template <typename T> struct TwoShock {
 // sqrt(r2) < h, with the correctly rounded FP64 sqrt.
 SPH_DEV bool r_less(double r2, double h) { return __builtin_sqrt(r2) < h; }
 
+// FP32 prefilter for the FP64 membership tests r^2 < h^2.  With positions
+// bounded by X (|x_k| <= X) and u = 2^-24, rounding the positions and the
+// periodic shift to FP32 moves each r_ij component by at most 4uX + 2u|r_ij|,
+// and the FP32 sum of squares adds (D+1)u r^2, so near r = h
+//   |r2_fp32 - r2_fp64| <= 8 sqrt(3) u X h + 11 u h^2   (first order).
+// The band half-width used, delta * h^2 with delta = k / h + c,
+//   k = 16 sqrt(3) u X,  c = 32 u,
+// is twice that bound (and also covers rounding h, h^2 and h^2 (1 +- delta)
+// in FP32).  Outside the band the FP32 comparison decides exactly as the
+// FP64 one would; inside it the caller evaluates the FP64 predicate.
+// root = tree root (center[3], edge); X is taken as max |center_k| + edge.
+SPH_DEV float fp32_margin_k(const double *root, int D)
+{
+    double X = 0.0;
+    for (int k = 0; k < D; ++k) {
+        const double a = root[k] < 0.0 ? -root[k] : root[k];
+        X = a > X ? a : X;
+    }
+    X += root[3];
+    // beyond 1e18 the FP32 squares could overflow: no band (always FP64)
+    return X < 1e18 ? (float)(16.0 * 1.7320508075688772 * 5.9604644775390625e-08 * X) : __builtin_inff();
+}
+constexpr float kMarginC = 32.0f * 5.9604644775390625e-08f;
+
+// band of the test r2 < h2: returns +1 (inside for certain), -1 (outside for
+// certain) or 0 (undecided: run the FP64 test)
+SPH_DEV int fp32_band(float r2, float h, float margin_k)
+{
+    const float h2 = h * h;
+    const float delta = margin_k / h + kMarginC;
+    // tiny h: the FP32 squares could be denormal
+    if (!(h2 > 1e-30f) || !(delta < 0.5f)) return 0;
+    if (r2 < h2 * (1.0f - delta)) return 1;
+    if (r2 > h2 * (1.0f + delta)) return -1;
+    return 0;
+}
+
 // Neumaier-compensated accumulator for FP32 sums (plain sum for FP64).
 template <typename T> struct Acc {
     T s, c;

@@ -45,7 +45,7 @@ template <typename T> SPH_DEV T hk_g(T r, T h)
 template <typename T> struct GravArgs {
     int n, n_nodes, node_cap;
     const double *x64[3];
-    PairView<T> pv;
+    RecView<T> rv;
     const int *nstart, *nend, *nskip, *nlevel;
     const double *com, *edge, *mass;
     double *acc[3];
@@ -61,14 +61,15 @@ __global__ __launch_bounds__(256) void k_gravity(GravArgs<T> a)
     const int i0 = blockIdx.x * blockDim.x + threadIdx.x;
     const bool valid = i0 < a.n;
     const int i = valid ? i0 : a.n - 1;
+    typedef typename V4T<T>::type V4;
+    const V4 *__restrict__ RP = a.rv.r[R_POSH];
+    const V4 *__restrict__ RV = a.rv.r[R_VELM];
     double ri[3] = {0, 0, 0};
-    T xi[3] = {0, 0, 0};
 #pragma unroll
-    for (int k = 0; k < D; ++k) {
-        ri[k] = a.x64[k][i];
-        xi[k] = a.pv.x[k][i];
-    }
-    const T hi = a.pv.sml[i];
+    for (int k = 0; k < D; ++k) ri[k] = a.x64[k][i];
+    const V4 pi_ = RP[i];
+    const T xi[3] = {pi_.x, pi_.y, pi_.z};
+    const T hi = pi_.w;
     const T G = T(a.G);
     Acc<T> phi, ac[3];
     long long inter = 0;
@@ -89,16 +90,18 @@ __global__ __launch_bounds__(256) void k_gravity(GravArgs<T> a)
                 const int s = a.nstart[node], en = a.nend[node];
                 for (int j = s; j < en; ++j) {
                     if (j == i) continue;
+                    const V4 pj = RP[j];
+                    const T xj[3] = {pj.x, pj.y, pj.z};
                     T rij[3];
                     T r2 = T(0);
 #pragma unroll
                     for (int k = 0; k < D; ++k) {
-                        rij[k] = rij_axis<T>(xi[k], a.pv.x[k][j], a.box, k);
+                        rij[k] = rij_axis<T>(xi[k], xj[k], a.box, k);
                         r2 = (k == 0) ? rij[k] * rij[k] : r2 + rij[k] * rij[k];
                     }
                     const T r = tsqrt(r2);
-                    const T hj = a.pv.sml[j];
-                    const T gm = G * a.pv.mass[j];
+                    const T hj = pj.w;
+                    const T gm = G * RV[j].w;
                     phi.add(-(gm * (hk_f<T>(r, hi) + hk_f<T>(r, hj)) * T(0.5)));
                     const T s2 = gm * (hk_g<T>(r, hi) + hk_g<T>(r, hj)) * T(0.5);
 #pragma unroll
@@ -134,7 +137,7 @@ template <typename T> void launch_gravity(sphgpu_ctx *c)
     a.n_nodes = c->tree.n_nodes;
     a.node_cap = c->tree.node_cap;
     for (int k = 0; k < 3; ++k) a.x64[k] = c->pos[k < c->dim ? k : 0];
-    a.pv = pair_view<T>(c);
+    a.rv = rec_view<T>(c);
     a.nstart = c->tree.start; a.nend = c->tree.end; a.nskip = c->tree.skip; a.nlevel = c->tree.level;
     a.com = c->tree.com; a.edge = c->tree.edge; a.mass = c->tree.mass;
     for (int k = 0; k < 3; ++k) a.acc[k] = c->acc[k];

@@ -29,32 +29,36 @@ constexpr double kPi = 3.1415926535897932384626433832795028841971693993751;
 template <int D> constexpr double area_const() { return D == 1 ? 2.0 : D == 2 ? kPi : 4.0 * kPi / 3.0; }
 
 template <typename T> struct PreArgs {
+    typedef typename V4T<T>::type V4;
     int n;
     int *idx;                     // compacted in place: members first
     const int *cnt;
-    PairView<T> pv;
+    RecView<T> rv;                // neighbour records (read)
+    T *rec_w[R_NREC];             // the same records as scalars (own-record writes)
+    float *posf_w;                // FP32 position/h records, written in FP64 mode
     const double *x64[3];
     const double *hs;
-    const double *mass64, *ene64, *sound64, *dens64;
+    const double *mass64;
     double *sml, *dens, *pres, *gradh, *balsara, *alpha;
     int *neighbor;
     double *grad[kNGrad];
-    float *mf[M_NFIELD];          // FP32 mirrors to refresh (FP32 mode), else null
     unsigned long long *hvs_min;
     const double *dscal;
     long long *stats;
     Box box;
     int nn, iterative, balsara_on, td_av, gsph2;
     double gamma, kappa, alpha_min, alpha_max, eps_av;
+    const double *root;           // tree root geometry (bounds |x| for the FP32 prefilter)
 };
 
 template <int D, typename T>
-SPH_DEV T pair_r2(const PairView<T> &pv, const T xi[3], int j, const Box &b, T d[3])
+SPH_DEV T rec_r2(const typename V4T<T>::type &pj, const T xi[3], const Box &b, T d[3])
 {
+    const T xj[3] = {pj.x, pj.y, pj.z};
     T s = T(0);
 #pragma unroll
     for (int k = 0; k < D; ++k) {
-        d[k] = rij_axis<T>(xi[k], pv.x[k][j], b, k);
+        d[k] = rij_axis<T>(xi[k], xj[k], b, k);
         s = (k == 0) ? d[k] * d[k] : s + d[k] * d[k];
     }
     return s;
@@ -88,20 +92,21 @@ SPH_DEV void wave_add_atomic(long long *dst, long long v)
 template <int D, int K, int TYPE, typename T>
 __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
 {
+    typedef typename V4T<T>::type V4;
     const int i0 = blockIdx.x * blockDim.x + threadIdx.x;
     const bool valid = i0 < a.n;
     const int i = valid ? i0 : a.n - 1;
-    const PairView<T> &pv = a.pv;
+    const V4 *__restrict__ RP = a.rv.r[R_POSH];
+    const V4 *__restrict__ RV = a.rv.r[R_VELM];
+    const V4 *__restrict__ RT = a.rv.r[R_THERMO];
     const int nc = valid ? a.cnt[i] : 0;
-    T xi[3] = {0, 0, 0}, vi[3] = {0, 0, 0};
+    const V4 pi_ = RP[i], vi_ = RV[i], ti_ = RT[i];
+    const T xi[3] = {pi_.x, pi_.y, pi_.z};
+    const T vi[3] = {vi_.x, vi_.y, vi_.z};
     double xi64[3] = {0, 0, 0};
 #pragma unroll
-    for (int k = 0; k < D; ++k) {
-        xi[k] = pv.x[k][i];
-        vi[k] = pv.v[k][i];
-        xi64[k] = a.x64[k][i];
-    }
-    const T mi = pv.mass[i], ui = pv.ene[i], ci = pv.sound[i];
+    for (int k = 0; k < D; ++k) xi64[k] = a.x64[k][i];
+    const T ui = ti_.w, ci = ti_.z;
     const double hs64 = a.hs[i];
     const double A = area_const<D>();
     long long newton_iters = 0, newton_fail = 0;
@@ -119,9 +124,9 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
             for (int q = 0; q < nc; ++q) {
                 const int j = a.idx[(size_t)q * a.n + i];
                 T d[3];
-                const T r = tsqrt(pair_r2<D, T>(pv, xi, j, a.box, d));
+                const T r = tsqrt(rec_r2<D, T>(RP[j], xi, a.box, d));
                 if (r >= h) continue;
-                const T wt = (TYPE == SPHGPU_DISPH) ? T(1) : pv.mass[j];
+                const T wt = (TYPE == SPHGPU_DISPH) ? T(1) : RV[j].w;
                 dn.add(wt * Kern<D, K, T>::w(r, h));
                 ddn.add(wt * Kern<D, K, T>::dhw(r, h));
             }
@@ -141,27 +146,34 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
         h = T(hs64);
     }
     const double h64 = (double)h;
+    // FP32 form: membership by the FP32 prefilter, FP64 inside its band
+    const float mk = (sizeof(T) == 4) ? fp32_margin_k(a.root, D) : 0.f;
 
-    // ---- density, pressure, grad-h, v_sig (members r < h, decided in FP64) ----
+    // ---- density, pressure, grad-h, v_sig (members r < h, decided exactly) ----
     Acc<T> dens_a, dhd_a, n_a, pres_a, dhp_a, dhn_a;
     T vsig = ci * T(2.0);
     int w = 0;
     for (int q = 0; q < nc; ++q) {
         const int j = a.idx[(size_t)q * a.n + i];
         T d[3];
-        const T r2 = pair_r2<D, T>(pv, xi, j, a.box, d);
+        const T r2 = rec_r2<D, T>(RP[j], xi, a.box, d);
         bool in;
-        if (sizeof(T) == 8) in = tsqrt(r2) < h;
-        else in = r_less(r2_64(a.x64, xi64, j, a.box, D), h64);
+        if (sizeof(T) == 8) {
+            in = tsqrt(r2) < h;
+        } else {
+            const int band = fp32_band((float)r2, (float)h, mk);
+            in = band > 0 || (band == 0 && r_less(r2_64(a.x64, xi64, j, a.box, D), h64));
+        }
         if (!in) continue;
         a.idx[(size_t)w * a.n + i] = j;  // compact: members first
         ++w;
         const T r = tsqrt(r2);
         const T wk = Kern<D, K, T>::w(r, h);
-        const T mj = pv.mass[j];
+        const V4 vj = RV[j];
+        const T mj = vj.w;
         if (TYPE == SPHGPU_DISPH) {
             const T dh = Kern<D, K, T>::dhw(r, h);
-            const T uj = pv.ene[j];
+            const T uj = RT[j].w;
             dens_a.add(mj * wk);
             n_a.add(wk);
             pres_a.add(mj * uj * wk);
@@ -174,10 +186,11 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
             dens_a.add(mj * wk);
         }
         if (j != i) {
+            const T vjv[3] = {vj.x, vj.y, vj.z};
             T dot = T(0);
 #pragma unroll
-            for (int k = 0; k < D; ++k) dot = (k == 0) ? d[k] * (vi[k] - pv.v[k][j]) : dot + d[k] * (vi[k] - pv.v[k][j]);
-            const T vs = ci + pv.sound[j] - T(3.0) * dot / r;
+            for (int k = 0; k < D; ++k) dot = (k == 0) ? d[k] * (vi[k] - vjv[k]) : dot + d[k] * (vi[k] - vjv[k]);
+            const T vs = ci + RT[j].z - T(3.0) * dot / r;
             if (vs > vsig) vsig = vs;
         }
     }
@@ -200,12 +213,12 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
         a.pres[i] = (double)pres;
         if (TYPE != SPHGPU_GSPH) a.gradh[i] = (double)gradh;
         a.neighbor[i] = w;
-        if (a.mf[M_SML]) {
-            a.mf[M_SML][i] = (float)h;
-            a.mf[M_DENS][i] = (float)dens;
-            a.mf[M_PRES][i] = (float)pres;
-            if (TYPE != SPHGPU_GSPH) a.mf[M_GRADH][i] = (float)gradh;
-        }
+        // own record: h, dens, pres, gradh (components other threads do not read here)
+        a.rec_w[R_POSH][4 * (size_t)i + 3] = h;
+        a.rec_w[R_THERMO][4 * (size_t)i + 0] = dens;
+        a.rec_w[R_THERMO][4 * (size_t)i + 1] = pres;
+        if (TYPE != SPHGPU_GSPH) a.rec_w[R_AUX][4 * (size_t)i + 0] = gradh;
+        if (a.posf_w) a.posf_w[4 * (size_t)i + 3] = (float)h;
     }
     wave_min_atomic(a.hvs_min, valid ? (double)(h / vsig) : 1.7976931348623157e308);
     wave_add_atomic(&a.stats[1], valid ? newton_iters : 0);
@@ -219,13 +232,15 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
         for (int q = 0; q < w; ++q) {
             const int j = a.idx[(size_t)q * a.n + i];
             T d[3];
-            const T r = tsqrt(pair_r2<D, T>(pv, xi, j, a.box, d));
+            const T r = tsqrt(rec_r2<D, T>(RP[j], xi, a.box, d));
             const T cw = Kern<D, K, T>::dwc(r, h);
-            const T mj = pv.mass[j];
-            const T mdu = mj * (pv.ene[j] - ui);
+            const V4 vj = RV[j];
+            const T mj = vj.w;
+            const T mdu = mj * (RT[j].w - ui);
+            const T vjv[3] = {vj.x, vj.y, vj.z};
             T mdv[3];
 #pragma unroll
-            for (int b2 = 0; b2 < D; ++b2) mdv[b2] = mj * (pv.v[b2][j] - vi[b2]);
+            for (int b2 = 0; b2 < D; ++b2) mdv[b2] = mj * (vjv[b2] - vi[b2]);
 #pragma unroll
             for (int k = 0; k < D; ++k) {
                 const T dwk = d[k] * cw;
@@ -236,43 +251,48 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
             }
         }
         const T rho_inv = T(1.0) / dens;
+        T g[16];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) g[q] = T(0);
 #pragma unroll
         for (int k = 0; k < D; ++k) {
             const T gp = (dd[k] * ui + du[k]) * gm1;
+            g[k] = dd[k];
+            g[3 + k] = gp;
             a.grad[k][i] = (double)dd[k];
             a.grad[3 + k][i] = (double)gp;
-            if (a.mf[M_SML]) {
-                a.mf[M_GD0 + k][i] = (float)dd[k];
-                a.mf[M_GP0 + k][i] = (float)gp;
-            }
 #pragma unroll
             for (int b2 = 0; b2 < D; ++b2) {
                 const T gv = dv[b2][k] * rho_inv;
+                g[6 + 3 * b2 + k] = gv;
                 a.grad[6 + 3 * b2 + k][i] = (double)gv;
-                if (a.mf[M_SML]) a.mf[M_GV0 + 3 * b2 + k][i] = (float)gv;
             }
         }
+#pragma unroll
+        for (int q = 0; q < 16; ++q) a.rec_w[R_G0 + (q >> 2)][4 * (size_t)i + (q & 3)] = g[q];
         return;
     }
 
     // ---- Balsara switch and time-dependent AV (pre_interaction.cpp:115-161) ----
     const T dt = T(a.dscal[0]);
-    T alpha = pv.alpha[i];
+    T alpha = a.rv.r[R_AUX][i].z;
     const bool do_bal = a.balsara_on && D != 1;
     if (do_bal || a.td_av) {
         T div = T(0), rot[3] = {0, 0, 0};
         for (int q = 0; q < w; ++q) {
             const int j = a.idx[(size_t)q * a.n + i];
             T d[3];
-            const T r = tsqrt(pair_r2<D, T>(pv, xi, j, a.box, d));
+            const T r = tsqrt(rec_r2<D, T>(RP[j], xi, a.box, d));
             const T cw = Kern<D, K, T>::dwc(r, h);
+            const V4 vj = RV[j];
+            const T vjv[3] = {vj.x, vj.y, vj.z};
             T dw[3], vij[3];
 #pragma unroll
             for (int k = 0; k < D; ++k) {
                 dw[k] = d[k] * cw;
-                vij[k] = vi[k] - pv.v[k][j];
+                vij[k] = vi[k] - vjv[k];
             }
-            const T wt = (TYPE == SPHGPU_DISPH) ? pv.mass[j] * pv.ene[j] : pv.mass[j];
+            const T wt = (TYPE == SPHGPU_DISPH) ? vj.w * RT[j].w : vj.w;
             T dot = T(0);
 #pragma unroll
             for (int k = 0; k < D; ++k) dot = (k == 0) ? vij[k] * dw[k] : dot + vij[k] * dw[k];
@@ -300,7 +320,7 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
         if (do_bal) {
             const T bal = tabs(div) / (tabs(div) + rot_abs + T(1e-4) * ci / h);
             a.balsara[i] = (double)bal;
-            if (a.mf[M_SML]) a.mf[M_BALSARA][i] = (float)bal;
+            a.rec_w[R_AUX][4 * (size_t)i + 1] = bal;
             if (a.td_av) {
                 const T mdiv = -div > T(0) ? -div : T(0);
                 const T dalpha = (-(alpha - T(a.alpha_min)) * tau_inv + mdiv * (T(a.alpha_max) - alpha)) * dt;
@@ -313,7 +333,7 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
         }
         if (a.td_av) {
             a.alpha[i] = (double)alpha;
-            if (a.mf[M_SML]) a.mf[M_ALPHA][i] = (float)alpha;
+            a.rec_w[R_AUX][4 * (size_t)i + 2] = alpha;
         }
     }
 }
@@ -363,7 +383,7 @@ __global__ void k_store_hvs(const unsigned long long *m, double *dscal) { dscal[
 template <typename T> struct ForceArgs {
     int n;
     const int *idx, *cnt;
-    PairView<T> pv;
+    RecView<T> rv;
     double *acc[3];
     double *dene;
     const double *dscal;
@@ -399,14 +419,18 @@ SPH_DEV T av_pi(const T vij[3], const T d[3], T r, T ai, T aj, T bi, T bj, T ci,
 template <int D, int K, int TYPE, typename T>
 __global__ __launch_bounds__(256) void k_force_sph(ForceArgs<T> a)
 {
+    typedef typename V4T<T>::type V4;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.n) return;
-    const PairView<T> &pv = a.pv;
-    T xi[3] = {0, 0, 0}, vi[3] = {0, 0, 0};
-#pragma unroll
-    for (int k = 0; k < D; ++k) { xi[k] = pv.x[k][i]; vi[k] = pv.v[k][i]; }
-    const T hi = pv.sml[i], rhoi = pv.dens[i], presi = pv.pres[i], gradhi = pv.gradh[i];
-    const T ci = pv.sound[i], ai = pv.alpha[i], bi = pv.balsara[i], mi = pv.mass[i], ui = pv.ene[i];
+    const V4 *__restrict__ RP = a.rv.r[R_POSH];
+    const V4 *__restrict__ RV = a.rv.r[R_VELM];
+    const V4 *__restrict__ RT = a.rv.r[R_THERMO];
+    const V4 *__restrict__ RA = a.rv.r[R_AUX];
+    const V4 pi_ = RP[i], vi_ = RV[i], ti_ = RT[i], ai_ = RA[i];
+    const T xi[3] = {pi_.x, pi_.y, pi_.z};
+    const T vi[3] = {vi_.x, vi_.y, vi_.z};
+    const T hi = pi_.w, rhoi = ti_.x, presi = ti_.y, gradhi = ai_.x;
+    const T ci = ti_.z, ai = ai_.z, bi = ai_.y, mi = vi_.w, ui = ti_.w;
     const T gm1 = T(a.gamma - 1.0);
     const T p_per_rho2_i = presi / sqr(rhoi);
     const T g2u_i = sqr(gm1) * ui;
@@ -416,20 +440,22 @@ __global__ __launch_bounds__(256) void k_force_sph(ForceArgs<T> a)
     const int nc = a.cnt[i];
     for (int q = 0; q < nc; ++q) {
         const int j = a.idx[(size_t)q * a.n + i];
+        const V4 pj = RP[j], vj = RV[j], tj = RT[j], aj = RA[j];
         T d[3];
-        const T r = tsqrt(pair_r2<D, T>(pv, xi, j, a.box, d));
-        const T hj = pv.sml[j], rhoj = pv.dens[j], presj = pv.pres[j], mj = pv.mass[j], uj = pv.ene[j];
+        const T r = tsqrt(rec_r2<D, T>(pj, xi, a.box, d));
+        const T hj = pj.w, rhoj = tj.x, presj = tj.y, mj = vj.w, uj = tj.w;
         const T cwi = Kern<D, K, T>::dwc(r, hi);
         const T cwj = Kern<D, K, T>::dwc(r, hj);
+        const T vjv[3] = {vj.x, vj.y, vj.z};
         T dwi[3], dwj[3], dwij[3], vij[3];
 #pragma unroll
         for (int k = 0; k < D; ++k) {
             dwi[k] = d[k] * cwi;
             dwj[k] = d[k] * cwj;
             dwij[k] = (dwi[k] + dwj[k]) * T(0.5);
-            vij[k] = vi[k] - pv.v[k][j];
+            vij[k] = vi[k] - vjv[k];
         }
-        const T pi_ij = av_pi<D, T>(vij, d, r, ai, pv.alpha[j], bi, pv.balsara[j], ci, pv.sound[j], rhoi, rhoj);
+        const T pi_ij = av_pi<D, T>(vij, d, r, ai, aj.z, bi, aj.y, ci, tj.z, rhoi, rhoj);
         T dene_ac = T(0);
         if (a.use_ac) {
             // FluidForce::artificial_conductivity, src/fluid_force.cpp:108-116
@@ -456,11 +482,11 @@ __global__ __launch_bounds__(256) void k_force_sph(ForceArgs<T> a)
         }
         if (TYPE == SPHGPU_SSPH) {
             s1 = mj * (p_per_rho2_i * gradhi + T(0.5) * pi_ij);
-            s2 = mj * (presj / sqr(rhoj) * pv.gradh[j] + T(0.5) * pi_ij);
+            s2 = mj * (presj / sqr(rhoj) * aj.x + T(0.5) * pi_ij);
             e1 = mj * p_per_rho2_i * gradhi * vdwi + T(0.5) * mj * pi_ij * vdwij + dene_ac;
         } else {
             const T f_ij = T(1.0) - gradhi / (mj * uj);
-            const T f_ji = T(1.0) - pv.gradh[j] * m_u_inv;
+            const T f_ji = T(1.0) - aj.x * m_u_inv;
             const T u_per_pres_j = uj / presj;
             s1 = mj * (g2upp_i * uj * f_ij + T(0.5) * pi_ij);
             s2 = mj * (g2u_i * u_per_pres_j * f_ji + T(0.5) * pi_ij);
@@ -535,58 +561,49 @@ SPH_DEV int solve_pair(const T l[4], const T r[4], T gamma, int refine64, const 
     return it;
 }
 
-template <int D, int K, typename T, int SOLVER, int LEARNED>
-__global__ __launch_bounds__(256) void k_force_gsph(ForceArgs<T> a)
+// The pair-slot loop of the GSPH force.  Slot q holds each lane's q-th pair;
+// the reconstruction and the force run per lane, the learned guess (when
+// enabled) runs for the whole wavefront on MFMA between them.
+template <int D, int K, typename T, int SOLVER, bool LEARNED>
+SPH_DEV void force_gsph_slots(const ForceArgs<T> &a, int i, int nc, int nmax, const T xi[3], const T vi[3],
+                              const T gdi[3], const T gpi[3], const T gvi[3][3], T hi, T rhoi, T presi, T ci,
+                              T gamma, T rho2_inv_i, T dt, Acc<T> ac[3], Acc<T> &de, const MlpWeightsLDS *wl,
+                              MlpWaveLDS *wsc, bool valid)
 {
-    const int i0 = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool valid = i0 < a.n;
-    const int i = valid ? i0 : a.n - 1;
-    const PairView<T> &pv = a.pv;
-    T xi[3] = {0, 0, 0}, vi[3] = {0, 0, 0}, gdi[3] = {0, 0, 0}, gpi[3] = {0, 0, 0}, gvi[3][3] = {};
-#pragma unroll
-    for (int k = 0; k < D; ++k) {
-        xi[k] = pv.x[k][i];
-        vi[k] = pv.v[k][i];
-        if (a.gsph2) {
-            gdi[k] = pv.gd[k][i];
-            gpi[k] = pv.gp[k][i];
-#pragma unroll
-            for (int b2 = 0; b2 < D; ++b2) gvi[b2][k] = pv.gv[3 * b2 + k][i];
-        }
-    }
-    const T hi = pv.sml[i], rhoi = pv.dens[i], presi = pv.pres[i], ci = pv.sound[i];
-    const T gamma = T(a.gamma);
-    const T rho2_inv_i = T(1.0) / sqr(rhoi);
-    const T dt = T(a.dscal[0]);
-    Acc<T> ac[3], de;
-    const int nc = valid ? a.cnt[i] : 0;
-    // the wave runs as many pair slots as its longest list
-    int nmax = nc;
-    for (int off = kWave / 2; off > 0; off >>= 1) {
-        const int o = __shfl_xor(nmax, off, kWave);
-        nmax = o > nmax ? o : nmax;
-    }
+    typedef typename V4T<T>::type V4;
+    const V4 *__restrict__ RP = a.rv.r[R_POSH];
+    const V4 *__restrict__ RV = a.rv.r[R_VELM];
+    const V4 *__restrict__ RT = a.rv.r[R_THERMO];
     long long iters = 0, wave_max_sum = 0, calls = 0, fallbacks = 0, learned = 0;
     for (int q = 0; q < nmax; ++q) {
         const bool act = q < nc;
         int it = 0;
+        int j = 0;
+        T d[3] = {0, 0, 0}, e[3] = {0, 0, 0}, r = T(1), r_inv = T(1);
+        T left[4] = {0, 1, 1, 1}, right[4] = {0, 1, 1, 1};
+        T rhoj = T(1), hj = T(1), mj = T(0);
         if (act) {
-            const int j = a.idx[(size_t)q * a.n + i];
-            T d[3];
-            const T r = tsqrt(pair_r2<D, T>(pv, xi, j, a.box, d));
-            const T r_inv = T(1.0) / r;
-            T e[3], vj[3];
+            j = a.idx[(size_t)q * a.n + i];
+            const V4 pj = RP[j], vjr = RV[j], tj = RT[j];
+            r = tsqrt(rec_r2<D, T>(pj, xi, a.box, d));
+            r_inv = T(1.0) / r;
+            const T vj[3] = {vjr.x, vjr.y, vjr.z};
             T ve_i = T(0), ve_j = T(0);
 #pragma unroll
             for (int k = 0; k < D; ++k) {
                 e[k] = d[k] * r_inv;
-                vj[k] = pv.v[k][j];
                 ve_i = (k == 0) ? vi[k] * e[k] : ve_i + vi[k] * e[k];
                 ve_j = (k == 0) ? vj[k] * e[k] : ve_j + vj[k] * e[k];
             }
-            const T rhoj = pv.dens[j], presj = pv.pres[j], cj = pv.sound[j];
-            T left[4], right[4];
+            hj = pj.w;
+            mj = vjr.w;
+            rhoj = tj.x;
+            const T presj = tj.y, cj = tj.z;
             if (a.gsph2) {
+                // neighbour gradients: (gd, gp, gv) packed in four records
+                const V4 g0 = a.rv.r[R_G0][j], g1 = a.rv.r[R_G1][j], g2 = a.rv.r[R_G2][j], g3 = a.rv.r[R_G3][j];
+                const T gj[16] = {g0.x, g0.y, g0.z, g0.w, g1.x, g1.y, g1.z, g1.w,
+                                  g2.x, g2.y, g2.z, g2.w, g3.x, g3.y, g3.z, g3.w};
                 // MUSCL reconstruction, g_fluid_force.cpp:103-140
                 const T delta_i = T(0.5) * (T(1.0) - ci * dt * r_inv);
                 const T delta_j = T(0.5) * (T(1.0) - cj * dt * r_inv);
@@ -598,7 +615,7 @@ __global__ __launch_bounds__(256) void k_force_gsph(ForceArgs<T> a)
 #pragma unroll
                     for (int k = 0; k < D; ++k) {
                         si = (k == 0) ? gvi[b2][k] * e[k] : si + gvi[b2][k] * e[k];
-                        const T gvj = pv.gv[3 * b2 + k][j];
+                        const T gvj = gj[6 + 3 * b2 + k];
                         sj = (k == 0) ? gvj * e[k] : sj + gvj * e[k];
                     }
                     dvi[b2] = si;
@@ -610,10 +627,10 @@ __global__ __launch_bounds__(256) void k_force_gsph(ForceArgs<T> a)
                     dve_i = (k == 0) ? dvi[k] * e[k] : dve_i + dvi[k] * e[k];
                     dve_j = (k == 0) ? dvj[k] * e[k] : dve_j + dvj[k] * e[k];
                     dd_i = (k == 0) ? gdi[k] * e[k] : dd_i + gdi[k] * e[k];
-                    const T gdj = pv.gd[k][j];
+                    const T gdj = gj[k];
                     dd_j = (k == 0) ? gdj * e[k] : dd_j + gdj * e[k];
                     dp_i = (k == 0) ? gpi[k] * e[k] : dp_i + gpi[k] * e[k];
-                    const T gpj = pv.gp[k][j];
+                    const T gpj = gj[3 + k];
                     dp_j = (k == 0) ? gpj * e[k] : dp_j + gpj * e[k];
                 }
                 dve_i = dve_i * r;
@@ -636,20 +653,38 @@ __global__ __launch_bounds__(256) void k_force_gsph(ForceArgs<T> a)
                 right[0] = ve_i; right[1] = rhoi; right[2] = presi; right[3] = ci;
                 left[0] = ve_j; left[1] = rhoj; left[2] = presj; left[3] = cj;
             }
+        }
+        // learned starting guess for the "hard" pairs of this slot: strong
+        // pressure ratio or velocity jump above the mean sound speed
+        T guess = T(0);
+        bool use_guess = false;
+        if constexpr (LEARNED) {
+            const float pl = (float)left[2], pr = (float)right[2];
+            const float pmin = fminf(pl, pr), pmax = fmaxf(pl, pr);
+            const float du = (float)(left[0] - right[0]);
+            const float r_m = 0.5f * (float)(left[1] + right[1]), p_m = 0.5f * (pl + pr);
+            const bool hard = act && pmin > 0.f && r_m > 0.f &&
+                              (pmax > 4.f * pmin || du * du * r_m > (float)gamma * p_m);
+            if (__ballot(hard)) {
+                float x[6] = {0, 0, 0, 0, 0, 0}, pm = 1.f;
+                if (hard)
+                    mlp_features((float)gamma, (float)left[0], (float)left[1], pl, (float)right[0],
+                                 (float)right[1], pr, x, pm);
+                const float y = mlp_wave_infer(*wl, *wsc, x);
+                if (hard) {
+                    guess = (T)(pm * __expf(y));
+                    use_guess = true;
+                }
+            }
+        }
+        if (act) {
             T pstar, vstar;
             int fb = 0;
-            T guess;
-            const T *gp = nullptr;
-            if (LEARNED) {
-                guess = (T)mlp_guess(a.mlp, (float)left[0], (float)left[1], (float)left[2], (float)right[0],
-                                     (float)right[1], (float)right[2]);
-                gp = &guess;
-            }
-            it = solve_pair<T, SOLVER>(left, right, gamma, a.refine64, gp, pstar, vstar, fb);
+            it = solve_pair<T, SOLVER>(left, right, gamma, a.refine64, use_guess ? &guess : nullptr, pstar, vstar, fb);
             if (SOLVER == SPHGPU_RIEMANN_ITERATIVE) {
                 ++calls;
                 fallbacks += fb;
-                learned += (LEARNED && !fb) ? 1 : 0;
+                learned += (use_guess && !fb) ? 1 : 0;
                 if (a.rlog) {
                     const unsigned long long slot = atomicAdd(a.rlog_count, 1ull);
                     if ((long long)(slot % (unsigned long long)a.rlog_stride) == 0) {
@@ -662,7 +697,6 @@ __global__ __launch_bounds__(256) void k_force_gsph(ForceArgs<T> a)
                 }
             }
             // Godunov force, g_fluid_force.cpp:158-165
-            const T hj = pv.sml[j], mj = pv.mass[j];
             const T cwi = Kern<D, K, T>::dwc(r, hi);
             const T cwj = Kern<D, K, T>::dwc(r, hj);
             const T rho2_inv_j = T(1.0) / sqr(rhoj);
@@ -688,11 +722,6 @@ __global__ __launch_bounds__(256) void k_force_gsph(ForceArgs<T> a)
             wave_max_sum += m;
         }
     }
-    if (valid) {
-#pragma unroll
-        for (int k = 0; k < D; ++k) a.acc[k][i] = (double)ac[k].get();
-        a.dene[i] = (double)de.get();
-    }
     wave_add_atomic(&a.stats[3], valid ? nc : 0);
     if (SOLVER == SPHGPU_RIEMANN_ITERATIVE) {
         wave_add_atomic(&a.stats[5], iters);
@@ -700,6 +729,60 @@ __global__ __launch_bounds__(256) void k_force_gsph(ForceArgs<T> a)
         wave_add_atomic(&a.stats[7], calls);
         wave_add_atomic(&a.stats[8], fallbacks);
         wave_add_atomic(&a.stats[9], learned);
+    }
+}
+
+template <int D, int K, typename T, int SOLVER, int LEARNED>
+__global__ __launch_bounds__(256) void k_force_gsph(ForceArgs<T> a)
+{
+    const int i0 = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = i0 < a.n;
+    const int i = valid ? i0 : a.n - 1;
+    typedef typename V4T<T>::type V4;
+    const V4 pi_ = a.rv.r[R_POSH][i], vi_ = a.rv.r[R_VELM][i], ti_ = a.rv.r[R_THERMO][i];
+    const T xi[3] = {pi_.x, pi_.y, pi_.z};
+    const T vi[3] = {vi_.x, vi_.y, vi_.z};
+    T gdi[3] = {0, 0, 0}, gpi[3] = {0, 0, 0}, gvi[3][3] = {};
+    if (a.gsph2) {
+        const V4 g0 = a.rv.r[R_G0][i], g1 = a.rv.r[R_G1][i], g2 = a.rv.r[R_G2][i], g3 = a.rv.r[R_G3][i];
+        const T g[16] = {g0.x, g0.y, g0.z, g0.w, g1.x, g1.y, g1.z, g1.w,
+                         g2.x, g2.y, g2.z, g2.w, g3.x, g3.y, g3.z, g3.w};
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            gdi[k] = g[k];
+            gpi[k] = g[3 + k];
+#pragma unroll
+            for (int b2 = 0; b2 < D; ++b2) gvi[b2][k] = g[6 + 3 * b2 + k];
+        }
+    }
+    const T hi = pi_.w, rhoi = ti_.x, presi = ti_.y, ci = ti_.z;
+    const T gamma = T(a.gamma);
+    const T rho2_inv_i = T(1.0) / sqr(rhoi);
+    const T dt = T(a.dscal[0]);
+    Acc<T> ac[3], de;
+    const int nc = valid ? a.cnt[i] : 0;
+    // the wave runs as many pair slots as its longest list
+    int nmax = nc;
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+        const int o = __shfl_xor(nmax, off, kWave);
+        nmax = o > nmax ? o : nmax;
+    }
+    if constexpr (LEARNED) {
+        // the learned guess needs the network's weights in LDS (whole block)
+        __shared__ MlpWeightsLDS wl;
+        __shared__ MlpWaveLDS wsc[4];
+        mlp_load_weights(wl, a.mlp.w);
+        __syncthreads();
+        force_gsph_slots<D, K, T, SOLVER, true>(a, i, nc, nmax, xi, vi, gdi, gpi, gvi, hi, rhoi, presi, ci, gamma,
+                                               rho2_inv_i, dt, ac, de, &wl, &wsc[(threadIdx.x >> 6) & 3], valid);
+    } else {
+        force_gsph_slots<D, K, T, SOLVER, false>(a, i, nc, nmax, xi, vi, gdi, gpi, gvi, hi, rhoi, presi, ci, gamma,
+                                                rho2_inv_i, dt, ac, de, nullptr, nullptr, valid);
+    }
+    if (valid) {
+#pragma unroll
+        for (int k = 0; k < D; ++k) a.acc[k][i] = (double)ac[k].get();
+        a.dene[i] = (double)de.get();
     }
 }
 
@@ -721,26 +804,23 @@ __global__ void k_riemann_batch(int n, const double *__restrict__ L, const doubl
 }
 
 // ---------------------------------------------------------------------------------
-template <typename T> void fill_mirror_ptrs(sphgpu_ctx *c, PreArgs<T> &a)
-{
-    for (int m = 0; m < M_NFIELD; ++m) a.mf[m] = (sizeof(T) == 4) ? c->mf[m] : nullptr;
-}
-
 template <int D, int K, int TYPE, typename T> void launch_pre(sphgpu_ctx *c)
 {
     PreArgs<T> a;
     a.n = c->n;
     a.idx = c->lpre.idx;
     a.cnt = c->lpre.cnt;
-    a.pv = pair_view<T>(c);
+    a.rv = rec_view<T>(c);
+    for (int r = 0; r < R_NREC; ++r) a.rec_w[r] = (T *)a.rv.r[r];
+    a.posf_w = (sizeof(T) == 8) ? (float *)c->rec32[R_POSH] : nullptr;
+    a.root = c->tree.root;
     for (int k = 0; k < 3; ++k) a.x64[k] = c->pos[k < c->dim ? k : 0];
     a.hs = c->hs;
-    a.mass64 = c->sc[S_MASS]; a.ene64 = c->sc[S_ENE]; a.sound64 = c->sc[S_SOUND]; a.dens64 = c->sc[S_DENS];
+    a.mass64 = c->sc[S_MASS];
     a.sml = c->sc[S_SML]; a.dens = c->sc[S_DENS]; a.pres = c->sc[S_PRES]; a.gradh = c->sc[S_GRADH];
     a.balsara = c->sc[S_BALSARA]; a.alpha = c->sc[S_ALPHA];
     a.neighbor = c->neighbor;
     for (int g = 0; g < kNGrad; ++g) a.grad[g] = c->grad[g];
-    fill_mirror_ptrs<T>(c, a);
     a.hvs_min = c->dmin;
     a.dscal = c->dscal;
     a.stats = c->dstats;
@@ -822,7 +902,7 @@ template <typename T> ForceArgs<T> force_args(sphgpu_ctx *c)
     a.n = c->n;
     a.idx = c->lforce.idx;
     a.cnt = c->lforce.cnt;
-    a.pv = pair_view<T>(c);
+    a.rv = rec_view<T>(c);
     for (int k = 0; k < 3; ++k) a.acc[k] = c->acc[k];
     a.dene = c->sc[S_DENE];
     a.dscal = c->dscal;

@@ -151,18 +151,26 @@ __global__ void k_energy_final(const double *__restrict__ part, int nb, double *
     out[6] = a[5];
 }
 
-struct MirrorArgs {
-    int n;
-    const double *src[M_NFIELD];
-    float *dst[M_NFIELD];
+// pair records from the FP64 masters: 32 values per particle in record
+// order (see RecField); null sources read as zero
+struct RecArgs {
+    int n, nrec;                  // records written: R_AUX + 1, or all with gradients
+    const double *src[4 * R_NREC];
+    float4 *r32[R_NREC];          // null entries are not written
+    double4 *r64[R_NREC];
 };
 
-__global__ void k_mirrors(MirrorArgs a)
+__global__ void k_records(RecArgs a)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.n) return;
-    for (int f = 0; f < M_NFIELD; ++f)
-        if (a.src[f]) a.dst[f][i] = (float)a.src[f][i];
+    for (int r = 0; r < a.nrec; ++r) {
+        double v[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) v[q] = a.src[4 * r + q] ? a.src[4 * r + q][i] : 0.0;
+        if (a.r64[r]) a.r64[r][i] = make_double4(v[0], v[1], v[2], v[3]);
+        if (a.r32[r]) a.r32[r][i] = make_float4((float)v[0], (float)v[1], (float)v[2], (float)v[3]);
+    }
 }
 
 IntArgs int_args(sphgpu_ctx *c)
@@ -257,31 +265,38 @@ int energy_momentum(sphgpu_ctx *c, double out[7])
 
 int refresh_mirrors(sphgpu_ctx *c)
 {
-    if (c->prm.pair_precision == SPHGPU_PAIR_FP64) {
-        c->mirror_dirty = false;
-        return 0;
-    }
-    MirrorArgs a;
+    RecArgs a;
     a.n = c->n;
-    for (int f = 0; f < M_NFIELD; ++f) { a.src[f] = nullptr; a.dst[f] = c->mf[f]; }
+    const bool gsph2 = c->prm.sph_type == SPHGPU_GSPH && c->prm.gsph_2nd_order;
+    a.nrec = gsph2 ? R_NREC : R_AUX + 1;
+    for (int f = 0; f < 4 * R_NREC; ++f) a.src[f] = nullptr;
     for (int k = 0; k < c->dim; ++k) {
-        a.src[M_X + k] = c->pos[k];
-        a.src[M_VX + k] = c->vel[k];
-        a.src[M_GD0 + k] = c->grad[k];
-        a.src[M_GP0 + k] = c->grad[3 + k];
-        for (int b = 0; b < c->dim; ++b) a.src[M_GV0 + 3 * b + k] = c->grad[6 + 3 * b + k];
+        a.src[4 * R_POSH + k] = c->pos[k];
+        a.src[4 * R_VELM + k] = c->vel[k];
     }
-    a.src[M_MASS] = c->sc[S_MASS];
-    a.src[M_ENE] = c->sc[S_ENE];
-    a.src[M_SOUND] = c->sc[S_SOUND];
-    a.src[M_SML] = c->sc[S_SML];
-    a.src[M_DENS] = c->sc[S_DENS];
-    a.src[M_PRES] = c->sc[S_PRES];
-    a.src[M_GRADH] = c->sc[S_GRADH];
-    a.src[M_BALSARA] = c->sc[S_BALSARA];
-    a.src[M_ALPHA] = c->sc[S_ALPHA];
-    time_begin(c, "mirrors");
-    hipLaunchKernelGGL(k_mirrors, dim3(grid_for(c->n, 256)), dim3(256), 0, c->stream, a);
+    a.src[4 * R_POSH + 3] = c->sc[S_SML];
+    a.src[4 * R_VELM + 3] = c->sc[S_MASS];
+    a.src[4 * R_THERMO + 0] = c->sc[S_DENS];
+    a.src[4 * R_THERMO + 1] = c->sc[S_PRES];
+    a.src[4 * R_THERMO + 2] = c->sc[S_SOUND];
+    a.src[4 * R_THERMO + 3] = c->sc[S_ENE];
+    a.src[4 * R_AUX + 0] = c->sc[S_GRADH];
+    a.src[4 * R_AUX + 1] = c->sc[S_BALSARA];
+    a.src[4 * R_AUX + 2] = c->sc[S_ALPHA];
+    // gradients: flat index q of (gd[3], gp[3], gv[9]) goes to slot 4*R_G0 + q
+    for (int k = 0; k < c->dim; ++k) {
+        a.src[4 * R_G0 + k] = c->grad[k];
+        a.src[4 * R_G0 + 3 + k] = c->grad[3 + k];
+        for (int b = 0; b < c->dim; ++b) a.src[4 * R_G0 + 6 + 3 * b + k] = c->grad[6 + 3 * b + k];
+    }
+    const bool fp64 = c->prm.pair_precision == SPHGPU_PAIR_FP64;
+    for (int r = 0; r < R_NREC; ++r) {
+        a.r64[r] = fp64 ? c->rec64[r] : nullptr;
+        // the FP32 position/h record always exists: it feeds the search prefilter
+        a.r32[r] = (!fp64 || r == R_POSH) ? c->rec32[r] : nullptr;
+    }
+    time_begin(c, "records");
+    hipLaunchKernelGGL(k_records, dim3(grid_for(c->n, 256)), dim3(256), 0, c->stream, a);
     time_end(c);
     HIPCHK(c, hipGetLastError());
     c->mirror_dirty = false;

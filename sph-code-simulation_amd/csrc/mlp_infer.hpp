@@ -79,4 +79,114 @@ SPH_DEV float mlp_guess(const MLPDev &m, float uL, float rL, float pL, float uR,
     return p_m * __expf(y);
 }
 
+// ---- wavefront-cooperative bf16 MFMA inference --------------------------------
+// The 64 lanes of a wavefront each contribute one pair (row); the network runs
+// as three small GEMMs on v_mfma_f32_16x16x32_bf16, one 16-row chunk at a
+// time: z1 = X W1^T (K = 32, inputs zero-padded), z2 = relu(z1) W2^T (K = 64),
+// out = relu(z2) w3 (w3 as column 0 of a 64x16 B operand).  Tile lane layout
+// (CDNA4): A[16][32] -- lane l holds row l%16, k = 8*(l/16)..+7; B[32][16] --
+// lane l holds column l%16, same k; D[16][16] -- lane l holds rows
+// 4*(l/16)..+3 of column l%16.
+typedef __bf16 mlp_bf16;
+typedef __bf16 mlp_bf16x8 __attribute__((ext_vector_type(8)));
+typedef float mlp_f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kInfK1 = 32;
+constexpr int kInfLd1 = kInfK1 + 8;
+constexpr int kInfLd = kMlpH + 8;
+
+struct MlpWeightsLDS {            // shared by the waves of a workgroup
+    mlp_bf16 W1[kMlpH][kInfLd1];  // [out][in], inputs padded to 32
+    mlp_bf16 W2[kMlpH][kInfLd];   // [out][in]
+    mlp_bf16 W3[16][kInfLd];      // row 0 = w3, rows 1..15 zero
+    float b1[kMlpH], b2[kMlpH];
+    float b3;
+};
+
+struct MlpWaveLDS {               // per wavefront scratch
+    mlp_bf16 X[64][kInfLd1];      // features, row = lane
+    mlp_bf16 H[16][kInfLd];       // relu(z1) or relu(z2) of one 16-row chunk
+    float out[64];
+};
+
+// cooperative load of the FP32 parameters into bf16 LDS (whole workgroup)
+SPH_DEV void mlp_load_weights(MlpWeightsLDS &L, const float *w)
+{
+    for (int e = threadIdx.x; e < kMlpH * kInfK1; e += blockDim.x) {
+        const int o = e / kInfK1, k = e % kInfK1;
+        L.W1[o][k] = (mlp_bf16)(k < kMlpIn ? w[kOffW1 + o * kMlpIn + k] : 0.f);
+    }
+    for (int e = threadIdx.x; e < kMlpH * kMlpH; e += blockDim.x) {
+        const int o = e / kMlpH, k = e % kMlpH;
+        L.W2[o][k] = (mlp_bf16)w[kOffW2 + o * kMlpH + k];
+    }
+    for (int e = threadIdx.x; e < 16 * kMlpH; e += blockDim.x) {
+        const int r = e / kMlpH, k = e % kMlpH;
+        L.W3[r][k] = (mlp_bf16)(r == 0 ? w[kOffW3 + k] : 0.f);
+    }
+    for (int e = threadIdx.x; e < kMlpH; e += blockDim.x) {
+        L.b1[e] = w[kOffB1 + e];
+        L.b2[e] = w[kOffB2 + e];
+    }
+    if (threadIdx.x == 0) L.b3 = w[kOffB3];
+}
+
+// LDS hand-off between lanes of one wavefront: order the LDS operations and
+// keep the compiler from moving accesses across
+SPH_DEV void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// whole wavefront: returns the network output (log(p*/p_m)) for this lane's row
+SPH_DEV float mlp_wave_infer(const MlpWeightsLDS &L, MlpWaveLDS &S, const float x[6])
+{
+    const int lane = threadIdx.x & 63;
+    for (int k = 0; k < kInfK1; ++k) S.X[lane][k] = (mlp_bf16)(k < kMlpIn ? x[k] : 0.f);
+    const int r16 = lane & 15, kq = 8 * (lane >> 4);
+    for (int chunk = 0; chunk < 4; ++chunk) {
+        wave_lds_sync();
+        // layer 1: z1 (16 x 64) = X[chunk] (16 x 32) W1^T
+        const mlp_bf16x8 a1 = *reinterpret_cast<const mlp_bf16x8 *>(&S.X[chunk * 16 + r16][kq]);
+        for (int tn = 0; tn < 4; ++tn) {
+            const mlp_bf16x8 b = *reinterpret_cast<const mlp_bf16x8 *>(&L.W1[tn * 16 + r16][kq]);
+            mlp_f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b, acc, 0, 0, 0);
+            const int col = tn * 16 + r16;
+            for (int i = 0; i < 4; ++i) S.H[4 * (lane >> 4) + i][col] = (mlp_bf16)fmaxf(acc[i] + L.b1[col], 0.f);
+        }
+        wave_lds_sync();
+        // layer 2: z2 (16 x 64) = H (16 x 64) W2^T
+        mlp_f32x4 acc2[4];
+        for (int tn = 0; tn < 4; ++tn) {
+            acc2[tn] = mlp_f32x4{0.f, 0.f, 0.f, 0.f};
+            for (int k0 = 0; k0 < kMlpH; k0 += 32) {
+                const mlp_bf16x8 a = *reinterpret_cast<const mlp_bf16x8 *>(&S.H[r16][k0 + kq]);
+                const mlp_bf16x8 b = *reinterpret_cast<const mlp_bf16x8 *>(&L.W2[tn * 16 + r16][k0 + kq]);
+                acc2[tn] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc2[tn], 0, 0, 0);
+            }
+        }
+        wave_lds_sync();
+        for (int tn = 0; tn < 4; ++tn) {
+            const int col = tn * 16 + r16;
+            for (int i = 0; i < 4; ++i) S.H[4 * (lane >> 4) + i][col] = (mlp_bf16)fmaxf(acc2[tn][i] + L.b2[col], 0.f);
+        }
+        wave_lds_sync();
+        // layer 3: out (16 x 16, column 0 used) = H (16 x 64) W3^T
+        mlp_f32x4 acc3 = {0.f, 0.f, 0.f, 0.f};
+        for (int k0 = 0; k0 < kMlpH; k0 += 32) {
+            const mlp_bf16x8 a = *reinterpret_cast<const mlp_bf16x8 *>(&S.H[r16][k0 + kq]);
+            const mlp_bf16x8 b = *reinterpret_cast<const mlp_bf16x8 *>(&L.W3[r16][k0 + kq]);
+            acc3 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc3, 0, 0, 0);
+        }
+        if (r16 == 0)
+            for (int i = 0; i < 4; ++i) S.out[chunk * 16 + 4 * (lane >> 4) + i] = acc3[i] + L.b3;
+    }
+    wave_lds_sync();
+    const float y = S.out[lane];
+    wave_lds_sync();
+    return y;
+}
+
 } // namespace sphgpu

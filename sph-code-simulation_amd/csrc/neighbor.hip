@@ -14,7 +14,9 @@
 // the group's largest radius (and the node's kernel_size for is_ij), can reach
 // the node cell; otherwise the walk jumps to the node's skip index.  In an
 // entered leaf every lane evaluates the reference's FP64 predicate for its own
-// particle, so the resulting sets are exactly the reference's:
+// particle (behind an FP32 prefilter that only settles candidates whose FP32
+// distance lies outside a proven rounding band), so the resulting sets are
+// exactly the reference's:
 //   is_ij = 0 :  r_ij^2 < hs_i^2
 //   is_ij = 1 :  sqrt(r_ij^2) < max(h_i, h_j)  and  r_ij^2 != 0
 // The group test only prunes; it never decides membership.
@@ -32,6 +34,8 @@ struct SearchArgs {
     const double *x[3];
     const double *radius;     // hs_i (is_ij = 0) or h_i (is_ij = 1)
     const double *sml;        // h_j for is_ij
+    const float4 *posf;       // FP32 (x, y, z, h) records for the prefilter
+    const double *root;       // tree root geometry (bounds |x|)
     const int *nstart, *nend, *nskip, *nlevel;
     const double *ncenter;    // [3][cap]
     const double *nedge, *nks;
@@ -73,6 +77,9 @@ template <int D> SPH_DEV double wave_max(double v)
     return v;
 }
 
+// particles per "chunk" node: a wavefront walks once per chunk it touches
+constexpr int kChunk = 128;
+
 template <int D, int IS_IJ>
 __global__ __launch_bounds__(256) void k_search(SearchArgs a)
 {
@@ -86,70 +93,104 @@ __global__ __launch_bounds__(256) void k_search(SearchArgs a)
 #pragma unroll
     for (int k = 0; k < D; ++k) xi[k] = a.x[k][ii];
     const double R = a.radius[ii];
-    // group bounding box and radius
-    double g[3] = {0, 0, 0}, hb[3] = {0, 0, 0};
-#pragma unroll
-    for (int k = 0; k < D; ++k) {
-        const double lo = wave_min<D>(xi[k]);
-        const double hi = wave_max<D>(xi[k]);
-        g[k] = 0.5 * (lo + hi);
-        hb[k] = 0.5 * (hi - lo);
-    }
-    const double Rg = wave_max<D>(R);
-    const double hi_sml = IS_IJ ? a.sml[ii] : 0.0;
-    (void)hi_sml;
+    const float4 pfi = a.posf[ii];
+    const float xf[3] = {pfi.x, pfi.y, pfi.z};
+    const float Rf = (float)R;
+    const float mk = fp32_margin_k(a.root, D);
     int count = 0;
     bool over = false;
-    int node = 0;
     long long visits = 0, tests = 0;
-    while (node < a.n_nodes) {
-        ++visits;
-        const double e = a.nedge[node];
-        double rn = Rg;
-        if (IS_IJ) {
-            const double ks = a.nks[node];
-            rn = ks > rn ? ks : rn;
+    const int p_end = min(group * kWave + kWave, a.n);
+    int sub = group * kWave;
+    // Consecutive particles in key order can straddle the boundary of a large
+    // cell, which would make the group's bounding box span that cell.  The
+    // wave therefore splits its particles at the boundaries of "chunk" nodes
+    // (the deepest nodes holding their start particle with <= kChunk members)
+    // and walks once per piece.
+    while (sub < p_end) {
+        int cn = 0;
+        while (a.nlevel[cn] > 0 && a.nend[cn] - a.nstart[cn] > kChunk) {
+            int ch = cn + 1;
+            while (!(a.nstart[ch] <= sub && sub < a.nend[ch])) ch = a.nskip[ch];
+            cn = ch;
         }
-        // conservative reach test, padded against rounding
-        const double lim = (0.5 * e + rn) * (1.0 + 1e-12) + 1e-300;
-        bool reach = true;
+        const int sub_end = min(p_end, a.nend[cn]);
+        const bool mine = valid && i >= sub && i < sub_end;
+        // bounding box and radius of this piece
+        double g[3] = {0, 0, 0}, hb[3] = {0, 0, 0};
 #pragma unroll
         for (int k = 0; k < D; ++k) {
-            const double c = a.ncenter[(size_t)k * a.node_cap + node];
-            const double d = tabs(rij_axis<double>(c, g[k], a.box, k)) - hb[k];
-            reach = reach && (d <= lim);
+            const double lo = wave_min<D>(mine ? xi[k] : 1.7976931348623157e308);
+            const double hi = wave_max<D>(mine ? xi[k] : -1.7976931348623157e308);
+            g[k] = 0.5 * (lo + hi);
+            hb[k] = 0.5 * (hi - lo);
         }
-        const int lv = a.nlevel[node];
-        if (!reach) {
-            node = a.nskip[node];
-            continue;
-        }
-        if (lv > 0) {  // internal node: descend
-            ++node;
-            continue;
-        }
-        const int s = a.nstart[node], en = a.nend[node];
-        tests += en - s;
-        for (int j = s; j < en; ++j) {
-            double xj[3] = {0, 0, 0};
-#pragma unroll
-            for (int k = 0; k < D; ++k) xj[k] = a.x[k][j];
-            const double r2 = r2_exact<D>(xi, xj, a.box);
-            bool keep;
+        const double Rg = wave_max<D>(mine ? R : 0.0);
+        int node = 0;
+        while (node < a.n_nodes) {
+            ++visits;
+            const double e = a.nedge[node];
+            double rn = Rg;
             if (IS_IJ) {
-                const double hj = a.sml[j];
-                const double H = (R < hj) ? hj : R;  // std::max(h_i, h_j)
-                keep = r_less(r2, H) && r2 != 0.0;
-            } else {
-                keep = r2 < R * R;
+                const double ks = a.nks[node];
+                rn = ks > rn ? ks : rn;
             }
-            if (valid && keep) {
-                if (count < a.cap) a.idx[(size_t)count * a.n + i] = j;
-                else over = true;
-                ++count;
+            // conservative reach test, padded against rounding
+            const double lim = (0.5 * e + rn) * (1.0 + 1e-12) + 1e-300;
+            bool reach = true;
+#pragma unroll
+            for (int k = 0; k < D; ++k) {
+                const double c = a.ncenter[(size_t)k * a.node_cap + node];
+                const double d = tabs(rij_axis<double>(c, g[k], a.box, k)) - hb[k];
+                reach = reach && (d <= lim);
             }
+            const int lv = a.nlevel[node];
+            if (!reach) {
+                node = a.nskip[node];
+                continue;
+            }
+            if (lv > 0) {  // internal node: descend
+                ++node;
+                continue;
+            }
+            const int s = a.nstart[node], en = a.nend[node];
+            tests += en - s;
+            for (int j = s; j < en; ++j) {
+                if (!mine) continue;
+                // FP32 prefilter; the FP64 predicate decides inside its band
+                const float4 pj = a.posf[j];
+                const float pjx[3] = {pj.x, pj.y, pj.z};
+                float r2f = 0.f;
+#pragma unroll
+                for (int k = 0; k < D; ++k) {
+                    const float d = rij_axis<float>(xf[k], pjx[k], a.box, k);
+                    r2f = (k == 0) ? d * d : r2f + d * d;
+                }
+                int band = fp32_band(r2f, IS_IJ ? fmaxf(Rf, pj.w) : Rf, mk);
+                if (IS_IJ && r2f == 0.f) band = 0;  // r_ij != 0 needs the FP64 test
+                bool keep = band > 0;
+                if (band == 0) {
+                    double xj[3] = {0, 0, 0};
+#pragma unroll
+                    for (int k = 0; k < D; ++k) xj[k] = a.x[k][j];
+                    const double r2 = r2_exact<D>(xi, xj, a.box);
+                    if (IS_IJ) {
+                        const double hj = a.sml[j];
+                        const double H = (R < hj) ? hj : R;  // std::max(h_i, h_j)
+                        keep = r_less(r2, H) && r2 != 0.0;
+                    } else {
+                        keep = r2 < R * R;
+                    }
+                }
+                if (keep) {
+                    if (count < a.cap) a.idx[(size_t)count * a.n + i] = j;
+                    else over = true;
+                    ++count;
+                }
+            }
+            ++node;
         }
-        ++node;
+        sub = sub_end;
     }
     if (valid) a.cnt[i] = count < a.cap ? count : a.cap;
     if (over) atomicOr(a.overflow, 1);
@@ -193,6 +234,8 @@ int search(sphgpu_ctx *c, Lists &l, int is_ij, const double *radius)
         for (int k = 0; k < 3; ++k) a.x[k] = c->pos[k < c->dim ? k : 0];
         a.radius = radius;
         a.sml = c->sc[S_SML];
+        a.posf = c->rec32[R_POSH];
+        a.root = t.root;
         a.nstart = t.start; a.nend = t.end; a.nskip = t.skip; a.nlevel = t.level;
         a.ncenter = t.center; a.nedge = t.edge; a.nks = t.ks; a.node_cap = t.node_cap;
         a.box = c->box;
