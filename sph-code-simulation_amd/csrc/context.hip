@@ -457,17 +457,18 @@ int sphgpu_neighbor_sets(sphgpu_ctx *c, const double *h, int is_ij, int cap, int
     HIPCHK(c, hipMemcpyAsync(c->hs, c->sc[S_SML], sizeof(double) * c->n, hipMemcpyDeviceToDevice, c->stream));
     Lists &l = is_ij ? c->lforce : c->lpre;
     if ((rc = search(c, l, is_ij, c->hs))) return rc;
-    std::vector<int> ids(c->n), cnt(c->n), idx((size_t)l.cap * c->n);
+    const size_t list_len = (size_t)l.cap * (((size_t)c->n + 63) / 64) * 64;
+    std::vector<int> ids(c->n), cnt(c->n), idx(list_len);
     HIPCHK(c, hipMemcpyAsync(ids.data(), c->id, sizeof(int) * c->n, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipMemcpyAsync(cnt.data(), l.cnt, sizeof(int) * c->n, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipMemcpyAsync(idx.data(), l.idx, sizeof(int) * (size_t)l.cap * c->n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(idx.data(), l.idx, sizeof(int) * list_len, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     for (int k = 0; k < c->n; ++k) {
         const int i = ids[k];
         const int m = cnt[k];
         if (m > cap) return fail(c, SPHGPU_E_NEIGHBOR_OVERFLOW, "neighbor_sets: cap too small");
         counts[i] = m;
-        for (int s = 0; s < m; ++s) lists[(size_t)i * cap + s] = ids[idx[(size_t)s * c->n + k]];
+        for (int s = 0; s < m; ++s) lists[(size_t)i * cap + s] = ids[idx[list_slot(k, s, l.cap)]];
     }
     return 0;
 }

@@ -78,8 +78,17 @@ struct Tree {
     int hist_blocks = 0;
 };
 
+// Neighbour-list layout: particles in blocks of 64 (one wavefront), slot-major
+// inside a block.  A wavefront's slot-q loads are one 256-byte segment, and
+// the list block a wavefront writes during the search is contiguous
+// (cap * 256 bytes), which keeps its scattered appends on few pages.
+SPH_DEV __host__ inline size_t list_slot(int i, int q, int cap)
+{
+    return ((size_t)(i >> 6) * (size_t)cap + (size_t)q) * 64 + (size_t)(i & 63);
+}
+
 struct Lists {
-    int *idx = nullptr;   // [cap][n] sorted indices (slot-major for coalescing)
+    int *idx = nullptr;   // see list_slot; ceil(n/64) * cap * 64 entries
     int *cnt = nullptr;   // [n]
     int cap = 0;
 };

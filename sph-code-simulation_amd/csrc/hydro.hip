@@ -33,6 +33,7 @@ template <typename T> struct PreArgs {
     int n;
     int *idx;                     // compacted in place: members first
     const int *cnt;
+    int cap;
     RecView<T> rv;                // neighbour records (read)
     T *rec_w[R_NREC];             // the same records as scalars (own-record writes)
     float *posf_w;                // FP32 position/h records, written in FP64 mode
@@ -122,7 +123,7 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
             const T hb = h;
             Acc<T> dn, ddn;
             for (int q = 0; q < nc; ++q) {
-                const int j = a.idx[(size_t)q * a.n + i];
+                const int j = a.idx[list_slot(i, q, a.cap)];
                 T d[3];
                 const T r = tsqrt(rec_r2<D, T>(RP[j], xi, a.box, d));
                 if (r >= h) continue;
@@ -154,7 +155,7 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
     T vsig = ci * T(2.0);
     int w = 0;
     for (int q = 0; q < nc; ++q) {
-        const int j = a.idx[(size_t)q * a.n + i];
+        const int j = a.idx[list_slot(i, q, a.cap)];
         T d[3];
         const T r2 = rec_r2<D, T>(RP[j], xi, a.box, d);
         bool in;
@@ -165,7 +166,7 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
             in = band > 0 || (band == 0 && r_less(r2_64(a.x64, xi64, j, a.box, D), h64));
         }
         if (!in) continue;
-        a.idx[(size_t)w * a.n + i] = j;  // compact: members first
+        a.idx[list_slot(i, w, a.cap)] = j;  // compact: members first
         ++w;
         const T r = tsqrt(r2);
         const T wk = Kern<D, K, T>::w(r, h);
@@ -230,7 +231,7 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
         if (!a.gsph2) return;
         T dd[3] = {0, 0, 0}, du[3] = {0, 0, 0}, dv[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
         for (int q = 0; q < w; ++q) {
-            const int j = a.idx[(size_t)q * a.n + i];
+            const int j = a.idx[list_slot(i, q, a.cap)];
             T d[3];
             const T r = tsqrt(rec_r2<D, T>(RP[j], xi, a.box, d));
             const T cw = Kern<D, K, T>::dwc(r, h);
@@ -280,7 +281,7 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
     if (do_bal || a.td_av) {
         T div = T(0), rot[3] = {0, 0, 0};
         for (int q = 0; q < w; ++q) {
-            const int j = a.idx[(size_t)q * a.n + i];
+            const int j = a.idx[list_slot(i, q, a.cap)];
             T d[3];
             const T r = tsqrt(rec_r2<D, T>(RP[j], xi, a.box, d));
             const T cw = Kern<D, K, T>::dwc(r, h);
@@ -340,7 +341,7 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
 
 // ---- initial_smoothing (pre_interaction.cpp:171-215) ----------------------------
 template <int D, int K>
-__global__ __launch_bounds__(256) void k_initial_density(int n, const int *__restrict__ idx, const int *__restrict__ cnt,
+__global__ __launch_bounds__(256) void k_initial_density(int n, int cap, const int *__restrict__ idx, const int *__restrict__ cnt,
                                                          const double *const *x, const double *__restrict__ mass,
                                                          const double *__restrict__ sml, double *__restrict__ dens,
                                                          Box box)
@@ -353,7 +354,7 @@ __global__ __launch_bounds__(256) void k_initial_density(int n, const int *__res
     double s = 0.0;
     const int nc = cnt[i];
     for (int q = 0; q < nc; ++q) {
-        const int j = idx[(size_t)q * n + i];
+        const int j = idx[list_slot(i, q, cap)];
         const double r = __builtin_sqrt(r2_64(x, xi, j, box, D));
         if (r >= h) continue;
         s += mass[j] * Kern<D, K, double>::w(r, h);
@@ -383,6 +384,7 @@ __global__ void k_store_hvs(const unsigned long long *m, double *dscal) { dscal[
 template <typename T> struct ForceArgs {
     int n;
     const int *idx, *cnt;
+    int cap;
     RecView<T> rv;
     double *acc[3];
     double *dene;
@@ -439,7 +441,7 @@ __global__ __launch_bounds__(256) void k_force_sph(ForceArgs<T> a)
     Acc<T> ac[3], de;
     const int nc = a.cnt[i];
     for (int q = 0; q < nc; ++q) {
-        const int j = a.idx[(size_t)q * a.n + i];
+        const int j = a.idx[list_slot(i, q, a.cap)];
         const V4 pj = RP[j], vj = RV[j], tj = RT[j], aj = RA[j];
         T d[3];
         const T r = tsqrt(rec_r2<D, T>(pj, xi, a.box, d));
@@ -583,7 +585,7 @@ SPH_DEV void force_gsph_slots(const ForceArgs<T> &a, int i, int nc, int nmax, co
         T left[4] = {0, 1, 1, 1}, right[4] = {0, 1, 1, 1};
         T rhoj = T(1), hj = T(1), mj = T(0);
         if (act) {
-            j = a.idx[(size_t)q * a.n + i];
+            j = a.idx[list_slot(i, q, a.cap)];
             const V4 pj = RP[j], vjr = RV[j], tj = RT[j];
             r = tsqrt(rec_r2<D, T>(pj, xi, a.box, d));
             r_inv = T(1.0) / r;
@@ -810,6 +812,7 @@ template <int D, int K, int TYPE, typename T> void launch_pre(sphgpu_ctx *c)
     a.n = c->n;
     a.idx = c->lpre.idx;
     a.cnt = c->lpre.cnt;
+    a.cap = c->lpre.cap;
     a.rv = rec_view<T>(c);
     for (int r = 0; r < R_NREC; ++r) a.rec_w[r] = (T *)a.rv.r[r];
     a.posf_w = (sizeof(T) == 8) ? (float *)c->rec32[R_POSH] : nullptr;
@@ -868,16 +871,16 @@ template <int D> void launch_initial_density(sphgpu_ctx *c, const double *const 
     const dim3 g(grid_for(c->n, 256)), b(256);
     switch (c->prm.kernel) {
     case SPHGPU_CUBIC_SPLINE:
-        hipLaunchKernelGGL((k_initial_density<D, SPHGPU_CUBIC_SPLINE>), g, b, 0, c->stream, c->n, c->lpre.idx,
+        hipLaunchKernelGGL((k_initial_density<D, SPHGPU_CUBIC_SPLINE>), g, b, 0, c->stream, c->n, c->lpre.cap, c->lpre.idx,
                            c->lpre.cnt, xdev, c->sc[S_MASS], c->sc[S_SML], c->sc[S_DENS], c->box);
         break;
     case SPHGPU_WENDLAND:
         if (D > 1)
-            hipLaunchKernelGGL((k_initial_density<D, SPHGPU_WENDLAND>), g, b, 0, c->stream, c->n, c->lpre.idx,
+            hipLaunchKernelGGL((k_initial_density<D, SPHGPU_WENDLAND>), g, b, 0, c->stream, c->n, c->lpre.cap, c->lpre.idx,
                                c->lpre.cnt, xdev, c->sc[S_MASS], c->sc[S_SML], c->sc[S_DENS], c->box);
         break;
     default:
-        hipLaunchKernelGGL((k_initial_density<D, SPHGPU_GAUSSIAN>), g, b, 0, c->stream, c->n, c->lpre.idx,
+        hipLaunchKernelGGL((k_initial_density<D, SPHGPU_GAUSSIAN>), g, b, 0, c->stream, c->n, c->lpre.cap, c->lpre.idx,
                            c->lpre.cnt, xdev, c->sc[S_MASS], c->sc[S_SML], c->sc[S_DENS], c->box);
     }
 }
@@ -902,6 +905,7 @@ template <typename T> ForceArgs<T> force_args(sphgpu_ctx *c)
     a.n = c->n;
     a.idx = c->lforce.idx;
     a.cnt = c->lforce.cnt;
+    a.cap = c->lforce.cap;
     a.rv = rec_view<T>(c);
     for (int k = 0; k < 3; ++k) a.acc[k] = c->acc[k];
     a.dene = c->sc[S_DENE];

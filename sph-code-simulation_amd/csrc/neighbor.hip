@@ -183,7 +183,7 @@ __global__ __launch_bounds__(256) void k_search(SearchArgs a)
                     }
                 }
                 if (keep) {
-                    if (count < a.cap) a.idx[(size_t)count * a.n + i] = j;
+                    if (count < a.cap) a.idx[list_slot(i, count, a.cap)] = j;
                     else over = true;
                     ++count;
                 }
@@ -210,7 +210,8 @@ int lists_alloc(sphgpu_ctx *c, Lists &l, int cap)
 {
     lists_free(l);
     l.cap = cap;
-    if (hipMalloc((void **)&l.idx, sizeof(int) * (size_t)cap * c->n) != hipSuccess) return SPHGPU_E_NOMEM;
+    const size_t blocks = ((size_t)c->n + 63) / 64;
+    if (hipMalloc((void **)&l.idx, sizeof(int) * (size_t)cap * blocks * 64) != hipSuccess) return SPHGPU_E_NOMEM;
     if (hipMalloc((void **)&l.cnt, sizeof(int) * (size_t)c->n) != hipSuccess) return SPHGPU_E_NOMEM;
     if (hipMemset(l.cnt, 0, sizeof(int) * (size_t)c->n) != hipSuccess) return SPHGPU_E_HIP;
     return 0;

@@ -1,0 +1,68 @@
+"""Training the learned Riemann starting guess from the solver's own pair log.
+
+The GSPH force kernel can log the left/right states and the converged p* of
+the pairs it solves (sphgpu_riemann_log_enable / _read).  The network is then
+trained on the GPU by the library's bf16 MFMA forward/backward kernels
+(sphgpu_mlp_train) on the pairs that the force kernel hands to the network:
+those with a pressure ratio above 4 or a velocity jump above the mean sound
+speed (the same test as the `hard` gate in the force kernel).
+"""
+from __future__ import annotations
+
+import time
+
+import numpy as np
+
+import sphgpu
+
+
+def hard_rows(rows: np.ndarray, gamma: float) -> np.ndarray:
+    """Rows (uL, rhoL, pL, uR, rhoR, pR, p*) that the force kernel sends to
+    the network, with finite entries and positive p*."""
+    pl, pr = rows[:, 2], rows[:, 5]
+    pmin, pmax = np.minimum(pl, pr), np.maximum(pl, pr)
+    du = rows[:, 0] - rows[:, 3]
+    r_m, p_m = 0.5 * (rows[:, 1] + rows[:, 4]), 0.5 * (pl + pr)
+    with np.errstate(invalid="ignore", over="ignore"):
+        hard = (pmin > 0) & (r_m > 0) & ((pmax > 4 * pmin) | (du * du * r_m > gamma * p_m))
+    return rows[hard & np.isfinite(rows).all(axis=1) & (rows[:, 6] > 0)]
+
+
+def log_pairs(params: dict, fields: np.ndarray, steps: int, capacity: int, stride: int = 1) -> np.ndarray:
+    """Run `steps` steps of the given problem with the plain guess and return
+    the logged pair rows."""
+    p = dict(params)
+    p["riemann_guess"] = 0
+    ctx = sphgpu.Context(p, fields.shape[1])
+    try:
+        ctx.upload(fields)
+        ctx.initialize()
+        ctx.riemann_log_enable(capacity, stride)
+        ctx.integrate(steps)
+        return ctx.riemann_log_read(capacity)
+    finally:
+        ctx.close()
+
+
+def train(params: dict, rows: np.ndarray, epochs: int = 4, batch: int = 4096, lr: float = 2e-3,
+          seed: int = 7) -> tuple[np.ndarray, dict]:
+    """Train the network on the hard rows; returns the FP32 parameter block
+    and a summary (initial and final loss, row counts, seconds)."""
+    hard = hard_rows(rows, params["gamma"])
+    if hard.shape[0] == 0:
+        raise ValueError("no hard pairs in the log")
+    hard = hard[np.random.default_rng(seed).permutation(hard.shape[0])]
+    ctx = sphgpu.Context(params, 64)  # the network needs no particle state
+    try:
+        ctx.mlp_init(seed)
+        loss0 = ctx.mlp_loss_grad(hard[: min(hard.shape[0], 200_000)])[0]
+        t0 = time.perf_counter()
+        loss = ctx.mlp_train(hard, epochs, batch, lr)
+        t1 = time.perf_counter()
+        w = ctx.mlp_get_params()
+    finally:
+        ctx.close()
+    info = {"pairs_logged": int(rows.shape[0]), "hard_pairs_trained": int(hard.shape[0]), "epochs": epochs,
+            "batch": batch, "lr": lr, "loss_initial": round(float(loss0), 5), "loss_final": round(float(loss), 5),
+            "train_s": round(t1 - t0, 2)}
+    return w, info

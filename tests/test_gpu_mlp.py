@@ -77,3 +77,51 @@ def test_mlp_training_lowers_loss():
     l1 = c.mlp_train(rows, epochs=30, batch=256, lr=2e-3)
     print(f"loss {l0:.4e} -> {l1:.4e}")
     assert l1 < 0.1 * l0
+
+
+def _run_sedov(p, f, steps, guess, params=None):
+    q = dict(p)
+    q["riemann_guess"] = guess
+    c = sphgpu.Context(q, f.shape[1])
+    c.upload(f)
+    if params is not None:
+        c.mlp_set_params(params)
+    c.initialize()
+    c.reset_stats()
+    c.integrate(steps)
+    st = c.stats()
+    out = c.download()
+    c.close()
+    return st, out
+
+
+def test_learned_guess_gains_over_plain_guess():
+    """What the trained network gains: on a 24^3 Sedov blast the network is
+    trained on the pair states logged from a plain-guess run, then the same
+    steps run with the learned guess.  Newton iterations per Riemann call go
+    down, the fallback rate stays small, and the fields equal the plain-guess
+    run's to FP32 tolerance (both are converged to the solver's tolerance and
+    refined by the same final FP64 Newton step)."""
+    import learned
+    p, f = ics.sedov_3d(n1=24)
+    rows = learned.log_pairs(p, f, steps=3, capacity=4_000_000)
+    w, info = learned.train(p, rows, epochs=10, batch=1024, lr=2e-3)
+    print("training", info)
+    assert info["loss_final"] < 0.2 * info["loss_initial"]
+    st_p, out_p = _run_sedov(p, f, 3, ics.GUESS_PLAIN)
+    st_l, out_l = _run_sedov(p, f, 3, ics.GUESS_LEARNED, w)
+    it_p = st_p["riemann_iters"] / st_p["riemann_calls"]
+    it_l = st_l["riemann_iters"] / st_l["riemann_calls"]
+    used = st_l["riemann_learned"] + st_l["riemann_fallbacks"]
+    print(f"iterations per call: plain {it_p:.4f} learned {it_l:.4f}; learned guesses used {used} "
+          f"of {st_l['riemann_calls']} calls, fallbacks {st_l['riemann_fallbacks']}")
+    assert used > 0
+    assert it_l < it_p
+    assert st_l["riemann_fallbacks"] <= 0.05 * used
+    P, rho, h = out_p[ics.F_PRES], out_p[ics.F_DENS], out_p[ics.F_SML]
+    da = np.sqrt(((out_l[ics.F_ACC:ics.F_ACC + 3] - out_p[ics.F_ACC:ics.F_ACC + 3]) ** 2).sum(axis=0))
+    e_acc = da / (np.abs(P) / (rho * h) + np.sqrt((out_p[ics.F_ACC:ics.F_ACC + 3] ** 2).sum(axis=0)))
+    e_rho = np.abs(out_l[ics.F_DENS] - rho) / rho
+    print(f"learned vs plain: acc {e_acc.max():.3e} (p99.9 {np.quantile(e_acc, 0.999):.3e}) dens {e_rho.max():.3e}")
+    assert np.quantile(e_acc, 0.999) < 5e-5
+    assert np.quantile(e_rho, 0.999) < 5e-5

@@ -159,8 +159,17 @@ def _compare_fields(case, p, gf, of, prec):
     assert np.quantile(e_pres, 0.999) < tol_rho
     assert np.quantile(e_acc, 0.999) < tol_f
     assert np.quantile(e_dene, 0.999) < tol_f
-    if t != ics.GSPH:
+    if t == ics.SSPH:
         e_gh = rel_err(gf[ics.F_GRADH], of[ics.F_GRADH])
+        assert np.quantile(e_gh, 0.999) < tol_rho * 10
+    elif t == ics.DISPH:
+        # DISPH grad-h (d_pre_interaction.cpp) is a ratio whose denominator
+        # 1 + h/(D n) sum dW/dh nearly cancels on a uniform lattice, so its
+        # relative error is not meaningful; it enters the force only through
+        # f_ij = 1 - gradh_i / (m_j u_j), so it is compared on the m u scale
+        mu = of[ics.F_MASS] * of[ics.F_ENE]
+        e_gh = np.abs(gf[ics.F_GRADH] - of[ics.F_GRADH]) / (np.abs(of[ics.F_GRADH]) + mu)
+        print(f"  gradh on the m u scale {e_gh.max():.3e}")
         assert np.quantile(e_gh, 0.999) < tol_rho * 10
     if p["use_gravity"]:
         e_phi = rel_err(gf[ics.F_PHI], of[ics.F_PHI])
@@ -228,9 +237,18 @@ def test_riemann_batch_matches_oracle():
         ps32, vs32, it32 = g.riemann_batch(solver, ics.PAIR_FP32, L, R)
         ref = np.array([oracle_riemann(solver, gam, L[k], R[k])[:2] for k in range(n)])
         e64 = rel_err(ps64, ref[:, 0]).max()
-        e32 = rel_err(ps32, ref[:, 0]).max()
         ev = np.abs(vs64 - ref[:, 1]).max()
-        print(f"solver {solver}: p* fp64 {e64:.3e} fp32 {e32:.3e} v* fp64 {ev:.3e}")
+        # FP32: near vacuum (p* << p_L, p_R) p* is set by the cancellation of
+        # terms of size p_L, p_R, and the FP32 rounding of the input states
+        # alone moves it by ~1e-7 max(p_L, p_R); errors are therefore measured
+        # on the pair's pressure scale max(p*, p_L, p_R), the scale on which
+        # p* enters the force sum
+        pscale = np.maximum(np.abs(ref[:, 0]), np.maximum(L[:, 2], R[:, 2]))
+        e32 = (np.abs(ps32 - ref[:, 0]) / pscale).max()
+        vscale = np.abs(L[:, 0]) + np.abs(R[:, 0]) + L[:, 3] + R[:, 3]
+        ev32 = (np.abs(vs32 - ref[:, 1]) / vscale).max()
+        print(f"solver {solver}: p* fp64 {e64:.3e} fp32 {e32:.3e} v* fp64 {ev:.3e} fp32 {ev32:.3e}")
         assert e64 < 1e-13
         assert ev < 1e-12
-        assert e32 < 1e-4
+        assert e32 < 1e-5
+        assert ev32 < 1e-5
