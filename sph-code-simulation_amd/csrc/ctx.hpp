@@ -65,6 +65,10 @@ struct Tree {
     double *ks = nullptr;            // kernel_size (max h in subtree)
     double *mass = nullptr;
     double *com = nullptr;           // [3][cap]
+    // FP32 walk records for the neighbour search: (cx, cy, cz, half edge
+    // rounded up) and (start, end, skip, leaf flag | kernel_size rounded up)
+    float4 *nbox = nullptr;
+    int4 *nlink = nullptr;
     // hierarchy of block reductions over sorted particles (64 and 4096)
     double *bmax1 = nullptr, *bmax2 = nullptr;
     double *bsum1 = nullptr, *bsum2 = nullptr;  // [4][nb] : m, mx, my, mz
@@ -76,6 +80,7 @@ struct Tree {
     // radix sort scratch
     unsigned int *hist = nullptr;
     int hist_blocks = 0;
+    unsigned int *scan_tmp = nullptr;  // tile sums of the multi-block scans
 };
 
 // Neighbour-list layout: particles in blocks of 64 (one wavefront), slot-major

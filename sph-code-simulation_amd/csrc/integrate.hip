@@ -94,14 +94,25 @@ __global__ void k_dt_partial(int n, int dim, const double *ax, const double *ay,
     if (threadIdx.x == 0) part[blockIdx.x] = s[0];
 }
 
-__global__ void k_dt_final(const double *__restrict__ part, int nb, double c_sound, double *dscal)
+// final minimum over the block minima (one workgroup of 256; min is exact in
+// any order)
+__global__ __launch_bounds__(256) void k_dt_final(const double *__restrict__ part, int nb, double c_sound,
+                                                  double *dscal)
 {
-    if (threadIdx.x != 0) return;
+    __shared__ double s[256];
     double m = 1.7976931348623157e308;
-    for (int b = 0; b < nb; ++b)
+    for (int b = threadIdx.x; b < nb; b += blockDim.x)
         if (part[b] < m) m = part[b];
-    const double dt_sound = c_sound * dscal[2];
-    dscal[0] = (m < dt_sound) ? m : dt_sound;  // std::min(dt_sound, dt_force)
+    s[threadIdx.x] = m;
+    __syncthreads();
+    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o && s[threadIdx.x + o] < s[threadIdx.x]) s[threadIdx.x] = s[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double dt_sound = c_sound * dscal[2];
+        dscal[0] = (s[0] < dt_sound) ? s[0] : dt_sound;  // std::min(dt_sound, dt_force)
+    }
 }
 
 __global__ void k_advance_time(double *dscal) { dscal[1] = dscal[1] + dscal[0]; }
@@ -136,23 +147,32 @@ __global__ void k_energy_partial(int n, int dim, int grav, const double *vx, con
         for (int q = 0; q < 6; ++q) part[blockIdx.x * 6 + q] = s[q][0];
 }
 
-__global__ void k_energy_final(const double *__restrict__ part, int nb, double *__restrict__ out)
+// final sums over the block partials: fixed thread-to-partial assignment and a
+// fixed tree order, so the result is deterministic
+__global__ __launch_bounds__(256) void k_energy_final(const double *__restrict__ part, int nb, double *__restrict__ out)
 {
-    if (threadIdx.x != 0) return;
+    __shared__ double s[6][256];
     double a[6] = {0, 0, 0, 0, 0, 0};
-    for (int b = 0; b < nb; ++b)
+    for (int b = threadIdx.x; b < nb; b += blockDim.x)
         for (int q = 0; q < 6; ++q) a[q] += part[b * 6 + q];
-    out[0] = a[0];
-    out[1] = a[1];
-    out[2] = a[2];
-    out[3] = a[0] + a[1] + a[2];
-    out[4] = a[3];
-    out[5] = a[4];
-    out[6] = a[5];
+    for (int q = 0; q < 6; ++q) s[q][threadIdx.x] = a[q];
+    __syncthreads();
+    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o)
+            for (int q = 0; q < 6; ++q) s[q][threadIdx.x] += s[q][threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = s[0][0];
+        out[1] = s[1][0];
+        out[2] = s[2][0];
+        out[3] = s[0][0] + s[1][0] + s[2][0];
+        out[4] = s[3][0];
+        out[5] = s[4][0];
+        out[6] = s[5][0];
+    }
 }
 
-// pair records from the FP64 masters: 32 values per particle in record
-// order (see RecField); null sources read as zero
 struct RecArgs {
     int n, nrec;                  // records written: R_AUX + 1, or all with gradients
     const double *src[4 * R_NREC];
@@ -234,7 +254,7 @@ int timestep(sphgpu_ctx *c)
     time_begin(c, "timestep");
     hipLaunchKernelGGL(k_dt_partial, dim3(nb), dim3(256), 0, c->stream, c->n, c->dim, ax, ay, az,
                        (const double *)c->sc[S_SML], c->prm.cfl_force, c->red);
-    hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(64), 0, c->stream, (const double *)c->red, nb, c->prm.cfl_sound,
+    hipLaunchKernelGGL(k_dt_final, dim3(1), dim3(256), 0, c->stream, (const double *)c->red, nb, c->prm.cfl_sound,
                        c->dscal);
     time_end(c);
     HIPCHK(c, hipGetLastError());
@@ -255,7 +275,7 @@ int energy_momentum(sphgpu_ctx *c, double out[7])
     hipLaunchKernelGGL(k_energy_partial, dim3(nb), dim3(256), 0, c->stream, c->n, c->dim, c->prm.use_gravity, vx, vy,
                        vz, (const double *)c->sc[S_MASS], (const double *)c->sc[S_ENE], (const double *)c->sc[S_PHI],
                        c->red);
-    hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(64), 0, c->stream, (const double *)c->red, nb, c->dscal + 8);
+    hipLaunchKernelGGL(k_energy_final, dim3(1), dim3(256), 0, c->stream, (const double *)c->red, nb, c->dscal + 8);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipMemcpyAsync(c->hscal + 8, c->dscal + 8, sizeof(double) * 7, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));

@@ -10,13 +10,17 @@
 //
 // Here a 64-lane wavefront owns 64 consecutive particles in key order (a
 // compact region of space).  The wave walks the pre-ordered node array
-// stacklessly: a node is entered when the group's bounding box, widened by
-// the group's largest radius (and the node's kernel_size for is_ij), can reach
-// the node cell; otherwise the walk jumps to the node's skip index.  In an
-// entered leaf every lane evaluates the reference's FP64 predicate for its own
-// particle (behind an FP32 prefilter that only settles candidates whose FP32
-// distance lies outside a proven rounding band), so the resulting sets are
-// exactly the reference's:
+// stacklessly, 64 nodes per step: lane q tests node w0 + q against the
+// group's FP32 bounding box widened by the group's largest radius (and the
+// node's kernel_size for is_ij), conservatively padded for FP32 rounding; the
+// walk then advances through the window on the wave-uniform reach mask
+// (entered -> next node, pruned -> skip) and loads the next window where it
+// leaves this one.  Members of entered leaves are queued, 64 at a time, in a
+// per-lane candidate buffer; a flush loads the 64 candidate records in one
+// gather and broadcasts them lane by lane, and every lane evaluates the
+// reference's FP64 predicate for its own particle (behind an FP32 prefilter
+// that only settles candidates whose FP32 distance lies outside a proven
+// rounding band), so the resulting sets are exactly the reference's:
 //   is_ij = 0 :  r_ij^2 < hs_i^2
 //   is_ij = 1 :  sqrt(r_ij^2) < max(h_i, h_j)  and  r_ij^2 != 0
 // The group test only prunes; it never decides membership.
@@ -36,10 +40,8 @@ struct SearchArgs {
     const double *sml;        // h_j for is_ij
     const float4 *posf;       // FP32 (x, y, z, h) records for the prefilter
     const double *root;       // tree root geometry (bounds |x|)
-    const int *nstart, *nend, *nskip, *nlevel;
-    const double *ncenter;    // [3][cap]
-    const double *nedge, *nks;
-    int node_cap;
+    const float4 *nbox;       // FP32 node boxes (centre, half edge rounded up)
+    const int4 *nlink;        // start, end, skip, leaf flag | kernel_size
     Box box;
     int *idx;                 // [cap][n]
     int *cnt;
@@ -60,25 +62,20 @@ template <int D> SPH_DEV double r2_exact(const double xi[3], const double xj[3],
     return s;
 }
 
-template <int D> SPH_DEV double wave_min(double v)
-{
-    for (int off = kWave / 2; off > 0; off >>= 1) {
-        const double o = __shfl_xor(v, off, kWave);
-        v = o < v ? o : v;
-    }
-    return v;
-}
-template <int D> SPH_DEV double wave_max(double v)
-{
-    for (int off = kWave / 2; off > 0; off >>= 1) {
-        const double o = __shfl_xor(v, off, kWave);
-        v = o > v ? o : v;
-    }
-    return v;
-}
-
 // particles per "chunk" node: a wavefront walks once per chunk it touches
 constexpr int kChunk = 128;
+
+SPH_DEV float rdl(float v, int lane) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane)); }
+SPH_DEV int rdl(int v, int lane) { return __builtin_amdgcn_readlane(v, lane); }
+
+// |minimum-image distance| along one axis in FP32: r^2 only needs |d|, and
+// |d| < L inside the box, so min(|d|, L - |d|) is the minimum image (the
+// reference's tie order only decides the sign)
+SPH_DEV float abs_min_image_f(float d, int periodic, float L)
+{
+    const float a = fabsf(d);
+    return periodic ? fminf(a, fabsf(L - a)) : a;
+}
 
 template <int D, int IS_IJ>
 __global__ __launch_bounds__(256) void k_search(SearchArgs a)
@@ -97,11 +94,76 @@ __global__ __launch_bounds__(256) void k_search(SearchArgs a)
     const float xf[3] = {pfi.x, pfi.y, pfi.z};
     const float Rf = (float)R;
     const float mk = fp32_margin_k(a.root, D);
+    // candidate band of this lane (for is_ij the band of H = max(h_i, h_j) >= h_i
+    // is at most this wide relative to H^2)
+    const float delta = mk / Rf + kMarginC;
+    const bool fast = (Rf * Rf > 1e-30f) && (delta < 0.5f);
+    const float blo = 1.0f - delta, bhi = 1.0f + delta;
+    const float R2lo = Rf * Rf * blo, R2hi = Rf * Rf * bhi;
+    float Lf[3] = {1.f, 1.f, 1.f};
+#pragma unroll
+    for (int k = 0; k < D; ++k) Lf[k] = a.box.rangef[k];
+    // reach-test padding: FP32 rounding of centres, positions and the box
+    // shift is below 10 u X; 2^-18 X and a relative 2^-18 cover it
+    double Xb = 0.0;
+    for (int k = 0; k < D; ++k) Xb = fmax(Xb, fabs(a.root[k]));
+    const float padX = (float)((Xb + a.root[3]) * 3.814697265625e-06);
     int count = 0;
     bool over = false;
     long long visits = 0, tests = 0;
     const int p_end = min(group * kWave + kWave, a.n);
     int sub = group * kWave;
+    // candidate buffer: lane q holds the q-th pending candidate index
+    int cand = 0, nb = 0;
+
+    // test the nb pending candidates against this lane's particle
+    auto flush = [&](bool mine) {
+        float4 pj = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (lane < nb) pj = a.posf[cand];
+        tests += nb;
+        for (int t = 0; t < nb; ++t) {
+            const int j = rdl(cand, t);
+            const float xj[3] = {rdl(pj.x, t), rdl(pj.y, t), rdl(pj.z, t)};
+            float r2f = 0.f;
+#pragma unroll
+            for (int k = 0; k < D; ++k) {
+                const float dk = abs_min_image_f(xf[k] - xj[k], a.box.periodic, Lf[k]);
+                r2f = (k == 0) ? dk * dk : r2f + dk * dk;
+            }
+            int band = 0;
+            if (fast) {
+                float lo2 = R2lo, hi2 = R2hi;
+                if (IS_IJ) {
+                    const float Hf = fmaxf(Rf, rdl(pj.w, t));
+                    lo2 = Hf * Hf * blo;
+                    hi2 = Hf * Hf * bhi;
+                }
+                band = (r2f < lo2) ? 1 : (r2f > hi2) ? -1 : 0;
+                if (IS_IJ && r2f == 0.f) band = 0;  // r_ij != 0 needs the FP64 test
+            }
+            bool keep = band > 0;
+            if (band == 0 && mine) {
+                double xj64[3] = {0, 0, 0};
+#pragma unroll
+                for (int k = 0; k < D; ++k) xj64[k] = a.x[k][j];
+                const double r2 = r2_exact<D>(xi, xj64, a.box);
+                if (IS_IJ) {
+                    const double hj = a.sml[j];
+                    const double H = (R < hj) ? hj : R;  // std::max(h_i, h_j)
+                    keep = r_less(r2, H) && r2 != 0.0;
+                } else {
+                    keep = r2 < R * R;
+                }
+            }
+            if (mine && keep) {
+                if (count < a.cap) a.idx[list_slot(i, count, a.cap)] = j;
+                else over = true;
+                ++count;
+            }
+        }
+        nb = 0;
+    };
+
     // Consecutive particles in key order can straddle the boundary of a large
     // cell, which would make the group's bounding box span that cell.  The
     // wave therefore splits its particles at the boundaries of "chunk" nodes
@@ -109,87 +171,83 @@ __global__ __launch_bounds__(256) void k_search(SearchArgs a)
     // and walks once per piece.
     while (sub < p_end) {
         int cn = 0;
-        while (a.nlevel[cn] > 0 && a.nend[cn] - a.nstart[cn] > kChunk) {
+        int4 lc = a.nlink[0];
+        while (lc.w >= 0 && lc.y - lc.x > kChunk) {
             int ch = cn + 1;
-            while (!(a.nstart[ch] <= sub && sub < a.nend[ch])) ch = a.nskip[ch];
+            int4 l2 = a.nlink[ch];
+            while (!(l2.x <= sub && sub < l2.y)) {
+                ch = l2.z;
+                l2 = a.nlink[ch];
+            }
             cn = ch;
+            lc = l2;
         }
-        const int sub_end = min(p_end, a.nend[cn]);
+        const int sub_end = min(p_end, lc.y);
         const bool mine = valid && i >= sub && i < sub_end;
-        // bounding box and radius of this piece
-        double g[3] = {0, 0, 0}, hb[3] = {0, 0, 0};
+        // FP32 bounding box and radius of this piece
+        float g[3] = {0, 0, 0}, hb[3] = {0, 0, 0};
 #pragma unroll
         for (int k = 0; k < D; ++k) {
-            const double lo = wave_min<D>(mine ? xi[k] : 1.7976931348623157e308);
-            const double hi = wave_max<D>(mine ? xi[k] : -1.7976931348623157e308);
-            g[k] = 0.5 * (lo + hi);
-            hb[k] = 0.5 * (hi - lo);
+            float lo = mine ? xf[k] : 3.0e38f, hi = mine ? xf[k] : -3.0e38f;
+            for (int off = kWave / 2; off > 0; off >>= 1) {
+                lo = fminf(lo, __shfl_xor(lo, off, kWave));
+                hi = fmaxf(hi, __shfl_xor(hi, off, kWave));
+            }
+            g[k] = 0.5f * (lo + hi);
+            hb[k] = 0.5f * (hi - lo);
         }
-        const double Rg = wave_max<D>(mine ? R : 0.0);
-        int node = 0;
-        while (node < a.n_nodes) {
-            ++visits;
-            const double e = a.nedge[node];
-            double rn = Rg;
-            if (IS_IJ) {
-                const double ks = a.nks[node];
-                rn = ks > rn ? ks : rn;
-            }
-            // conservative reach test, padded against rounding
-            const double lim = (0.5 * e + rn) * (1.0 + 1e-12) + 1e-300;
-            bool reach = true;
+        float Rg = mine ? Rf : 0.f;
+        for (int off = kWave / 2; off > 0; off >>= 1) Rg = fmaxf(Rg, __shfl_xor(Rg, off, kWave));
+
+        // Stackless pre-order walk evaluated 64 nodes at a time: lane q tests
+        // node w0 + q; the walk then advances through the window on the
+        // wave-uniform reach mask (descend = +1, prune = skip) and loads the
+        // next window where it leaves this one.
+        int w0 = 0;
+        while (w0 < a.n_nodes) {
+            const int k = w0 + lane;
+            int4 lk = make_int4(0, 0, a.n_nodes, 0);
+            bool reach = false;
+            if (k < a.n_nodes) {
+                const float4 bx = a.nbox[k];
+                lk = a.nlink[k];
+                float rn = Rg;
+                if (IS_IJ) rn = fmaxf(rn, fabsf(__int_as_float(lk.w)));
+                const float lim = (bx.w + rn) * 1.000003814697265625f + padX;
+                const float bc[3] = {bx.x, bx.y, bx.z};
+                reach = true;
 #pragma unroll
-            for (int k = 0; k < D; ++k) {
-                const double c = a.ncenter[(size_t)k * a.node_cap + node];
-                const double d = tabs(rij_axis<double>(c, g[k], a.box, k)) - hb[k];
-                reach = reach && (d <= lim);
-            }
-            const int lv = a.nlevel[node];
-            if (!reach) {
-                node = a.nskip[node];
-                continue;
-            }
-            if (lv > 0) {  // internal node: descend
-                ++node;
-                continue;
-            }
-            const int s = a.nstart[node], en = a.nend[node];
-            tests += en - s;
-            for (int j = s; j < en; ++j) {
-                if (!mine) continue;
-                // FP32 prefilter; the FP64 predicate decides inside its band
-                const float4 pj = a.posf[j];
-                const float pjx[3] = {pj.x, pj.y, pj.z};
-                float r2f = 0.f;
-#pragma unroll
-                for (int k = 0; k < D; ++k) {
-                    const float d = rij_axis<float>(xf[k], pjx[k], a.box, k);
-                    r2f = (k == 0) ? d * d : r2f + d * d;
+                for (int q = 0; q < D; ++q) {
+                    const float d = abs_min_image_f(bc[q] - g[q], a.box.periodic, Lf[q]) - hb[q];
+                    reach = reach && (d <= lim);
                 }
-                int band = fp32_band(r2f, IS_IJ ? fmaxf(Rf, pj.w) : Rf, mk);
-                if (IS_IJ && r2f == 0.f) band = 0;  // r_ij != 0 needs the FP64 test
-                bool keep = band > 0;
-                if (band == 0) {
-                    double xj[3] = {0, 0, 0};
-#pragma unroll
-                    for (int k = 0; k < D; ++k) xj[k] = a.x[k][j];
-                    const double r2 = r2_exact<D>(xi, xj, a.box);
-                    if (IS_IJ) {
-                        const double hj = a.sml[j];
-                        const double H = (R < hj) ? hj : R;  // std::max(h_i, h_j)
-                        keep = r_less(r2, H) && r2 != 0.0;
-                    } else {
-                        keep = r2 < R * R;
+            }
+            const unsigned long long mask = __ballot(reach);
+            int node = w0;
+            const int wend = min(w0 + kWave, a.n_nodes);
+            while (node < wend) {
+                const int off = node - w0;
+                ++visits;
+                if ((mask >> off) & 1ull) {
+                    if (rdl(lk.w, off) < 0) {  // entered leaf: queue its members
+                        int s = rdl(lk.x, off);
+                        const int e = rdl(lk.y, off);
+                        while (s < e) {
+                            const int take = min(e - s, kWave - nb);
+                            if (lane >= nb && lane < nb + take) cand = s + (lane - nb);
+                            nb += take;
+                            s += take;
+                            if (nb == kWave) flush(mine);
+                        }
                     }
-                }
-                if (keep) {
-                    if (count < a.cap) a.idx[list_slot(i, count, a.cap)] = j;
-                    else over = true;
-                    ++count;
+                    ++node;
+                } else {
+                    node = rdl(lk.z, off);
                 }
             }
-            ++node;
+            w0 = node;
         }
+        if (nb) flush(mine);
         sub = sub_end;
     }
     if (valid) a.cnt[i] = count < a.cap ? count : a.cap;
@@ -237,8 +295,7 @@ int search(sphgpu_ctx *c, Lists &l, int is_ij, const double *radius)
         a.sml = c->sc[S_SML];
         a.posf = c->rec32[R_POSH];
         a.root = t.root;
-        a.nstart = t.start; a.nend = t.end; a.nskip = t.skip; a.nlevel = t.level;
-        a.ncenter = t.center; a.nedge = t.edge; a.nks = t.ks; a.node_cap = t.node_cap;
+        a.nbox = t.nbox; a.nlink = t.nlink;
         a.box = c->box;
         a.idx = l.idx; a.cnt = l.cnt; a.cap = l.cap;
         a.overflow = c->dflag;

@@ -146,28 +146,92 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_hist(const unsigned long
     hist[threadIdx.x * nblocks + blockIdx.x] = cnt[threadIdx.x];
 }
 
-// exclusive scan of hist[256 * nblocks] (digit-major), single workgroup
-__global__ __launch_bounds__(1024) void k_radix_scan(unsigned int *__restrict__ hist, int total)
+// ---- multi-block exclusive scan of unsigned counts -----------------------------
+// Three passes: per-tile sums, a scan of the tile sums in one workgroup, and
+// the per-tile scan with its offset.  Tiles of 2048 values are loaded
+// coalesced and transposed through LDS so each thread scans 8 consecutive
+// values.  In place (in == out) is allowed.
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kScanThreads * kScanItems;
+
+__device__ unsigned block_excl_scan(unsigned v, unsigned &total, unsigned *wsum)
 {
-    __shared__ unsigned int part[1024];
-    const int t = threadIdx.x;
-    const int per = (total + 1023) / 1024;
-    const int lo = t * per, hi = min(total, lo + per);
-    unsigned int s = 0;
-    for (int i = lo; i < hi; ++i) s += hist[i];
-    part[t] = s;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        unsigned int v = (t >= off) ? part[t - off] : 0u;
-        __syncthreads();
-        part[t] += v;
-        __syncthreads();
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+    unsigned x = v;
+    for (int off = 1; off < kWave; off <<= 1) {
+        const unsigned y = __shfl_up(x, off, kWave);
+        if (lane >= off) x += y;
     }
-    unsigned int run = (t == 0) ? 0u : part[t - 1];
-    for (int i = lo; i < hi; ++i) {
-        const unsigned int v = hist[i];
-        hist[i] = run;
-        run += v;
+    if (lane == kWave - 1) wsum[w] = x;
+    __syncthreads();
+    unsigned base = 0;
+    total = 0;
+    for (int k = 0; k < kScanThreads / kWave; ++k) {
+        if (k < w) base += wsum[k];
+        total += wsum[k];
+    }
+    __syncthreads();
+    return base + x - v;
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_tiles(const unsigned *__restrict__ in, int n,
+                                                             unsigned *__restrict__ tile_sum)
+{
+    __shared__ unsigned wsum[kScanThreads / kWave];
+    const long long base = (long long)blockIdx.x * kScanTile;
+    unsigned s = 0;
+    for (int r = 0; r < kScanItems; ++r) {
+        const long long e = base + r * kScanThreads + threadIdx.x;
+        if (e < n) s += in[e];
+    }
+    unsigned total;
+    block_excl_scan(s, total, wsum);
+    if (threadIdx.x == 0) tile_sum[blockIdx.x] = total;
+}
+
+// exclusive scan of the tile sums (one workgroup); tile_sum[ntiles] = total
+__global__ __launch_bounds__(kScanThreads) void k_scan_tile_sums(unsigned *__restrict__ tile_sum, int ntiles)
+{
+    __shared__ unsigned wsum[kScanThreads / kWave];
+    unsigned carry = 0;
+    for (int b = 0; b < ntiles; b += kScanThreads) {
+        const int e = b + threadIdx.x;
+        const unsigned v = e < ntiles ? tile_sum[e] : 0u;
+        unsigned total;
+        const unsigned ex = block_excl_scan(v, total, wsum);
+        if (e < ntiles) tile_sum[e] = carry + ex;
+        carry += total;
+    }
+    if (threadIdx.x == 0) tile_sum[ntiles] = carry;
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_apply(const unsigned *in, int n, const unsigned *__restrict__ tile_off,
+                                                             unsigned *out)
+{
+    __shared__ unsigned tile[kScanTile];
+    __shared__ unsigned wsum[kScanThreads / kWave];
+    const long long base = (long long)blockIdx.x * kScanTile;
+    for (int r = 0; r < kScanItems; ++r) {
+        const int q = r * kScanThreads + threadIdx.x;
+        tile[q] = (base + q < n) ? in[base + q] : 0u;
+    }
+    __syncthreads();
+    unsigned v[kScanItems], s = 0;
+    for (int r = 0; r < kScanItems; ++r) {
+        v[r] = tile[threadIdx.x * kScanItems + r];
+        s += v[r];
+    }
+    unsigned total;
+    unsigned run = tile_off[blockIdx.x] + block_excl_scan(s, total, wsum);
+    for (int r = 0; r < kScanItems; ++r) {
+        tile[threadIdx.x * kScanItems + r] = run;
+        run += v[r];
+    }
+    __syncthreads();
+    for (int r = 0; r < kScanItems; ++r) {
+        const int q = r * kScanThreads + threadIdx.x;
+        if (base + q < n) out[base + q] = tile[q];
     }
 }
 
@@ -314,30 +378,6 @@ __global__ void k_lambda(const unsigned char *__restrict__ kdig, TreeShape sh, u
 }
 
 // exclusive scan of ncount -> nbase (n+1 entries), single workgroup
-__global__ __launch_bounds__(1024) void k_scan_int(const int *__restrict__ in, int *__restrict__ out, int n)
-{
-    __shared__ long long part[1024];
-    const int t = threadIdx.x;
-    const int per = (n + 1023) / 1024;
-    const int lo = t * per, hi = min(n, lo + per);
-    long long s = 0;
-    for (int i = lo; i < hi; ++i) s += in[i];
-    part[t] = s;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        long long v = (t >= off) ? part[t - off] : 0;
-        __syncthreads();
-        part[t] += v;
-        __syncthreads();
-    }
-    long long run = (t == 0) ? 0 : part[t - 1];
-    for (int i = lo; i < hi; ++i) {
-        out[i] = (int)min(run, (long long)0x7fffffff);
-        run += in[i];
-    }
-    if (t == 1023) out[n] = (int)min(part[1023], (long long)0x7fffffff);
-}
-
 // ---- 6. node fill ------------------------------------------------------------------
 __device__ int prefix_end(const unsigned long long *__restrict__ key, int n, int s, int shift)
 {
@@ -474,12 +514,38 @@ __device__ double range_max(RangeTables t, int s, int e)
     return m;
 }
 
+// positive float not below x (x >= 0)
+__device__ float float_up(double x)
+{
+    float f = (float)x;
+    if ((double)f < x) f = __int_as_float(__float_as_int(f) + 1);
+    return f;
+}
+
 __global__ void k_node_ks(const int *__restrict__ nstart, const int *__restrict__ nend, int total, RangeTables t,
-                          double *__restrict__ ks)
+                          double *__restrict__ ks, int4 *__restrict__ nlink)
 {
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= total) return;
-    ks[idx] = range_max(t, nstart[idx], nend[idx]);
+    const double k = range_max(t, nstart[idx], nend[idx]);
+    ks[idx] = k;
+    int4 l = nlink[idx];
+    l.w = (l.w & (int)0x80000000) | __float_as_int(float_up(k));
+    nlink[idx] = l;
+}
+
+// FP32 walk records of the neighbour search (kernel_size is added by
+// k_node_ks): centres rounded to nearest, half edges rounded up
+__global__ void k_node_records(int total, const int *__restrict__ nstart, const int *__restrict__ nend,
+                               const int *__restrict__ nskip, const int *__restrict__ nlevel,
+                               const double *__restrict__ ncenter, const double *__restrict__ nedge, int cap,
+                               float4 *__restrict__ nbox, int4 *__restrict__ nlink)
+{
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= total) return;
+    nbox[idx] = make_float4((float)ncenter[idx], (float)ncenter[(size_t)cap + idx], (float)ncenter[2 * (size_t)cap + idx],
+                            float_up(0.5 * nedge[idx]));
+    nlink[idx] = make_int4(nstart[idx], nend[idx], nskip[idx], nlevel[idx] < 0 ? (int)0x80000000 : 0);
 }
 
 struct SumTables {
@@ -521,6 +587,17 @@ __global__ void k_node_mass(const int *__restrict__ nstart, const int *__restric
 
 } // namespace
 
+// exclusive scan of in[0..n) into out[0..n), with out[n] = total when
+// write_total (out then has n + 1 entries); tmp holds ceil(n / tile) + 1 values
+static void scan_counts(hipStream_t st, const unsigned *in, unsigned *out, int n, unsigned *tmp, bool write_total)
+{
+    const int nt = (n + kScanTile - 1) / kScanTile;
+    hipLaunchKernelGGL(k_scan_tiles, dim3(nt), dim3(kScanThreads), 0, st, in, n, tmp);
+    hipLaunchKernelGGL(k_scan_tile_sums, dim3(1), dim3(kScanThreads), 0, st, tmp, nt);
+    hipLaunchKernelGGL(k_scan_apply, dim3(nt), dim3(kScanThreads), 0, st, in, n, (const unsigned *)tmp, out);
+    if (write_total) hipMemcpyAsync(out + n, tmp + nt, sizeof(unsigned), hipMemcpyDeviceToDevice, st);
+}
+
 // ------------------------------------------------------------------------------
 int tree_alloc(sphgpu_ctx *c)
 {
@@ -540,7 +617,10 @@ int tree_alloc(sphgpu_ctx *c)
               al((void **)&t.mass, 8ull * t.node_cap) && al((void **)&t.com, 24ull * t.node_cap) &&
               al((void **)&t.bmax1, 8ull * nb1) && al((void **)&t.bmax2, 8ull * nb2) &&
               al((void **)&t.bsum1, 32ull * nb1) && al((void **)&t.bsum2, 32ull * nb2) &&
-              al((void **)&t.root, 8ull * 8) && al((void **)&t.hist, 4ull * 256 * t.hist_blocks);
+              al((void **)&t.root, 8ull * 8) && al((void **)&t.hist, 4ull * 256 * t.hist_blocks) &&
+              al((void **)&t.nbox, 16ull * t.node_cap) && al((void **)&t.nlink, 16ull * t.node_cap);
+    const long long scan_max = std::max<long long>(n + 1, 256ll * t.hist_blocks);
+    ok = ok && al((void **)&t.scan_tmp, 4ull * ((scan_max + kScanTile - 1) / kScanTile + 1));
     if (!ok) return SPHGPU_E_NOMEM;
     if (c->prm.periodic) {
         double r[8] = {0};
@@ -561,7 +641,7 @@ void tree_free(sphgpu_ctx *c)
     Tree &t = c->tree;
     void *ps[] = {t.key, t.key_alt, t.perm, t.perm_alt, t.kdig, t.lam, t.leaf_end, t.ncount, t.nbase, t.start,
                   t.end, t.skip, t.level, t.center, t.edge, t.ks, t.mass, t.com, t.bmax1, t.bmax2, t.bsum1,
-                  t.bsum2, t.root, t.hist};
+                  t.bsum2, t.root, t.hist, t.nbox, t.nlink, t.scan_tmp};
     for (void *p : ps)
         if (p) hipFree(p);
     t = Tree();
@@ -632,7 +712,7 @@ int tree_build(sphgpu_ctx *c)
         const int shift = 8 * p;
         hipLaunchKernelGGL(k_radix_hist, dim3(nb), dim3(kSortThreads), 0, c->stream, (const unsigned long long *)t.key, n,
                            shift, t.hist, nb);
-        hipLaunchKernelGGL(k_radix_scan, dim3(1), dim3(1024), 0, c->stream, t.hist, 256 * nb);
+        scan_counts(c->stream, t.hist, t.hist, 256 * nb, t.scan_tmp, false);
         hipLaunchKernelGGL(k_radix_scatter, dim3(nb), dim3(kSortThreads), 0, c->stream,
                            (const unsigned long long *)t.key, (const int *)t.perm, t.key_alt, t.perm_alt, n, shift,
                            (const unsigned int *)t.hist, nb);
@@ -652,7 +732,7 @@ int tree_build(sphgpu_ctx *c)
                        t.kdig);
     hipLaunchKernelGGL(k_lambda, dim3(grid_for(n, 128)), dim3(128), 0, c->stream, (const unsigned char *)t.kdig, sh,
                        t.lam, t.leaf_end, t.ncount);
-    hipLaunchKernelGGL(k_scan_int, dim3(1), dim3(1024), 0, c->stream, (const int *)t.ncount, t.nbase, n);
+    scan_counts(c->stream, (const unsigned *)t.ncount, (unsigned *)t.nbase, n, t.scan_tmp, true);
     HIPCHK(c, hipGetLastError());
     int total = 0;
     HIPCHK(c, hipMemcpyAsync(&total, t.nbase + n, sizeof(int), hipMemcpyDeviceToHost, c->stream));
@@ -666,6 +746,9 @@ int tree_build(sphgpu_ctx *c)
                        (const unsigned char *)t.kdig, (const unsigned char *)t.lam, (const int *)t.leaf_end,
                        (const int *)t.nbase, (const double *)t.root, sh, total, t.start, t.end, t.skip, t.level,
                        t.center, t.edge, t.node_cap, c->dstats);
+    hipLaunchKernelGGL(k_node_records, dim3(grid_for(total, 256)), dim3(256), 0, c->stream, total,
+                       (const int *)t.start, (const int *)t.end, (const int *)t.skip, (const int *)t.level,
+                       (const double *)t.center, (const double *)t.edge, t.node_cap, t.nbox, t.nlink);
     HIPCHK(c, hipGetLastError());
     // mass and centre of mass per node (needed by the gravity walk); the
     // positions are read from the permuted (key-ordered) arrays
@@ -705,7 +788,7 @@ int tree_set_kernel(sphgpu_ctx *c)
                        (const double *)t.bmax1, nb1, t.bmax2);
     RangeTables rt{c->sc[S_SML], t.bmax1, t.bmax2};
     hipLaunchKernelGGL(k_node_ks, dim3(grid_for(t.n_nodes, 128)), dim3(128), 0, c->stream, (const int *)t.start,
-                       (const int *)t.end, t.n_nodes, rt, t.ks);
+                       (const int *)t.end, t.n_nodes, rt, t.ks, t.nlink);
     time_end(c);
     HIPCHK(c, hipGetLastError());
     return 0;
