@@ -195,6 +195,8 @@ def main() -> None:
         "tree_nodes": st["nodes"],
         "search_node_visits_per_wave": round(st["search_node_visits"] * 64 / (2 * res["n"] * args.steps), 1),
         "search_member_tests_per_wave": round(st["search_member_tests"] * 64 / (2 * res["n"] * args.steps), 1),
+        "search_windows_per_wave": round(st["search_windows"] * 64 / (2 * res["n"] * args.steps), 1),
+        "search_leaves_per_wave": round(st["search_leaves"] * 64 / (2 * res["n"] * args.steps), 1),
     }
     if train_info is not None:
         out["training"] = train_info

@@ -149,6 +149,8 @@ typedef struct sphgpu_stats {
     int64_t search_member_tests; /* leaf members tested per wavefront (summed) */
     int32_t max_depth;           /* deepest node level (root = 1)             */
     int32_t pad;
+    int64_t search_windows;      /* 64-node windows loaded by search wavefronts */
+    int64_t search_leaves;       /* leaves whose members a wavefront tested   */
 } sphgpu_stats;
 
 /* ---- lifecycle ------------------------------------------------------------ */

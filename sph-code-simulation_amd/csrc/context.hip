@@ -432,6 +432,8 @@ int sphgpu_get_stats(sphgpu_ctx *c, sphgpu_stats *out)
     out->max_depth = (int32_t)h[12];
     out->search_node_visits = h[13];
     out->search_member_tests = h[14];
+    out->search_windows = h[15];
+    out->search_leaves = h[16];
     return 0;
 }
 

@@ -65,9 +65,11 @@ struct Tree {
     double *ks = nullptr;            // kernel_size (max h in subtree)
     double *mass = nullptr;
     double *com = nullptr;           // [3][cap]
-    // FP32 walk records for the neighbour search: (cx, cy, cz, half edge
-    // rounded up) and (start, end, skip, leaf flag | kernel_size rounded up)
+    // FP32 walk records for the neighbour search: (cx, cy, cz, half extent x),
+    // (half extents y, z), all rounded up, and (start, end, skip, leaf flag |
+    // kernel_size rounded up); a leaf's box is its members' bounding box
     float4 *nbox = nullptr;
+    float2 *nhalf = nullptr;
     int4 *nlink = nullptr;
     // hierarchy of block reductions over sorted particles (64 and 4096)
     double *bmax1 = nullptr, *bmax2 = nullptr;

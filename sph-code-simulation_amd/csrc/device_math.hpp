@@ -19,6 +19,18 @@ namespace sphgpu {
 
 constexpr int kWave = 64;
 
+// Workgroups are dispatched round-robin over the 8 XCDs of an MI355X, each
+// with its own L2.  Particle kernels remap the workgroup index so that XCD x
+// runs one contiguous range of logical workgroups: particles consecutive in
+// key order (spatial neighbours, whose pair data overlap) then share an L2.
+constexpr int kNumXcd = 8;
+SPH_DEV int xcd_block(int b, int nb)
+{
+    const int q = nb / kNumXcd, r = nb % kNumXcd;
+    const int x = b % kNumXcd, k = b / kNumXcd;
+    return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + k;
+}
+
 template <typename T> SPH_DEV T tabs(T x) { return x < T(0) ? -x : x; }
 SPH_DEV float tsqrt(float x) { return __builtin_sqrtf(x); }
 SPH_DEV double tsqrt(double x) { return __builtin_sqrt(x); }

@@ -58,7 +58,7 @@ template <typename T> struct GravArgs {
 template <int D, typename T>
 __global__ __launch_bounds__(256) void k_gravity(GravArgs<T> a)
 {
-    const int i0 = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i0 = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const bool valid = i0 < a.n;
     const int i = valid ? i0 : a.n - 1;
     typedef typename V4T<T>::type V4;

@@ -94,7 +94,7 @@ template <int D, int K, int TYPE, typename T>
 __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
 {
     typedef typename V4T<T>::type V4;
-    const int i0 = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i0 = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const bool valid = i0 < a.n;
     const int i = valid ? i0 : a.n - 1;
     const V4 *__restrict__ RP = a.rv.r[R_POSH];
@@ -346,7 +346,7 @@ __global__ __launch_bounds__(256) void k_initial_density(int n, int cap, const i
                                                          const double *__restrict__ sml, double *__restrict__ dens,
                                                          Box box)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
     double xi[3] = {0, 0, 0};
     for (int k = 0; k < D; ++k) xi[k] = x[k][i];
@@ -422,7 +422,7 @@ template <int D, int K, int TYPE, typename T>
 __global__ __launch_bounds__(256) void k_force_sph(ForceArgs<T> a)
 {
     typedef typename V4T<T>::type V4;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (i >= a.n) return;
     const V4 *__restrict__ RP = a.rv.r[R_POSH];
     const V4 *__restrict__ RV = a.rv.r[R_VELM];
@@ -737,7 +737,7 @@ SPH_DEV void force_gsph_slots(const ForceArgs<T> &a, int i, int nc, int nmax, co
 template <int D, int K, typename T, int SOLVER, int LEARNED>
 __global__ __launch_bounds__(256) void k_force_gsph(ForceArgs<T> a)
 {
-    const int i0 = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i0 = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const bool valid = i0 < a.n;
     const int i = valid ? i0 : a.n - 1;
     typedef typename V4T<T>::type V4;

@@ -40,7 +40,8 @@ struct SearchArgs {
     const double *sml;        // h_j for is_ij
     const float4 *posf;       // FP32 (x, y, z, h) records for the prefilter
     const double *root;       // tree root geometry (bounds |x|)
-    const float4 *nbox;       // FP32 node boxes (centre, half edge rounded up)
+    const float4 *nbox;       // FP32 node boxes: centre, half extent x (rounded up)
+    const float2 *nhalf;      // half extents y, z (rounded up)
     const int4 *nlink;        // start, end, skip, leaf flag | kernel_size
     Box box;
     int *idx;                 // [cap][n]
@@ -64,6 +65,8 @@ template <int D> SPH_DEV double r2_exact(const double xi[3], const double xj[3],
 
 // particles per "chunk" node: a wavefront walks once per chunk it touches
 constexpr int kChunk = 128;
+// relative widening of FP32 reach radii (covers rounding of radii and extents)
+constexpr float kReachRel = 1.000003814697265625f;
 
 SPH_DEV float rdl(float v, int lane) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane)); }
 SPH_DEV int rdl(int v, int lane) { return __builtin_amdgcn_readlane(v, lane); }
@@ -81,7 +84,7 @@ template <int D, int IS_IJ>
 __global__ __launch_bounds__(256) void k_search(SearchArgs a)
 {
     const int lane = threadIdx.x & (kWave - 1);
-    const int group = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    const int group = (xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x) / kWave;
     const int i = group * kWave + lane;
     if (group * kWave >= a.n) return;
     const bool valid = i < a.n;
@@ -110,7 +113,7 @@ __global__ __launch_bounds__(256) void k_search(SearchArgs a)
     const float padX = (float)((Xb + a.root[3]) * 3.814697265625e-06);
     int count = 0;
     bool over = false;
-    long long visits = 0, tests = 0;
+    long long visits = 0, tests = 0, windows = 0, leaves = 0;
     const int p_end = min(group * kWave + kWave, a.n);
     int sub = group * kWave;
     // candidate buffer: lane q holds the q-th pending candidate index
@@ -205,21 +208,25 @@ __global__ __launch_bounds__(256) void k_search(SearchArgs a)
         // next window where it leaves this one.
         int w0 = 0;
         while (w0 < a.n_nodes) {
+            ++windows;
             const int k = w0 + lane;
             int4 lk = make_int4(0, 0, a.n_nodes, 0);
+            float4 bx = make_float4(0.f, 0.f, 0.f, 0.f);
+            float2 bh = make_float2(0.f, 0.f);
             bool reach = false;
             if (k < a.n_nodes) {
-                const float4 bx = a.nbox[k];
+                bx = a.nbox[k];
+                bh = a.nhalf[k];
                 lk = a.nlink[k];
                 float rn = Rg;
                 if (IS_IJ) rn = fmaxf(rn, fabsf(__int_as_float(lk.w)));
-                const float lim = (bx.w + rn) * 1.000003814697265625f + padX;
                 const float bc[3] = {bx.x, bx.y, bx.z};
+                const float he[3] = {bx.w, bh.x, bh.y};
                 reach = true;
 #pragma unroll
                 for (int q = 0; q < D; ++q) {
                     const float d = abs_min_image_f(bc[q] - g[q], a.box.periodic, Lf[q]) - hb[q];
-                    reach = reach && (d <= lim);
+                    reach = reach && (d <= (he[q] + rn) * kReachRel + padX);
                 }
             }
             const unsigned long long mask = __ballot(reach);
@@ -229,15 +236,31 @@ __global__ __launch_bounds__(256) void k_search(SearchArgs a)
                 const int off = node - w0;
                 ++visits;
                 if ((mask >> off) & 1ull) {
-                    if (rdl(lk.w, off) < 0) {  // entered leaf: queue its members
-                        int s = rdl(lk.x, off);
-                        const int e = rdl(lk.y, off);
-                        while (s < e) {
-                            const int take = min(e - s, kWave - nb);
-                            if (lane >= nb && lane < nb + take) cand = s + (lane - nb);
-                            nb += take;
-                            s += take;
-                            if (nb == kWave) flush(mine);
+                    if (rdl(lk.w, off) < 0) {
+                        // entered leaf: queue its members if some lane's own
+                        // sphere reaches the leaf's box
+                        const float lc[3] = {rdl(bx.x, off), rdl(bx.y, off), rdl(bx.z, off)};
+                        const float lh[3] = {rdl(bx.w, off), rdl(bh.x, off), rdl(bh.y, off)};
+                        float rl = Rf;
+                        if (IS_IJ) rl = fmaxf(rl, fabsf(__int_as_float(rdl(lk.w, off))));
+                        rl = rl * kReachRel + padX;
+                        float dist2 = 0.f;
+#pragma unroll
+                        for (int q = 0; q < D; ++q) {
+                            const float e = fmaxf(abs_min_image_f(lc[q] - xf[q], a.box.periodic, Lf[q]) - lh[q], 0.f);
+                            dist2 = (q == 0) ? e * e : dist2 + e * e;
+                        }
+                        if (__ballot(mine && dist2 <= rl * rl)) {
+                            ++leaves;
+                            int s = rdl(lk.x, off);
+                            const int e = rdl(lk.y, off);
+                            while (s < e) {
+                                const int take = min(e - s, kWave - nb);
+                                if (lane >= nb && lane < nb + take) cand = s + (lane - nb);
+                                nb += take;
+                                s += take;
+                                if (nb == kWave) flush(mine);
+                            }
                         }
                     }
                     ++node;
@@ -259,6 +282,8 @@ __global__ __launch_bounds__(256) void k_search(SearchArgs a)
         atomicAdd((unsigned long long *)&a.stats[IS_IJ ? 4 : 0], (unsigned long long)c);
         atomicAdd((unsigned long long *)&a.stats[13], (unsigned long long)visits);
         atomicAdd((unsigned long long *)&a.stats[14], (unsigned long long)tests);
+        atomicAdd((unsigned long long *)&a.stats[15], (unsigned long long)windows);
+        atomicAdd((unsigned long long *)&a.stats[16], (unsigned long long)leaves);
     }
 }
 
@@ -295,7 +320,7 @@ int search(sphgpu_ctx *c, Lists &l, int is_ij, const double *radius)
         a.sml = c->sc[S_SML];
         a.posf = c->rec32[R_POSH];
         a.root = t.root;
-        a.nbox = t.nbox; a.nlink = t.nlink;
+        a.nbox = t.nbox; a.nhalf = t.nhalf; a.nlink = t.nlink;
         a.box = c->box;
         a.idx = l.idx; a.cnt = l.cnt; a.cap = l.cap;
         a.overflow = c->dflag;

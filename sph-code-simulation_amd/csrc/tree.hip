@@ -535,17 +535,40 @@ __global__ void k_node_ks(const int *__restrict__ nstart, const int *__restrict_
 }
 
 // FP32 walk records of the neighbour search (kernel_size is added by
-// k_node_ks): centres rounded to nearest, half edges rounded up
-__global__ void k_node_records(int total, const int *__restrict__ nstart, const int *__restrict__ nend,
+// k_node_ks).  Internal nodes keep their cell (centre rounded to nearest, half
+// edge rounded up); a leaf gets the bounding box of its members, which can be
+// much smaller than its cell.  FP32 rounding of the centre is covered by the
+// search's reach padding.
+__global__ void k_node_records(int total, int dim, const int *__restrict__ nstart, const int *__restrict__ nend,
                                const int *__restrict__ nskip, const int *__restrict__ nlevel,
                                const double *__restrict__ ncenter, const double *__restrict__ nedge, int cap,
-                               float4 *__restrict__ nbox, int4 *__restrict__ nlink)
+                               const double *__restrict__ px, const double *__restrict__ py,
+                               const double *__restrict__ pz, float4 *__restrict__ nbox, float2 *__restrict__ nhalf,
+                               int4 *__restrict__ nlink)
 {
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= total) return;
-    nbox[idx] = make_float4((float)ncenter[idx], (float)ncenter[(size_t)cap + idx], (float)ncenter[2 * (size_t)cap + idx],
-                            float_up(0.5 * nedge[idx]));
-    nlink[idx] = make_int4(nstart[idx], nend[idx], nskip[idx], nlevel[idx] < 0 ? (int)0x80000000 : 0);
+    const int s = nstart[idx], e = nend[idx];
+    const bool leaf = nlevel[idx] < 0;
+    double c[3] = {ncenter[idx], ncenter[(size_t)cap + idx], ncenter[2 * (size_t)cap + idx]};
+    double h[3] = {0.5 * nedge[idx], 0.5 * nedge[idx], 0.5 * nedge[idx]};
+    if (leaf && e > s) {
+        const double *p[3] = {px, py, pz};
+        for (int k = 0; k < dim; ++k) {
+            double lo = p[k][s], hi = lo;
+            for (int j = s + 1; j < e; ++j) {
+                const double v = p[k][j];
+                lo = v < lo ? v : lo;
+                hi = v > hi ? v : hi;
+            }
+            c[k] = 0.5 * (lo + hi);
+            // half extent, and rounding of the centre in FP64
+            h[k] = 0.5 * (hi - lo) + 4.0 * 1.1102230246251565e-16 * (fabs(lo) + fabs(hi));
+        }
+    }
+    nbox[idx] = make_float4((float)c[0], (float)c[1], (float)c[2], float_up(h[0]));
+    nhalf[idx] = make_float2(float_up(h[1]), float_up(h[2]));
+    nlink[idx] = make_int4(s, e, nskip[idx], leaf ? (int)0x80000000 : 0);
 }
 
 struct SumTables {
@@ -618,7 +641,8 @@ int tree_alloc(sphgpu_ctx *c)
               al((void **)&t.bmax1, 8ull * nb1) && al((void **)&t.bmax2, 8ull * nb2) &&
               al((void **)&t.bsum1, 32ull * nb1) && al((void **)&t.bsum2, 32ull * nb2) &&
               al((void **)&t.root, 8ull * 8) && al((void **)&t.hist, 4ull * 256 * t.hist_blocks) &&
-              al((void **)&t.nbox, 16ull * t.node_cap) && al((void **)&t.nlink, 16ull * t.node_cap);
+              al((void **)&t.nbox, 16ull * t.node_cap) && al((void **)&t.nlink, 16ull * t.node_cap) &&
+              al((void **)&t.nhalf, 8ull * t.node_cap);
     const long long scan_max = std::max<long long>(n + 1, 256ll * t.hist_blocks);
     ok = ok && al((void **)&t.scan_tmp, 4ull * ((scan_max + kScanTile - 1) / kScanTile + 1));
     if (!ok) return SPHGPU_E_NOMEM;
@@ -641,7 +665,7 @@ void tree_free(sphgpu_ctx *c)
     Tree &t = c->tree;
     void *ps[] = {t.key, t.key_alt, t.perm, t.perm_alt, t.kdig, t.lam, t.leaf_end, t.ncount, t.nbase, t.start,
                   t.end, t.skip, t.level, t.center, t.edge, t.ks, t.mass, t.com, t.bmax1, t.bmax2, t.bsum1,
-                  t.bsum2, t.root, t.hist, t.nbox, t.nlink, t.scan_tmp};
+                  t.bsum2, t.root, t.hist, t.nbox, t.nlink, t.scan_tmp, t.nhalf};
     for (void *p : ps)
         if (p) hipFree(p);
     t = Tree();
@@ -746,9 +770,11 @@ int tree_build(sphgpu_ctx *c)
                        (const unsigned char *)t.kdig, (const unsigned char *)t.lam, (const int *)t.leaf_end,
                        (const int *)t.nbase, (const double *)t.root, sh, total, t.start, t.end, t.skip, t.level,
                        t.center, t.edge, t.node_cap, c->dstats);
-    hipLaunchKernelGGL(k_node_records, dim3(grid_for(total, 256)), dim3(256), 0, c->stream, total,
+    hipLaunchKernelGGL(k_node_records, dim3(grid_for(total, 256)), dim3(256), 0, c->stream, total, c->dim,
                        (const int *)t.start, (const int *)t.end, (const int *)t.skip, (const int *)t.level,
-                       (const double *)t.center, (const double *)t.edge, t.node_cap, t.nbox, t.nlink);
+                       (const double *)t.center, (const double *)t.edge, t.node_cap, (const double *)c->pos[0],
+                       (const double *)(c->dim > 1 ? c->pos[1] : c->pos[0]),
+                       (const double *)(c->dim > 2 ? c->pos[2] : c->pos[0]), t.nbox, t.nhalf, t.nlink);
     HIPCHK(c, hipGetLastError());
     // mass and centre of mass per node (needed by the gravity walk); the
     // positions are read from the permuted (key-ordered) arrays
