@@ -31,7 +31,7 @@ template <int D> constexpr double area_const() { return D == 1 ? 2.0 : D == 2 ? 
 template <typename T> struct PreArgs {
     typedef typename V4T<T>::type V4;
     int n;
-    int *idx;                     // compacted in place: members first
+    const int *idx;
     const int *cnt;
     int cap;
     RecView<T> rv;                // neighbour records (read)
@@ -151,6 +151,15 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
     const float mk = (sizeof(T) == 4) ? fp32_margin_k(a.root, D) : 0.f;
 
     // ---- density, pressure, grad-h, v_sig (members r < h, decided exactly) ----
+    // membership r < h, decided exactly (FP32 prefilter, FP64 inside its band)
+    auto member = [&](int j, T r2) -> bool {
+        if (sizeof(T) == 8) return tsqrt(r2) < h;
+        const int band = fp32_band((float)r2, (float)h, mk);
+        return band > 0 || (band == 0 && r_less(r2_64(a.x64, xi64, j, a.box, D), h64));
+    };
+    // membership bits of the first 256 candidates, for the later member
+    // passes (beyond 256 the test is repeated)
+    unsigned long long mb[4] = {0, 0, 0, 0};
     Acc<T> dens_a, dhd_a, n_a, pres_a, dhp_a, dhn_a;
     T vsig = ci * T(2.0);
     int w = 0;
@@ -158,15 +167,10 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
         const int j = a.idx[list_slot(i, q, a.cap)];
         T d[3];
         const T r2 = rec_r2<D, T>(RP[j], xi, a.box, d);
-        bool in;
-        if (sizeof(T) == 8) {
-            in = tsqrt(r2) < h;
-        } else {
-            const int band = fp32_band((float)r2, (float)h, mk);
-            in = band > 0 || (band == 0 && r_less(r2_64(a.x64, xi64, j, a.box, D), h64));
-        }
-        if (!in) continue;
-        a.idx[list_slot(i, w, a.cap)] = j;  // compact: members first
+        if (!member(j, r2)) continue;
+        const unsigned long long bit = 1ull << (q & 63);
+#pragma unroll
+        for (int b = 0; b < 4; ++b) mb[b] |= ((q >> 6) == b) ? bit : 0ull;
         ++w;
         const T r = tsqrt(r2);
         const T wk = Kern<D, K, T>::w(r, h);
@@ -230,10 +234,18 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
         // MUSCL gradients, g_pre_interaction.cpp:112-136
         if (!a.gsph2) return;
         T dd[3] = {0, 0, 0}, du[3] = {0, 0, 0}, dv[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-        for (int q = 0; q < w; ++q) {
+        for (int q = 0; q < nc; ++q) {
+            if (q < 256) {
+                unsigned long long m = 0;
+#pragma unroll
+                for (int b = 0; b < 4; ++b) m = ((q >> 6) == b) ? mb[b] : m;
+                if (!((m >> (q & 63)) & 1ull)) continue;
+            }
             const int j = a.idx[list_slot(i, q, a.cap)];
             T d[3];
-            const T r = tsqrt(rec_r2<D, T>(RP[j], xi, a.box, d));
+            const T r2 = rec_r2<D, T>(RP[j], xi, a.box, d);
+            if (q >= 256 && !member(j, r2)) continue;
+            const T r = tsqrt(r2);
             const T cw = Kern<D, K, T>::dwc(r, h);
             const V4 vj = RV[j];
             const T mj = vj.w;
@@ -280,10 +292,18 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
     const bool do_bal = a.balsara_on && D != 1;
     if (do_bal || a.td_av) {
         T div = T(0), rot[3] = {0, 0, 0};
-        for (int q = 0; q < w; ++q) {
+        for (int q = 0; q < nc; ++q) {
+            if (q < 256) {
+                unsigned long long m = 0;
+#pragma unroll
+                for (int b = 0; b < 4; ++b) m = ((q >> 6) == b) ? mb[b] : m;
+                if (!((m >> (q & 63)) & 1ull)) continue;
+            }
             const int j = a.idx[list_slot(i, q, a.cap)];
             T d[3];
-            const T r = tsqrt(rec_r2<D, T>(RP[j], xi, a.box, d));
+            const T r2 = rec_r2<D, T>(RP[j], xi, a.box, d);
+            if (q >= 256 && !member(j, r2)) continue;
+            const T r = tsqrt(r2);
             const T cw = Kern<D, K, T>::dwc(r, h);
             const V4 vj = RV[j];
             const T vjv[3] = {vj.x, vj.y, vj.z};

@@ -124,8 +124,8 @@ __global__ __launch_bounds__(256) void k_search(SearchArgs a)
         float4 pj = make_float4(0.f, 0.f, 0.f, 0.f);
         if (lane < nb) pj = a.posf[cand];
         tests += nb;
+        unsigned long long km = 0;  // candidates this lane keeps
         for (int t = 0; t < nb; ++t) {
-            const int j = rdl(cand, t);
             const float xj[3] = {rdl(pj.x, t), rdl(pj.y, t), rdl(pj.z, t)};
             float r2f = 0.f;
 #pragma unroll
@@ -146,6 +146,7 @@ __global__ __launch_bounds__(256) void k_search(SearchArgs a)
             }
             bool keep = band > 0;
             if (band == 0 && mine) {
+                const int j = rdl(cand, t);
                 double xj64[3] = {0, 0, 0};
 #pragma unroll
                 for (int k = 0; k < D; ++k) xj64[k] = a.x[k][j];
@@ -158,7 +159,17 @@ __global__ __launch_bounds__(256) void k_search(SearchArgs a)
                     keep = r2 < R * R;
                 }
             }
-            if (mine && keep) {
+            if (mine && keep) km |= 1ull << t;
+        }
+        // append the kept candidates in candidate order; one store
+        // instruction per rank instead of one per kept candidate
+        int nk = __popcll(km);
+        for (int off = kWave / 2; off > 0; off >>= 1) nk = max(nk, __shfl_xor(nk, off, kWave));
+        for (int r = 0; r < nk; ++r) {
+            const int t = km ? __ffsll((long long)km) - 1 : 0;
+            const int j = __shfl(cand, t, kWave);
+            if (km) {
+                km &= km - 1;
                 if (count < a.cap) a.idx[list_slot(i, count, a.cap)] = j;
                 else over = true;
                 ++count;
