@@ -68,6 +68,13 @@ constexpr int kChunk = 128;
 // relative widening of FP32 reach radii (covers rounding of radii and extents)
 constexpr float kReachRel = 1.000003814697265625f;
 
+// order LDS accesses between the lanes of one wavefront
+SPH_DEV void wave_lds_fence()
+{
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
 SPH_DEV float rdl(float v, int lane) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane)); }
 SPH_DEV int rdl(int v, int lane) { return __builtin_amdgcn_readlane(v, lane); }
 
@@ -96,7 +103,11 @@ __global__ __launch_bounds__(256) void k_search(SearchArgs a)
     const float4 pfi = a.posf[ii];
     const float xf[3] = {pfi.x, pfi.y, pfi.z};
     const float Rf = (float)R;
-    const float mk = fp32_margin_k(a.root, D);
+    // three times the pair kernels' band: pre-shifting a candidate to the
+    // image nearest the group centre adds up to 5 u X of rounding to each
+    // component (9 u X + u |r| in all, against the 4 u X + 2 u |r| the band
+    // is sized for with a factor 2)
+    const float mk = 3.0f * fp32_margin_k(a.root, D);
     // candidate band of this lane (for is_ij the band of H = max(h_i, h_j) >= h_i
     // is at most this wide relative to H^2)
     const float delta = mk / Rf + kMarginC;
@@ -119,55 +130,94 @@ __global__ __launch_bounds__(256) void k_search(SearchArgs a)
     // candidate buffer: lane q holds the q-th pending candidate index
     int cand = 0, nb = 0;
 
+    // per-wave LDS broadcast buffer of the pending candidates
+    __shared__ float4 s_rec[4][kWave];
+    __shared__ int s_idx[4][kWave];
+    float4 *srec = s_rec[(threadIdx.x / kWave) & 3];
+    int *sidx = s_idx[(threadIdx.x / kWave) & 3];
+    // per-piece state used by flush: group centre, and whether candidates can
+    // be pre-shifted to the minimum image seen from the group centre
+    float g[3] = {0, 0, 0};
+    bool shift_ok = true;
+
+    // the FP64 predicate (the reference's) for candidate j
+    auto exact = [&](int j) -> bool {
+        double xj64[3] = {0, 0, 0};
+#pragma unroll
+        for (int k = 0; k < D; ++k) xj64[k] = a.x[k][j];
+        const double r2 = r2_exact<D>(xi, xj64, a.box);
+        if (IS_IJ) {
+            const double hj = a.sml[j];
+            const double H = (R < hj) ? hj : R;  // std::max(h_i, h_j)
+            return r_less(r2, H) && r2 != 0.0;
+        }
+        return r2 < R * R;
+    };
+
     // test the nb pending candidates against this lane's particle
     auto flush = [&](bool mine) {
-        float4 pj = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (lane < nb) pj = a.posf[cand];
         tests += nb;
-        unsigned long long km = 0;  // candidates this lane keeps
-        for (int t = 0; t < nb; ++t) {
-            const float xj[3] = {rdl(pj.x, t), rdl(pj.y, t), rdl(pj.z, t)};
-            float r2f = 0.f;
+        // each loading lane stores its candidate (x, y, z, h) in LDS; with
+        // shift_ok the position is moved to its image nearest the group centre,
+        // which for every pair closer than the search radius is the minimum
+        // image seen from the group's particles
+        if (lane < nb) {
+            float4 pj = a.posf[cand];
+            if (a.box.periodic && shift_ok) {
+                float q[3] = {pj.x, pj.y, pj.z};
 #pragma unroll
-            for (int k = 0; k < D; ++k) {
-                const float dk = abs_min_image_f(xf[k] - xj[k], a.box.periodic, Lf[k]);
-                r2f = (k == 0) ? dk * dk : r2f + dk * dk;
-            }
-            int band = 0;
-            if (fast) {
-                float lo2 = R2lo, hi2 = R2hi;
-                if (IS_IJ) {
-                    const float Hf = fmaxf(Rf, rdl(pj.w, t));
-                    lo2 = Hf * Hf * blo;
-                    hi2 = Hf * Hf * bhi;
+                for (int k = 0; k < D; ++k) {
+                    const float d = q[k] - g[k];
+                    q[k] = g[k] + (fabsf(d) <= 0.5f * Lf[k] ? d : (d > 0.f ? d - Lf[k] : d + Lf[k]));
                 }
-                band = (r2f < lo2) ? 1 : (r2f > hi2) ? -1 : 0;
-                if (IS_IJ && r2f == 0.f) band = 0;  // r_ij != 0 needs the FP64 test
+                pj.x = q[0]; pj.y = q[1]; pj.z = q[2];
             }
-            bool keep = band > 0;
-            if (band == 0 && mine) {
-                const int j = rdl(cand, t);
-                double xj64[3] = {0, 0, 0};
-#pragma unroll
-                for (int k = 0; k < D; ++k) xj64[k] = a.x[k][j];
-                const double r2 = r2_exact<D>(xi, xj64, a.box);
-                if (IS_IJ) {
-                    const double hj = a.sml[j];
-                    const double H = (R < hj) ? hj : R;  // std::max(h_i, h_j)
-                    keep = r_less(r2, H) && r2 != 0.0;
-                } else {
-                    keep = r2 < R * R;
-                }
-            }
-            if (mine && keep) km |= 1ull << t;
+            srec[lane] = pj;
+            sidx[lane] = cand;
         }
+        wave_lds_fence();
+        unsigned km0 = 0, km1 = 0;  // candidates 0..31 and 32..63 this lane keeps
+        for (int t = 0; t < nb; ++t) {
+            const float4 c = srec[t];
+            float r2f;
+            if (!a.box.periodic || shift_ok) {
+                const float dx = xf[0] - c.x;
+                const float dy = D > 1 ? xf[1] - c.y : 0.f;
+                const float dz = D > 2 ? xf[2] - c.z : 0.f;
+                r2f = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+            } else {
+                const float cj[3] = {c.x, c.y, c.z};
+                r2f = 0.f;
+#pragma unroll
+                for (int k = 0; k < D; ++k) {
+                    const float dk = abs_min_image_f(xf[k] - cj[k], 1, Lf[k]);
+                    r2f = (k == 0) ? dk * dk : r2f + dk * dk;
+                }
+            }
+            float lo2 = R2lo, hi2 = R2hi;
+            if (IS_IJ) {
+                const float Hf = fmaxf(Rf, c.w);
+                lo2 = Hf * Hf * blo;
+                hi2 = Hf * Hf * bhi;
+            }
+            bool keep = r2f < lo2;
+            if (IS_IJ) keep = keep && r2f != 0.f;
+            const bool und = mine && (!fast || (!keep && !(r2f > hi2)));
+            if (__ballot(und) != 0ull) {
+                // inside the rounding band (or no FP32 band): FP64 decides
+                if (und) keep = exact(sidx[t]);
+            }
+            if (t < 32) km0 |= keep ? (1u << t) : 0u;
+            else km1 |= keep ? (1u << (t - 32)) : 0u;
+        }
+        unsigned long long km = mine ? (((unsigned long long)km1 << 32) | km0) : 0ull;
         // append the kept candidates in candidate order; one store
         // instruction per rank instead of one per kept candidate
         int nk = __popcll(km);
         for (int off = kWave / 2; off > 0; off >>= 1) nk = max(nk, __shfl_xor(nk, off, kWave));
         for (int r = 0; r < nk; ++r) {
             const int t = km ? __ffsll((long long)km) - 1 : 0;
-            const int j = __shfl(cand, t, kWave);
+            const int j = sidx[t];
             if (km) {
                 km &= km - 1;
                 if (count < a.cap) a.idx[list_slot(i, count, a.cap)] = j;
@@ -175,6 +225,7 @@ __global__ __launch_bounds__(256) void k_search(SearchArgs a)
                 ++count;
             }
         }
+        wave_lds_fence();
         nb = 0;
     };
 
@@ -199,7 +250,7 @@ __global__ __launch_bounds__(256) void k_search(SearchArgs a)
         const int sub_end = min(p_end, lc.y);
         const bool mine = valid && i >= sub && i < sub_end;
         // FP32 bounding box and radius of this piece
-        float g[3] = {0, 0, 0}, hb[3] = {0, 0, 0};
+        float hb[3] = {0, 0, 0};
 #pragma unroll
         for (int k = 0; k < D; ++k) {
             float lo = mine ? xf[k] : 3.0e38f, hi = mine ? xf[k] : -3.0e38f;
@@ -212,6 +263,17 @@ __global__ __launch_bounds__(256) void k_search(SearchArgs a)
         }
         float Rg = mine ? Rf : 0.f;
         for (int off = kWave / 2; off > 0; off >>= 1) Rg = fmaxf(Rg, __shfl_xor(Rg, off, kWave));
+        // pre-shifting candidates is exact for pairs closer than R when
+        // R + (group half extent) < L/2 on every axis (with room for rounding);
+        // for is_ij the radius is max(h_i, h_j), bounded by the root's
+        // kernel_size
+        {
+            float rmax = Rg;
+            if (IS_IJ) rmax = fmaxf(rmax, fabsf(__int_as_float(a.nlink[0].w)));
+            shift_ok = true;
+#pragma unroll
+            for (int k = 0; k < D; ++k) shift_ok = shift_ok && (rmax + hb[k] + 2.f * padX < 0.49f * Lf[k]);
+        }
 
         // Stackless pre-order walk evaluated 64 nodes at a time: lane q tests
         // node w0 + q; the walk then advances through the window on the
