@@ -37,6 +37,12 @@ SPH_DEV double tsqrt(double x) { return __builtin_sqrt(x); }
 SPH_DEV float texp(float x) { return __expf(x); }
 SPH_DEV double texp(double x) { return exp(x); }
 
+// Division in the pair precision: IEEE division for double (the reference's
+// expressions), a * rcp(b) for float (v_rcp_f32, 1 ulp; the FP32 form trades
+// the correctly rounded quotient for a 5x shorter sequence).
+SPH_DEV double tdiv(double a, double b) { return a / b; }
+SPH_DEV float tdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+
 // Periodic::calc_r_ij per axis (include/periodic.hpp:34-59): choose among
 // d, d + L, d - L the one of smallest magnitude, ties in the order (d, d+L, d-L).
 template <typename T> SPH_DEV T min_image(T d, T L)
@@ -105,8 +111,8 @@ template <int D, typename T> struct Kern<D, SPHGPU_CUBIC_SPLINE, T> {
     {
         const T sig = T(Sigma<D>::cubic);
         const T h_ = h * T(0.5);
-        const T q = r / h_;
-        return sig / powh<D>(h_) * (T(0.25) * pow3(T(0.5) * (T(2.0) - q + tabs(T(2.0) - q))) -
+        const T q = tdiv(r, h_);
+        return tdiv(sig, powh<D>(h_)) * (T(0.25) * pow3(T(0.5) * (T(2.0) - q + tabs(T(2.0) - q))) -
                                     pow3(T(0.5) * (T(1.0) - q + tabs(T(1.0) - q))));
     }
     static SPH_DEV T dwc(T r, T h)
@@ -114,16 +120,16 @@ template <int D, typename T> struct Kern<D, SPHGPU_CUBIC_SPLINE, T> {
         if (r == T(0)) return T(0);
         const T sig = T(Sigma<D>::cubic);
         const T h_ = h * T(0.5);
-        const T q = r / h_;
-        return -sig / (powh<D>(h_) * h_ * r) *
+        const T q = tdiv(r, h_);
+        return tdiv(-sig, powh<D>(h_) * h_ * r) *
                (T(0.75) * sqr(T(0.5) * (T(2.0) - q + tabs(T(2.0) - q))) - T(3.0) * sqr(T(0.5) * (T(1.0) - q + tabs(T(1.0) - q))));
     }
     static SPH_DEV T dhw(T r, T h)
     {
         const T sig = T(Sigma<D>::cubic);
         const T h_ = h * T(0.5);
-        const T q = r / h_;
-        return T(0.5) * sig / (powh<D>(h_) * h_) *
+        const T q = tdiv(r, h_);
+        return tdiv(T(0.5) * sig, powh<D>(h_) * h_) *
                (sqr((tabs(T(2.0) - q) + T(2.0) - q) * T(0.5)) * (T((3. + D) * 0.25) * q - T(0.5 * D)) +
                 sqr((tabs(T(1.0) - q) + T(1.0) - q) * T(0.5)) * (T(-3.0 - D) * q + T(D)));
     }
@@ -133,20 +139,20 @@ template <int D, typename T> struct Kern<D, SPHGPU_WENDLAND, T> {
     static SPH_DEV T w(T r, T h)
     {
         const T sig = T(Sigma<D>::c4);
-        const T q = r / h;
-        return sig / powh<D>(h) * pow6(T(0.5) * (T(1.0) - q + tabs(T(1.0) - q))) * (T(1.0) + T(6.0) * q + T(35.0 / 3.0) * q * q);
+        const T q = tdiv(r, h);
+        return tdiv(sig, powh<D>(h)) * pow6(T(0.5) * (T(1.0) - q + tabs(T(1.0) - q))) * (T(1.0) + T(6.0) * q + T(35.0 / 3.0) * q * q);
     }
     static SPH_DEV T dwc(T r, T h)
     {
         const T sig = T(Sigma<D>::c4);
-        const T q = r / h;
-        return T(-56.0 / 3.0) * sig / (powh<D>(h) * sqr(h)) * pow5(T(0.5) * (T(1.0) - q + tabs(T(1.0) - q))) * (T(1.0) + T(5.0) * q);
+        const T q = tdiv(r, h);
+        return tdiv(T(-56.0 / 3.0) * sig, powh<D>(h) * sqr(h)) * pow5(T(0.5) * (T(1.0) - q + tabs(T(1.0) - q))) * (T(1.0) + T(5.0) * q);
     }
     static SPH_DEV T dhw(T r, T h)
     {
         const T sig = T(Sigma<D>::c4);
-        const T q = r / h;
-        return -sig / (powh<D>(h) * h * T(3.0)) * pow5(T(0.5) * (T(1.0) - q + tabs(T(1.0) - q))) *
+        const T q = tdiv(r, h);
+        return tdiv(-sig, powh<D>(h) * h * T(3.0)) * pow5(T(0.5) * (T(1.0) - q + tabs(T(1.0) - q))) *
                (T(3.0 * D) + T(15.0 * D) * q + T(-56.0 + 17.0 * D) * q * q - T(35.0 * (8.0 + D)) * pow3(q));
     }
 };
@@ -154,22 +160,22 @@ template <int D, typename T> struct Kern<D, SPHGPU_WENDLAND, T> {
 template <int D, typename T> struct Kern<D, SPHGPU_GAUSSIAN, T> {
     static SPH_DEV T w(T r, T h)
     {
-        const T q = r / h;
-        return T(Sigma<D>::gauss) / powh<D>(h) * texp(-(q * q));
+        const T q = tdiv(r, h);
+        return tdiv(T(Sigma<D>::gauss), powh<D>(h)) * texp(-(q * q));
     }
     static SPH_DEV T dwc(T r, T h)
     {
         if (r == T(0)) return T(0);
-        const T q = r / h;
-        const T w_val = T(Sigma<D>::gauss) / powh<D>(h) * texp(-(q * q));
-        return T(-2.0) * w_val * q / (h * r);
+        const T q = tdiv(r, h);
+        const T w_val = tdiv(T(Sigma<D>::gauss), powh<D>(h)) * texp(-(q * q));
+        return tdiv(T(-2.0) * w_val * q, h * r);
     }
     static SPH_DEV T dhw(T r, T h)
     {
-        const T q = r / h;
+        const T q = tdiv(r, h);
         const T q2 = q * q;
-        const T w_val = T(Sigma<D>::gauss) / powh<D>(h) * texp(-q2);
-        return w_val * (T(2.0) * q2 - T(D)) / h;
+        const T w_val = tdiv(T(Sigma<D>::gauss), powh<D>(h)) * texp(-q2);
+        return tdiv(w_val * (T(2.0) * q2 - T(D)), h);
     }
 };
 
@@ -178,7 +184,7 @@ template <typename T> SPH_DEV T limiter(T dq1, T dq2)
 {
     const T p = dq1 * dq2;
     if (p <= T(0)) return T(0);
-    return T(2.0) * p / (dq1 + dq2);
+    return tdiv(T(2.0) * p, dq1 + dq2);
 }
 
 // HLL solver, src/gsph/g_fluid_force.cpp:173-204.  State = (u, rho, p, c).
@@ -188,7 +194,7 @@ template <typename T> SPH_DEV void hll_solver(const T l[4], const T r[4], T &pst
     const T u_r = r[0], rho_r = r[1], p_r = r[2], c_r = r[3];
     const T roe_l = tsqrt(rho_l);
     const T roe_r = tsqrt(rho_r);
-    const T roe_inv = T(1.0) / (roe_l + roe_r);
+    const T roe_inv = tdiv(T(1.0), roe_l + roe_r);
     const T u_t = (roe_l * u_l + roe_r * u_r) * roe_inv;
     const T c_t = (roe_l * c_l + roe_r * c_r) * roe_inv;
     const T a = u_l - c_l, b = u_t - c_t;
@@ -197,7 +203,7 @@ template <typename T> SPH_DEV void hll_solver(const T l[4], const T r[4], T &pst
     const T s_r = (c < d) ? d : c;      // std::max
     const T c1 = rho_l * (s_l - u_l);
     const T c2 = rho_r * (s_r - u_r);
-    const T c3 = T(1.0) / (c1 - c2);
+    const T c3 = tdiv(T(1.0), c1 - c2);
     const T c4 = p_l - u_l * c1;
     const T c5 = p_r - u_r * c2;
     vstar = (c5 - c4) * c3;
@@ -215,9 +221,9 @@ template <typename T> struct TwoShock {
         const T rho_l = l[1], rho_r = r[1];
         bad = (rho_l < T(0) || rho_r < T(0) || p_l < T(0) || p_r < T(0));
         gamma2 = T(1.0) + gamma;
-        gamma1 = T(0.5) * gamma2 / gamma;
-        V_l = T(1.0) / rho_l;
-        V_r = T(1.0) / rho_r;
+        gamma1 = tdiv(T(0.5) * gamma2, gamma);
+        V_l = tdiv(T(1.0), rho_l);
+        V_r = tdiv(T(1.0), rho_r);
         cl = tsqrt(gamma * p_l * rho_l);
         cr = tsqrt(gamma * p_r * rho_r);
     }
@@ -225,24 +231,37 @@ template <typename T> struct TwoShock {
     SPH_DEV T plain_guess() const
     {
         T g = p_r - p_l - cr * (u_r - u_l);
-        g = p_l + g * cl / (cl + cr);
+        g = p_l + tdiv(g * cl, cl + cr);
         return (g < T(1e-25)) ? T(1e-25) : g;  // std::max(g, smallp)
     }
     // one Newton step, g_fluid_force.cpp:257-279
     SPH_DEV T step(T p) const
     {
-        T w_l = T(1.0) + gamma1 * (p - p_l) / p_l;
+        T w_l = T(1.0) + tdiv(gamma1 * (p - p_l), p_l);
         w_l = cl * tsqrt(w_l);
-        T w_r = T(1.0) + gamma1 * (p - p_r) / p_r;
+        T w_r = T(1.0) + tdiv(gamma1 * (p - p_r), p_r);
         w_r = cr * tsqrt(w_r);
         T z_l = T(4.0) * V_l * w_l * w_l;
-        z_l = -z_l * w_l / (z_l - gamma2 * (p - p_l));
+        z_l = tdiv(-z_l * w_l, z_l - gamma2 * (p - p_l));
         T z_r = T(4.0) * V_r * w_r * w_r;
-        z_r = z_r * w_r / (z_r - gamma2 * (p - p_r));
-        const T ustar_l = u_l - (p - p_l) / w_l;
-        const T ustar_r = u_r + (p - p_r) / w_r;
-        T pn = p + (ustar_r - ustar_l) * (z_l * z_r) / (z_r - z_l);
+        z_r = tdiv(z_r * w_r, z_r - gamma2 * (p - p_r));
+        const T ustar_l = u_l - tdiv(p - p_l, w_l);
+        const T ustar_r = u_r + tdiv(p - p_r, w_r);
+        T pn = p + tdiv((ustar_r - ustar_l) * (z_l * z_r), z_r - z_l);
         return (T(1e-25) < pn) ? pn : T(1e-25);  // std::max(smallp, p)
+    }
+    // the Newton factor z_l z_r / (z_r - z_l) of step() at p
+    SPH_DEV T newton_factor(T p) const
+    {
+        T w_l = T(1.0) + tdiv(gamma1 * (p - p_l), p_l);
+        w_l = cl * tsqrt(w_l);
+        T w_r = T(1.0) + tdiv(gamma1 * (p - p_r), p_r);
+        w_r = cr * tsqrt(w_r);
+        T z_l = T(4.0) * V_l * w_l * w_l;
+        z_l = tdiv(-z_l * w_l, z_l - gamma2 * (p - p_l));
+        T z_r = T(4.0) * V_r * w_r * w_r;
+        z_r = tdiv(z_r * w_r, z_r - gamma2 * (p - p_r));
+        return tdiv(z_l * z_r, z_r - z_l);
     }
     // g_fluid_force.cpp:254-286; returns the iteration count
     SPH_DEV int iterate(T &p) const
@@ -252,22 +271,41 @@ template <typename T> struct TwoShock {
             const T p_old = p;
             p = step(p);
             it = iter + 1;
-            if (tabs(p - p_old) / p < T(1e-6)) break;
+            if (tdiv(tabs(p - p_old), p) < T(1e-6)) break;
         }
         return it;
     }
     // g_fluid_force.cpp:288-301
     SPH_DEV T vstar(T p) const
     {
-        T w_l = T(1.0) + gamma1 * (p - p_l) / p_l;
+        T w_l = T(1.0) + tdiv(gamma1 * (p - p_l), p_l);
         w_l = cl * tsqrt(w_l);
-        T w_r = T(1.0) + gamma1 * (p - p_r) / p_r;
+        T w_r = T(1.0) + tdiv(gamma1 * (p - p_r), p_r);
         w_r = cr * tsqrt(w_r);
-        const T ustar_l = u_l - (p - p_l) / w_l;
-        const T ustar_r = u_r + (p - p_r) / w_r;
+        const T ustar_l = u_l - tdiv(p - p_l, w_l);
+        const T ustar_r = u_r + tdiv(p - p_r, w_r);
         return T(0.5) * (ustar_l + ustar_r);
     }
 };
+
+// One FP64 Newton step of the two-shock iteration from an FP32 iterate p32:
+// the residual u*_R(p) - u*_L(p), which cancels near the root, is evaluated
+// in FP64 (w = c sqrt(1 + gamma1 (p - p0)/p0) written as
+// sqrt(gamma rho (p0 + gamma1 (p - p0)))), while the Newton factor F =
+// z_l z_r / (z_r - z_l), which only scales the small correction, comes from
+// FP32.  The step's relative error is about (FP32 rounding) x (correction).
+SPH_DEV double refine_step64(const float l[4], const float r[4], double gamma, double gamma1, float p32, float F)
+{
+    const double p = p32;
+    const double u_l = l[0], rho_l = l[1], p_l = l[2];
+    const double u_r = r[0], rho_r = r[1], p_r = r[2];
+    const double w_l = __builtin_sqrt(gamma * rho_l * (p_l + gamma1 * (p - p_l)));
+    const double w_r = __builtin_sqrt(gamma * rho_r * (p_r + gamma1 * (p - p_r)));
+    const double ustar_l = u_l - (p - p_l) / w_l;
+    const double ustar_r = u_r + (p - p_r) / w_r;
+    const double pn = p + (ustar_r - ustar_l) * (double)F;
+    return (1e-25 < pn) ? pn : 1e-25;  // std::max(smallp, p)
+}
 
 // Exact inequality of the reference's FP64 neighbour predicates.
 // sqrt(r2) < h, with the correctly rounded FP64 sqrt.

@@ -134,7 +134,7 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
             const T dens = dn.get(), ddens = ddn.get();
             const T f = dens * powh<D>(h) - b;
             const T df = ddens * powh<D>(h) + T(D) * dens * powh_m1<D>(h);
-            h -= f / df;
+            h -= tdiv(f, df);
             ++it;
             if (tabs(h - hb) < (h + hb) * T(1e-4)) { conv = true; break; }
         }
@@ -195,7 +195,7 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
             T dot = T(0);
 #pragma unroll
             for (int k = 0; k < D; ++k) dot = (k == 0) ? d[k] * (vi[k] - vjv[k]) : dot + d[k] * (vi[k] - vjv[k]);
-            const T vs = ci + RT[j].z - T(3.0) * dot / r;
+            const T vs = ci + RT[j].z - tdiv(T(3.0) * dot, r);
             if (vs > vsig) vsig = vs;
         }
     }
@@ -205,10 +205,10 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
     if (TYPE == SPHGPU_DISPH) {
         const T ni = n_a.get();
         pres = gm1 * pres_a.get();
-        gradh = h / (T(D) * ni) * dhp_a.get() / (T(1.0) + h / (T(D) * ni) * dhn_a.get());
+        gradh = tdiv(tdiv(h, T(D) * ni) * dhp_a.get(), T(1.0) + tdiv(h, T(D) * ni) * dhn_a.get());
     } else if (TYPE == SPHGPU_SSPH) {
         pres = gm1 * dens * ui;
-        gradh = T(1.0) / (T(1.0) + h / (T(D) * dens) * dhd_a.get());
+        gradh = tdiv(T(1.0), T(1.0) + tdiv(h, T(D) * dens) * dhd_a.get());
     } else {
         pres = gm1 * dens * ui;
     }
@@ -263,7 +263,7 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
                 for (int b2 = 0; b2 < D; ++b2) dv[b2][k] += dwk * mdv[b2];
             }
         }
-        const T rho_inv = T(1.0) / dens;
+        const T rho_inv = tdiv(T(1.0), dens);
         T g[16];
 #pragma unroll
         for (int q = 0; q < 16; ++q) g[q] = T(0);
@@ -337,9 +337,9 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
         }
         if (D == 3) rot_abs = tsqrt(rot[0] * rot[0] + rot[1] * rot[1] + rot[2] * rot[2]);
         else rot_abs = tabs(rot[0]);
-        const T tau_inv = T(a.eps_av) * ci / h;
+        const T tau_inv = tdiv(T(a.eps_av) * ci, h);
         if (do_bal) {
-            const T bal = tabs(div) / (tabs(div) + rot_abs + T(1e-4) * ci / h);
+            const T bal = tdiv(tabs(div), tabs(div) + rot_abs + tdiv(T(1e-4) * ci, h));
             a.balsara[i] = (double)bal;
             a.rec_w[R_AUX][4 * (size_t)i + 1] = bal;
             if (a.td_av) {
@@ -430,9 +430,9 @@ SPH_DEV T av_pi(const T vij[3], const T d[3], T r, T ai, T aj, T bi, T bj, T ci,
     if (vr < T(0)) {
         const T alpha = T(0.5) * (ai + aj);
         const T balsara = T(0.5) * (bi + bj);
-        const T w_ij = vr / r;
+        const T w_ij = tdiv(vr, r);
         const T v_sig = ci + cj - T(3.0) * w_ij;
-        const T rho_ij_inv = T(2.0) / (rhoi + rhoj);
+        const T rho_ij_inv = tdiv(T(2.0), rhoi + rhoj);
         return T(-0.5) * balsara * alpha * v_sig * w_ij * rho_ij_inv;
     }
     return T(0);
@@ -454,10 +454,10 @@ __global__ __launch_bounds__(256) void k_force_sph(ForceArgs<T> a)
     const T hi = pi_.w, rhoi = ti_.x, presi = ti_.y, gradhi = ai_.x;
     const T ci = ti_.z, ai = ai_.z, bi = ai_.y, mi = vi_.w, ui = ti_.w;
     const T gm1 = T(a.gamma - 1.0);
-    const T p_per_rho2_i = presi / sqr(rhoi);
+    const T p_per_rho2_i = tdiv(presi, sqr(rhoi));
     const T g2u_i = sqr(gm1) * ui;
-    const T g2upp_i = g2u_i / presi;
-    const T m_u_inv = T(1.0) / (mi * ui);
+    const T g2upp_i = tdiv(g2u_i, presi);
+    const T m_u_inv = tdiv(T(1.0), mi * ui);
     Acc<T> ac[3], de;
     const int nc = a.cnt[i];
     for (int q = 0; q < nc; ++q) {
@@ -486,14 +486,14 @@ __global__ __launch_bounds__(256) void k_force_sph(ForceArgs<T> a)
                 T vr = T(0);
 #pragma unroll
                 for (int k = 0; k < D; ++k) vr = (k == 0) ? vij[k] * d[k] : vr + vij[k] * d[k];
-                vsig = tabs(vr / r);
+                vsig = tabs(tdiv(vr, r));
             } else {
-                vsig = tsqrt(T(2.0) * tabs(presi - presj) / (rhoi + rhoj));
+                vsig = tsqrt(tdiv(T(2.0) * tabs(presi - presj), rhoi + rhoj));
             }
             T dr = T(0);
 #pragma unroll
             for (int k = 0; k < D; ++k) dr = (k == 0) ? dwij[k] * d[k] : dr + dwij[k] * d[k];
-            dene_ac = T(a.alpha_ac) * mj * vsig * (ui - uj) * dr / r;
+            dene_ac = tdiv(T(a.alpha_ac) * mj * vsig * (ui - uj) * dr, r);
         }
         T s1, s2, e1;
         T vdwi = T(0), vdwij = T(0);
@@ -504,12 +504,12 @@ __global__ __launch_bounds__(256) void k_force_sph(ForceArgs<T> a)
         }
         if (TYPE == SPHGPU_SSPH) {
             s1 = mj * (p_per_rho2_i * gradhi + T(0.5) * pi_ij);
-            s2 = mj * (presj / sqr(rhoj) * aj.x + T(0.5) * pi_ij);
+            s2 = mj * (tdiv(presj, sqr(rhoj)) * aj.x + T(0.5) * pi_ij);
             e1 = mj * p_per_rho2_i * gradhi * vdwi + T(0.5) * mj * pi_ij * vdwij + dene_ac;
         } else {
-            const T f_ij = T(1.0) - gradhi / (mj * uj);
+            const T f_ij = T(1.0) - tdiv(gradhi, mj * uj);
             const T f_ji = T(1.0) - aj.x * m_u_inv;
-            const T u_per_pres_j = uj / presj;
+            const T u_per_pres_j = tdiv(uj, presj);
             s1 = mj * (g2upp_i * uj * f_ij + T(0.5) * pi_ij);
             s2 = mj * (g2u_i * u_per_pres_j * f_ji + T(0.5) * pi_ij);
             e1 = mj * g2upp_i * uj * f_ij * vdwi + T(0.5) * mj * pi_ij * vdwij + dene_ac;
@@ -556,7 +556,7 @@ SPH_DEV int solve_pair(const T l[4], const T r[4], T gamma, int refine64, const 
                 const T p_old = p;
                 p = ts.step(p);
                 ++it;
-                if (tabs(p - p_old) / p < T(1e-6)) { conv = true; break; }
+                if (tdiv(tabs(p - p_old), p) < T(1e-6)) { conv = true; break; }
             }
         }
         if (!conv) {
@@ -569,13 +569,13 @@ SPH_DEV int solve_pair(const T l[4], const T r[4], T gamma, int refine64, const 
         it = ts.iterate(p);
     }
     if (sizeof(T) == 4 && refine64) {
-        TwoShock<double> td;
-        const double ld[4] = {(double)l[0], (double)l[1], (double)l[2], (double)l[3]};
-        const double rd[4] = {(double)r[0], (double)r[1], (double)r[2], (double)r[3]};
-        td.setup((double)gamma, ld, rd);
-        const double pd = td.step((double)p);
+        // final Newton step with the residual in FP64; v* from the refined p*
+        const float lf[4] = {(float)l[0], (float)l[1], (float)l[2], (float)l[3]};
+        const float rf[4] = {(float)r[0], (float)r[1], (float)r[2], (float)r[3]};
+        const double g64 = (double)gamma;
+        const double pd = refine_step64(lf, rf, g64, 0.5 * (1.0 + g64) / g64, (float)p, (float)ts.newton_factor(p));
         pstar = (T)pd;
-        vstar = (T)td.vstar(pd);
+        vstar = ts.vstar(pstar);
     } else {
         pstar = p;
         vstar = ts.vstar(p);
@@ -608,7 +608,7 @@ SPH_DEV void force_gsph_slots(const ForceArgs<T> &a, int i, int nc, int nmax, co
             j = a.idx[list_slot(i, q, a.cap)];
             const V4 pj = RP[j], vjr = RV[j], tj = RT[j];
             r = tsqrt(rec_r2<D, T>(pj, xi, a.box, d));
-            r_inv = T(1.0) / r;
+            r_inv = tdiv(T(1.0), r);
             const T vj[3] = {vjr.x, vjr.y, vjr.z};
             T ve_i = T(0), ve_j = T(0);
 #pragma unroll
@@ -721,7 +721,7 @@ SPH_DEV void force_gsph_slots(const ForceArgs<T> &a, int i, int nc, int nmax, co
             // Godunov force, g_fluid_force.cpp:158-165
             const T cwi = Kern<D, K, T>::dwc(r, hi);
             const T cwj = Kern<D, K, T>::dwc(r, hj);
-            const T rho2_inv_j = T(1.0) / sqr(rhoj);
+            const T rho2_inv_j = tdiv(T(1.0), sqr(rhoj));
             const T s1 = mj * pstar * rho2_inv_i;
             const T s2 = mj * pstar * rho2_inv_j;
             T fv = T(0);
@@ -779,7 +779,7 @@ __global__ __launch_bounds__(256) void k_force_gsph(ForceArgs<T> a)
     }
     const T hi = pi_.w, rhoi = ti_.x, presi = ti_.y, ci = ti_.z;
     const T gamma = T(a.gamma);
-    const T rho2_inv_i = T(1.0) / sqr(rhoi);
+    const T rho2_inv_i = tdiv(T(1.0), sqr(rhoi));
     const T dt = T(a.dscal[0]);
     Acc<T> ac[3], de;
     const int nc = valid ? a.cnt[i] : 0;

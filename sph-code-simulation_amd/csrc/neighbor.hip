@@ -87,6 +87,57 @@ SPH_DEV float abs_min_image_f(float d, int periodic, float L)
     return periodic ? fminf(a, fabsf(L - a)) : a;
 }
 
+// FP32 test of the broadcast candidates [t0, t1) (at most 32) against this
+// lane's particle: bit t - t0 of km is set when the candidate is a member for
+// certain, of ku when the FP32 distance falls inside the rounding band (or,
+// for is_ij, is near zero) and the FP64 predicate has to decide.  SHIFTED:
+// candidates were moved to their image nearest the group centre.
+template <int D, int IS_IJ, bool SHIFTED>
+SPH_DEV void test_range(const float4 *srec, int t0, int t1, const float xf[3], const float Lf[3], float Rf,
+                        float R2lo, float R2hi, float blo, float bhi, float z2, unsigned &km, unsigned &ku)
+{
+    unsigned kk = 0, uu = 0;
+#pragma unroll 4
+    for (int t = t0; t < t1; ++t) {
+        const float4 c = srec[t];
+        float r2f;
+        if (SHIFTED) {
+            const float dx = xf[0] - c.x;
+            const float dy = D > 1 ? xf[1] - c.y : 0.f;
+            const float dz = D > 2 ? xf[2] - c.z : 0.f;
+            r2f = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+        } else {
+            const float cj[3] = {c.x, c.y, c.z};
+            r2f = 0.f;
+#pragma unroll
+            for (int k = 0; k < D; ++k) {
+                const float dk = abs_min_image_f(xf[k] - cj[k], 1, Lf[k]);
+                r2f = (k == 0) ? dk * dk : r2f + dk * dk;
+            }
+        }
+        float lo2 = R2lo, hi2 = R2hi;
+        if (IS_IJ) {
+            const float Hf = fmaxf(Rf, c.w);
+            lo2 = Hf * Hf * blo;
+            hi2 = Hf * Hf * bhi;
+        }
+        const unsigned bit = 1u << (t - t0);
+        bool keep = r2f < lo2;
+        bool und = !keep && !(r2f > hi2);
+        if (IS_IJ) {
+            // r_ij != 0: the pre-shift need not reproduce a coincident
+            // position exactly, so near-zero FP32 distances go to FP64
+            const bool n0 = !(r2f > z2);
+            und = und || n0;
+            keep = keep && !n0;
+        }
+        kk |= keep ? bit : 0u;
+        uu |= und ? bit : 0u;
+    }
+    km = kk;
+    ku = uu;
+}
+
 template <int D, int IS_IJ>
 __global__ __launch_bounds__(256) void k_search(SearchArgs a)
 {
@@ -122,6 +173,9 @@ __global__ __launch_bounds__(256) void k_search(SearchArgs a)
     double Xb = 0.0;
     for (int k = 0; k < D; ++k) Xb = fmax(Xb, fabs(a.root[k]));
     const float padX = (float)((Xb + a.root[3]) * 3.814697265625e-06);
+    // FP32 r^2 below this may come from a coincident pair (rounding of the
+    // pre-shift is below 9 u X per component): (2^-18 X)^2 leaves room
+    const float z2 = padX * padX;
     int count = 0;
     bool over = false;
     long long visits = 0, tests = 0, windows = 0, leaves = 0;
@@ -138,7 +192,7 @@ __global__ __launch_bounds__(256) void k_search(SearchArgs a)
     // per-piece state used by flush: group centre, and whether candidates can
     // be pre-shifted to the minimum image seen from the group centre
     float g[3] = {0, 0, 0};
-    bool shift_ok = true;
+    int shift = 1;
 
     // the FP64 predicate (the reference's) for candidate j
     auto exact = [&](int j) -> bool {
@@ -158,12 +212,12 @@ __global__ __launch_bounds__(256) void k_search(SearchArgs a)
     auto flush = [&](bool mine) {
         tests += nb;
         // each loading lane stores its candidate (x, y, z, h) in LDS; with
-        // shift_ok the position is moved to its image nearest the group centre,
+        // `shift` the position is moved to its image nearest the group centre,
         // which for every pair closer than the search radius is the minimum
         // image seen from the group's particles
         if (lane < nb) {
             float4 pj = a.posf[cand];
-            if (a.box.periodic && shift_ok) {
+            if (a.box.periodic && shift) {
                 float q[3] = {pj.x, pj.y, pj.z};
 #pragma unroll
                 for (int k = 0; k < D; ++k) {
@@ -176,41 +230,30 @@ __global__ __launch_bounds__(256) void k_search(SearchArgs a)
             sidx[lane] = cand;
         }
         wave_lds_fence();
-        unsigned km0 = 0, km1 = 0;  // candidates 0..31 and 32..63 this lane keeps
-        for (int t = 0; t < nb; ++t) {
-            const float4 c = srec[t];
-            float r2f;
-            if (!a.box.periodic || shift_ok) {
-                const float dx = xf[0] - c.x;
-                const float dy = D > 1 ? xf[1] - c.y : 0.f;
-                const float dz = D > 2 ? xf[2] - c.z : 0.f;
-                r2f = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
-            } else {
-                const float cj[3] = {c.x, c.y, c.z};
-                r2f = 0.f;
-#pragma unroll
-                for (int k = 0; k < D; ++k) {
-                    const float dk = abs_min_image_f(xf[k] - cj[k], 1, Lf[k]);
-                    r2f = (k == 0) ? dk * dk : r2f + dk * dk;
-                }
-            }
-            float lo2 = R2lo, hi2 = R2hi;
-            if (IS_IJ) {
-                const float Hf = fmaxf(Rf, c.w);
-                lo2 = Hf * Hf * blo;
-                hi2 = Hf * Hf * bhi;
-            }
-            bool keep = r2f < lo2;
-            if (IS_IJ) keep = keep && r2f != 0.f;
-            const bool und = mine && (!fast || (!keep && !(r2f > hi2)));
-            if (__ballot(und) != 0ull) {
-                // inside the rounding band (or no FP32 band): FP64 decides
-                if (und) keep = exact(sidx[t]);
-            }
-            if (t < 32) km0 |= keep ? (1u << t) : 0u;
-            else km1 |= keep ? (1u << (t - 32)) : 0u;
+        unsigned km0 = 0, km1 = 0, ku0 = 0, ku1 = 0;
+        const int n0 = min(nb, 32);
+        if (!a.box.periodic || shift) {
+            test_range<D, IS_IJ, true>(srec, 0, n0, xf, Lf, Rf, R2lo, R2hi, blo, bhi, z2, km0, ku0);
+            if (nb > 32) test_range<D, IS_IJ, true>(srec, 32, nb, xf, Lf, Rf, R2lo, R2hi, blo, bhi, z2, km1, ku1);
+        } else {
+            test_range<D, IS_IJ, false>(srec, 0, n0, xf, Lf, Rf, R2lo, R2hi, blo, bhi, z2, km0, ku0);
+            if (nb > 32) test_range<D, IS_IJ, false>(srec, 32, nb, xf, Lf, Rf, R2lo, R2hi, blo, bhi, z2, km1, ku1);
         }
-        unsigned long long km = mine ? (((unsigned long long)km1 << 32) | km0) : 0ull;
+        unsigned long long km = ((unsigned long long)km1 << 32) | km0;
+        unsigned long long ku = ((unsigned long long)ku1 << 32) | ku0;
+        if (!fast) {  // no FP32 band for this lane: FP64 decides every candidate
+            km = 0;
+            ku = (nb == 64) ? ~0ull : ((1ull << nb) - 1ull);
+        }
+        if (!mine) km = ku = 0;
+        if (__ballot(ku != 0ull) != 0ull) {
+            // inside the rounding band: the reference's FP64 predicate decides
+            while (ku) {
+                const int t = __ffsll((long long)ku) - 1;
+                ku &= ku - 1;
+                if (exact(sidx[t])) km |= 1ull << t;
+            }
+        }
         // append the kept candidates in candidate order; one store
         // instruction per rank instead of one per kept candidate
         int nk = __popcll(km);
@@ -270,9 +313,10 @@ __global__ __launch_bounds__(256) void k_search(SearchArgs a)
         {
             float rmax = Rg;
             if (IS_IJ) rmax = fmaxf(rmax, fabsf(__int_as_float(a.nlink[0].w)));
-            shift_ok = true;
+            bool ok = true;
 #pragma unroll
-            for (int k = 0; k < D; ++k) shift_ok = shift_ok && (rmax + hb[k] + 2.f * padX < 0.49f * Lf[k]);
+            for (int k = 0; k < D; ++k) ok = ok && (rmax + hb[k] + 2.f * padX < 0.49f * Lf[k]);
+            shift = __builtin_amdgcn_readfirstlane(ok ? 1 : 0);  // wave-uniform by construction
         }
 
         // Stackless pre-order walk evaluated 64 nodes at a time: lane q tests
@@ -303,42 +347,52 @@ __global__ __launch_bounds__(256) void k_search(SearchArgs a)
                 }
             }
             const unsigned long long mask = __ballot(reach);
+            // events: pruned nodes and entered leaves; entered internal nodes
+            // between events are simply descended through
+            const unsigned long long evt = __ballot(k < a.n_nodes && (!reach || lk.w < 0));
             int node = w0;
             const int wend = min(w0 + kWave, a.n_nodes);
             while (node < wend) {
                 const int off = node - w0;
-                ++visits;
-                if ((mask >> off) & 1ull) {
-                    if (rdl(lk.w, off) < 0) {
-                        // entered leaf: queue its members if some lane's own
-                        // sphere reaches the leaf's box
-                        const float lc[3] = {rdl(bx.x, off), rdl(bx.y, off), rdl(bx.z, off)};
-                        const float lh[3] = {rdl(bx.w, off), rdl(bh.x, off), rdl(bh.y, off)};
-                        float rl = Rf;
-                        if (IS_IJ) rl = fmaxf(rl, fabsf(__int_as_float(rdl(lk.w, off))));
-                        rl = rl * kReachRel + padX;
-                        float dist2 = 0.f;
+                const unsigned long long m = evt >> off;
+                if (m == 0ull) {
+                    visits += wend - node;
+                    node = wend;
+                    break;
+                }
+                const int step = __ffsll((long long)m) - 1;
+                visits += step + 1;
+                const int o = off + step;
+                node = w0 + o;
+                if ((mask >> o) & 1ull) {
+                    // entered leaf: queue its members if some lane's own
+                    // sphere reaches the leaf's box
+                    const float lc[3] = {rdl(bx.x, o), rdl(bx.y, o), rdl(bx.z, o)};
+                    const float lh[3] = {rdl(bx.w, o), rdl(bh.x, o), rdl(bh.y, o)};
+                    float rl = Rf;
+                    if (IS_IJ) rl = fmaxf(rl, fabsf(__int_as_float(rdl(lk.w, o))));
+                    rl = rl * kReachRel + padX;
+                    float dist2 = 0.f;
 #pragma unroll
-                        for (int q = 0; q < D; ++q) {
-                            const float e = fmaxf(abs_min_image_f(lc[q] - xf[q], a.box.periodic, Lf[q]) - lh[q], 0.f);
-                            dist2 = (q == 0) ? e * e : dist2 + e * e;
-                        }
-                        if (__ballot(mine && dist2 <= rl * rl)) {
-                            ++leaves;
-                            int s = rdl(lk.x, off);
-                            const int e = rdl(lk.y, off);
-                            while (s < e) {
-                                const int take = min(e - s, kWave - nb);
-                                if (lane >= nb && lane < nb + take) cand = s + (lane - nb);
-                                nb += take;
-                                s += take;
-                                if (nb == kWave) flush(mine);
-                            }
+                    for (int q = 0; q < D; ++q) {
+                        const float e = fmaxf(abs_min_image_f(lc[q] - xf[q], a.box.periodic, Lf[q]) - lh[q], 0.f);
+                        dist2 = (q == 0) ? e * e : dist2 + e * e;
+                    }
+                    if (__ballot(mine && dist2 <= rl * rl)) {
+                        ++leaves;
+                        int s = rdl(lk.x, o);
+                        const int e = rdl(lk.y, o);
+                        while (s < e) {
+                            const int take = min(e - s, kWave - nb);
+                            if (lane >= nb && lane < nb + take) cand = s + (lane - nb);
+                            nb += take;
+                            s += take;
+                            if (nb == kWave) flush(mine);
                         }
                     }
                     ++node;
                 } else {
-                    node = rdl(lk.z, off);
+                    node = rdl(lk.z, o);
                 }
             }
             w0 = node;
