@@ -32,20 +32,21 @@ import learned  # noqa: E402
 import sphgpu  # noqa: E402
 
 PEAK_FP32_TFLOPS = 157.3
+PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
 
-# Algorithmic FLOP per accepted GSPH force pair (second order, iterative
-# solver), counted from the pair loop of the force kernel: reconstruction,
-# two kernel gradients, force and energy terms ~300 FLOP, plus ~44 FLOP per
-# Newton iteration of the Riemann solver (FMA = 2, div/sqrt = 1).
-FORCE_PAIR_FLOP = 300.0
-RIEMANN_ITER_FLOP = 44.0
-# PreInteraction per candidate per h-Newton iteration (~85 members, ~25
-# rejected; a mean of ~60 is used), per candidate of the final density pass
-# (~80) and per member of the MUSCL gradient pass (~80).
-PRE_NEWTON_FLOP = 60.0
-PRE_FINAL_FLOP = 80.0
-PRE_GRAD_FLOP = 80.0
+# Algorithmic FLOP model (3D, +,-,*,compare = 1, FMA = 2, div and sqrt = 1),
+# with the measured per-step counters: N_p accepted force pairs, k_R Riemann
+# Newton iterations per pair, N_c PreInteraction candidates, k_h smoothing-
+# length Newton iterations, N_m members (neighbour counts):
+#   FluidForce (GSPH 2nd order)  N_p (270 + 35 k_R)
+#   PreInteraction (GSPH)        N_c (50 k_h + 60) + N_m 60 (MUSCL gradients)
+FORCE_PAIR_FLOP = 270.0
+RIEMANN_ITER_FLOP = 35.0
+PRE_NEWTON_FLOP = 50.0
+PRE_FINAL_FLOP = 60.0
+PRE_GRAD_FLOP = 60.0
+PROFILES = REPO / "profiles"
 
 
 def make_workload(name: str, form: str):
@@ -97,31 +98,56 @@ def run_form(name: str, form: str, steps: int, warmup: int, mlp_params=None):
     stats = ctx.stats()
     e1 = ctx.energy_momentum()
     sim_t = ctx.time
+    # members per particle (the neighbour counts of the last PreInteraction)
+    members = float(ctx.download()[ics.F_NEIGHBOR].sum())
     ctx.close()
     wall = t1 - t0
     return dict(n=n, wall=wall, value=n * steps / wall, times=times, stats=stats, e0=e0, e1=e1, time=sim_t,
+                members=members,
                 drift=float((e1[3] - e_init[3]) / abs(e_init[3])))
 
 
-def roofline(res: dict, steps: int) -> dict:
+def measured_traffic(kernel: str, workload: str, form: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (profiles/traffic.json), scaled by the
+    copy-kernel calibration recorded there; None when absent."""
+    f = PROFILES / "traffic.json"
+    if not f.exists():
+        return None
+    try:
+        d = json.loads(f.read_text())
+        e = d["kernels"][kernel]
+        if d.get("workload") != workload or d.get("form") != form:
+            return None
+        return float(e["traffic_bytes"])
+    except (KeyError, ValueError, TypeError):
+        return None
+
+
+def roofline(res: dict, steps: int, workload: str, form: str) -> dict:
     st = res["stats"]
     times = res["times"]
+    n = res["n"]
     pairs = st["force_pairs"] / steps
     k_r = st["riemann_iters"] / max(1, st["riemann_calls"])
     force_flop = pairs * (FORCE_PAIR_FLOP + RIEMANN_ITER_FLOP * k_r)
     cand = st["pre_candidates"] / steps
-    k_h = st["pre_newton_iters"] / max(1, res["n"] * steps)
-    pre_flop = cand * (PRE_NEWTON_FLOP * k_h + PRE_FINAL_FLOP) + res["n"] * 55 * PRE_GRAD_FLOP
+    k_h = st["pre_newton_iters"] / max(1, n * steps)
+    members = res["members"]
+    pre_flop = cand * (PRE_NEWTON_FLOP * k_h + PRE_FINAL_FLOP) + members * PRE_GRAD_FLOP
     flops = {"fluid_force": force_flop, "pre_interaction": pre_flop}
     totals = {k: v[0] * v[1] for k, v in times.items()}
     dom = max((k for k in flops if k in times), key=lambda k: totals[k])
     avg_ms = times[dom][0]
     achieved = flops[dom] / (avg_ms * 1e-3) / 1e12
+    peak = PEAK_FP32_TFLOPS if form != "fp64" else PEAK_FP64_TFLOPS
     return {
-        "kernel": dom, "bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
-        "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
+        "kernel": dom, "bound": "mfma", "achieved": round(achieved, 3), "peak": peak,
+        "unit": "TFLOP/s", "frac": round(achieved / peak, 4),
+        "traffic": measured_traffic(dom, workload, form),
         "flop_per_launch": flops[dom], "avg_ms": avg_ms,
-        "note": "FP32 VALU pair loop against the MI355X FP32 peak (157.3 TFLOP/s)",
+        "note": ("VALU-bound pair loop (FP32 vector, non-packed) priced against the MI355X FP32 peak; "
+                 "algorithmic FLOP from the measured pair and iteration counters"),
     }
 
 
@@ -151,7 +177,7 @@ def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="sedov_3d_gsph_1m")
     ap.add_argument("--form", default="learned", choices=["fp32", "fp64", "learned"])
     ap.add_argument("--no-cpu", action="store_true")
@@ -165,7 +191,7 @@ def main() -> None:
     if args.form == "learned":
         mlp_params, train_info = train_learned()
     res = run_form(args.workload, args.form, args.steps, args.warmup, mlp_params)
-    rl = roofline(res, args.steps)
+    rl = roofline(res, args.steps, args.workload, args.form)
     out = {
         "metric": "particle-steps/s at 1M 3D GSPH: learned-FP32 vs plain, energy drift <=1e-4",
         "value": round(res["value"], 1),
@@ -210,6 +236,15 @@ def main() -> None:
                              "riemann_iters_per_call": round(r64["stats"]["riemann_iters"]
                                                              / max(1, r64["stats"]["riemann_calls"]), 4),
                              "kernel_ms": {k: round(v[0], 4) for k, v in r64["times"].items()}}
+    # plain copy of known size: attainable HBM bandwidth, and the calibration
+    # kernel (k_copy_probe) for the FETCH_SIZE / WRITE_SIZE counters
+    pc, _ = ics.sedov_3d(n1=4)
+    probe = sphgpu.Context(pc, 64)
+    nbytes = 1 << 30
+    ms = probe.copy_probe(nbytes)
+    probe.close()
+    out["hbm_copy_probe"] = {"bytes_each_way": nbytes, "ms": round(ms, 4),
+                             "GBps_read_plus_write": round(2 * nbytes / (ms * 1e-3) / 1e9, 1)}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out), flush=True)

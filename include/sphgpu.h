@@ -246,6 +246,13 @@ int sphgpu_kernel_times(sphgpu_ctx *ctx, char *name_buf, int buf_len,
                         int *n_kernels);
 int sphgpu_enable_timing(sphgpu_ctx *ctx, int on);
 
+/* Measurement hook (no reference counterpart): copies nbytes (rounded down
+ * to 16 B) between two device buffers with a plain 16-byte-per-lane copy
+ * kernel (k_copy_probe) on the context's stream and returns its duration.
+ * Its HBM counter readings calibrate the FETCH_SIZE / WRITE_SIZE counters,
+ * and nbytes / ms is the attainable copy bandwidth. */
+int sphgpu_copy_probe(sphgpu_ctx *ctx, int64_t nbytes, double *ms);
+
 #ifdef __cplusplus
 }
 #endif

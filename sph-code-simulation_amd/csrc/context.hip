@@ -75,6 +75,15 @@ template <> RecView<double> rec_view<double>(sphgpu_ctx *c)
 } // namespace sphgpu
 
 // --------------------------------------------------------------------------
+// plain copy of known size (counter calibration and attainable bandwidth)
+__global__ __launch_bounds__(256) void k_copy_probe(const float4 *__restrict__ src, float4 *__restrict__ dst,
+                                                   long long n16)
+{
+    for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n16; k += (long long)gridDim.x * blockDim.x)
+        dst[k] = src[k];
+}
+
+// --------------------------------------------------------------------------
 // host-exchange kernels: row f of the staging block is in id order
 struct FieldPtrs {
     double *p[SPHGPU_NFIELDS];
@@ -517,6 +526,34 @@ int sphgpu_kernel_times(sphgpu_ctx *c, char *name_buf, int buf_len, double *avg_
     }
     std::snprintf(name_buf, buf_len, "%s", names.c_str());
     *n_kernels = k;
+    return 0;
+}
+
+int sphgpu_copy_probe(sphgpu_ctx *c, int64_t nbytes, double *ms)
+{
+    if (!c || !ms || nbytes < 16) return SPHGPU_E_ARG;
+    const long long n16 = nbytes / 16;
+    float4 *src = nullptr, *dst = nullptr;
+    HIPCHK(c, hipMalloc((void **)&src, 16ull * n16));
+    HIPCHK(c, hipMalloc((void **)&dst, 16ull * n16));
+    HIPCHK(c, hipMemsetAsync(src, 0, 16ull * n16, c->stream));
+    hipEvent_t e0, e1;
+    HIPCHK(c, hipEventCreate(&e0));
+    HIPCHK(c, hipEventCreate(&e1));
+    const int blocks = 4096;
+    hipLaunchKernelGGL(k_copy_probe, dim3(blocks), dim3(256), 0, c->stream, (const float4 *)src, dst, n16);
+    HIPCHK(c, hipEventRecord(e0, c->stream));
+    hipLaunchKernelGGL(k_copy_probe, dim3(blocks), dim3(256), 0, c->stream, (const float4 *)src, dst, n16);
+    HIPCHK(c, hipEventRecord(e1, c->stream));
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipEventSynchronize(e1));
+    float t = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&t, e0, e1));
+    *ms = t;
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    hipFree(src);
+    hipFree(dst);
     return 0;
 }
 

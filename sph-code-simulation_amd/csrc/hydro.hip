@@ -685,7 +685,7 @@ SPH_DEV void force_gsph_slots(const ForceArgs<T> &a, int i, int nc, int nmax, co
             const float pmin = fminf(pl, pr), pmax = fmaxf(pl, pr);
             const float du = (float)(left[0] - right[0]);
             const float r_m = 0.5f * (float)(left[1] + right[1]), p_m = 0.5f * (pl + pr);
-            const bool hard = act && pmin > 0.f && r_m > 0.f &&
+            const bool hard = a.mlp.w && act && pmin > 0.f && r_m > 0.f &&
                               (pmax > 4.f * pmin || du * du * r_m > (float)gamma * p_m);
             if (__ballot(hard)) {
                 float x[6] = {0, 0, 0, 0, 0, 0}, pm = 1.f;
@@ -793,7 +793,7 @@ __global__ __launch_bounds__(256) void k_force_gsph(ForceArgs<T> a)
         // the learned guess needs the network's weights in LDS (whole block)
         __shared__ MlpWeightsLDS wl;
         __shared__ MlpWaveLDS wsc[4];
-        mlp_load_weights(wl, a.mlp.w);
+        if (a.mlp.w) mlp_load_weights(wl, a.mlp.w);
         __syncthreads();
         force_gsph_slots<D, K, T, SOLVER, true>(a, i, nc, nmax, xi, vi, gdi, gpi, gvi, hi, rhoi, presi, ci, gamma,
                                                rho2_inv_i, dt, ac, de, &wl, &wsc[(threadIdx.x >> 6) & 3], valid);
@@ -940,6 +940,7 @@ template <typename T> ForceArgs<T> force_args(sphgpu_ctx *c)
     a.refine64 = riemann_refine64(c);
     a.guess = c->prm.riemann_guess;
     a.mlp = mlp_dev(c);
+    if (c->prm.riemann_guess != SPHGPU_GUESS_LEARNED) a.mlp.w = nullptr;  // plain guess
     a.rlog = c->rlog_on ? c->rlog : nullptr;
     a.rlog_cap = c->rlog_cap;
     a.rlog_stride = c->rlog_stride;
@@ -957,10 +958,10 @@ template <int D, int K, typename T> void launch_force(sphgpu_ctx *c)
         hipLaunchKernelGGL((k_force_sph<D, K, SPHGPU_DISPH, T>), g, b, 0, c->stream, a);
     } else if (c->prm.riemann_solver == SPHGPU_RIEMANN_HLL) {
         hipLaunchKernelGGL((k_force_gsph<D, K, T, SPHGPU_RIEMANN_HLL, 0>), g, b, 0, c->stream, a);
-    } else if (c->prm.riemann_guess == SPHGPU_GUESS_LEARNED && a.mlp.w) {
-        hipLaunchKernelGGL((k_force_gsph<D, K, T, SPHGPU_RIEMANN_ITERATIVE, 1>), g, b, 0, c->stream, a);
     } else {
-        hipLaunchKernelGGL((k_force_gsph<D, K, T, SPHGPU_RIEMANN_ITERATIVE, 0>), g, b, 0, c->stream, a);
+        // one kernel for both guesses (the network is off when a.mlp.w is
+        // null); this layout measured faster than a guess-free instantiation
+        hipLaunchKernelGGL((k_force_gsph<D, K, T, SPHGPU_RIEMANN_ITERATIVE, 1>), g, b, 0, c->stream, a);
     }
 }
 

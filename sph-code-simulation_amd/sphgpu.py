@@ -119,6 +119,7 @@ def load_library() -> C.CDLL:
         "sphgpu_mlp_loss_grad": (ip, [vp, fp, C.c_int64, fp, fp]),
         "sphgpu_mlp_predict": (ip, [vp, fp, C.c_int64, fp]),
         "sphgpu_kernel_times": (ip, [vp, C.c_char_p, ip, dp, i64p, ip, C.POINTER(C.c_int)]),
+        "sphgpu_copy_probe": (ip, [vp, C.c_int64, dp]),
         "sphgpu_enable_timing": (ip, [vp, ip]),
     }
     for name, (res, args) in sigs.items():
@@ -141,7 +142,7 @@ EXPORTED = (
     "sphgpu_tree_dump", "sphgpu_riemann_batch", "sphgpu_riemann_log_enable",
     "sphgpu_riemann_log_read", "sphgpu_mlp_init", "sphgpu_mlp_set_params",
     "sphgpu_mlp_get_params", "sphgpu_mlp_param_count", "sphgpu_mlp_train",
-    "sphgpu_mlp_loss_grad", "sphgpu_mlp_predict", "sphgpu_kernel_times",
+    "sphgpu_mlp_loss_grad", "sphgpu_mlp_predict", "sphgpu_kernel_times", "sphgpu_copy_probe",
     "sphgpu_enable_timing",
 )
 
@@ -332,6 +333,12 @@ class Context:
     # ---- timing ------------------------------------------------------------
     def enable_timing(self, on: bool = True):
         self._check(self.lib.sphgpu_enable_timing(self.h, int(on)), "enable_timing")
+
+    def copy_probe(self, nbytes: int) -> float:
+        """Duration (ms) of a plain device copy of nbytes (k_copy_probe)."""
+        ms = C.c_double(0)
+        self._check(self.lib.sphgpu_copy_probe(self.h, int(nbytes), C.byref(ms)), "copy_probe")
+        return ms.value
 
     def kernel_times(self) -> dict:
         buf = C.create_string_buffer(8192)
