@@ -210,10 +210,16 @@ int sphgpu_tree_dump(sphgpu_ctx *ctx, int64_t *n_nodes, int32_t *level,
                      double *center, double *edge, double *mass, double *com,
                      double *kernel_size, int32_t *members);
 /* Batched Riemann problems, states [u, rho, p, c] x n for left and right;
- * solver = SPHGPU_RIEMANN_*; precision = SPHGPU_PAIR_*; iters may be NULL. */
+ * solver = SPHGPU_RIEMANN_*; precision = SPHGPU_PAIR_*; iters may be NULL.
+ * With the FP32 iterative solver, a context whose riemann_guess is
+ * SPHGPU_GUESS_LEARNED and that holds a network starts the pairs the GSPH
+ * force kernel would hand to the network from the learned guess;
+ * learned_used (may be NULL) then reports 0 = plain guess, 1 = learned guess
+ * accepted, 2 = learned guess rejected for the plain one. */
 int sphgpu_riemann_batch(sphgpu_ctx *ctx, int solver, int precision, int n,
                          const double *left, const double *right,
-                         double *pstar, double *vstar, int32_t *iters);
+                         double *pstar, double *vstar, int32_t *iters,
+                         int32_t *learned_used);
 
 /* ---- learned Riemann starting guess ------------------------------------- */
 /* Pair-state log written by the GSPH force kernel: rows of 7 floats

@@ -203,7 +203,8 @@ int search(sphgpu_ctx *c, Lists &l, int is_ij, const double *radius);
 int pre_interaction(sphgpu_ctx *c);
 int fluid_force(sphgpu_ctx *c);
 int riemann_batch(sphgpu_ctx *c, int solver, int precision, int n, const double *left,
-                  const double *right, double *pstar, double *vstar, int32_t *iters);
+                  const double *right, double *pstar, double *vstar, int32_t *iters,
+                  int32_t *learned_used);
 
 // gravity.hip
 int gravity(sphgpu_ctx *c);

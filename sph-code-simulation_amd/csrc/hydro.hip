@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstring>
 #include <vector>
 
 #include "ctx.hpp"
@@ -825,6 +826,44 @@ __global__ void k_riemann_batch(int n, const double *__restrict__ L, const doubl
     it[k] = iters;
 }
 
+// batched Riemann problems with the learned starting guess (FP32 form): the
+// same gate and wave-cooperative MFMA inference as the GSPH force kernel
+__global__ __launch_bounds__(256) void k_riemann_batch_learned(int n, const double *__restrict__ L,
+                                                               const double *__restrict__ R, float gamma,
+                                                               int refine64, const float *w, double *__restrict__ ps,
+                                                               double *__restrict__ vs, int *__restrict__ it,
+                                                               int *__restrict__ used)
+{
+    __shared__ MlpWeightsLDS wl;
+    __shared__ MlpWaveLDS wsc[4];
+    mlp_load_weights(wl, w);
+    __syncthreads();
+    const int k0 = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool act = k0 < n;
+    const int k = act ? k0 : n - 1;
+    float l[4], r[4];
+    for (int q = 0; q < 4; ++q) { l[q] = (float)L[4 * k + q]; r[q] = (float)R[4 * k + q]; }
+    const float pmin = fminf(l[2], r[2]), pmax = fmaxf(l[2], r[2]);
+    const float du = l[0] - r[0];
+    const float r_m = 0.5f * (l[1] + r[1]), p_m = 0.5f * (l[2] + r[2]);
+    const bool hard = act && pmin > 0.f && r_m > 0.f && (pmax > 4.f * pmin || du * du * r_m > gamma * p_m);
+    float guess = 0.f;
+    if (__ballot(hard)) {
+        float x[6] = {0, 0, 0, 0, 0, 0}, pm = 1.f;
+        if (hard) mlp_features(gamma, l[0], l[1], l[2], r[0], r[1], r[2], x, pm);
+        const float y = mlp_wave_infer(wl, wsc[(threadIdx.x >> 6) & 3], x);
+        guess = pm * __expf(y);
+    }
+    if (!act) return;
+    float p, v;
+    int fb = 0;
+    const int iters = solve_pair<float, SPHGPU_RIEMANN_ITERATIVE>(l, r, gamma, refine64, hard ? &guess : nullptr, p, v, fb);
+    ps[k] = (double)p;
+    vs[k] = (double)v;
+    it[k] = iters;
+    used[k] = hard ? (fb ? 2 : 1) : 0;
+}
+
 // ---------------------------------------------------------------------------------
 template <int D, int K, int TYPE, typename T> void launch_pre(sphgpu_ctx *c)
 {
@@ -1038,19 +1077,24 @@ int fluid_force(sphgpu_ctx *c)
 }
 
 int riemann_batch(sphgpu_ctx *c, int solver, int precision, int n, const double *left, const double *right,
-                  double *pstar, double *vstar, int32_t *iters)
+                  double *pstar, double *vstar, int32_t *iters, int32_t *learned_used)
 {
     double *d = nullptr;
     int *di = nullptr;
     const size_t b = sizeof(double) * (size_t)n;
     HIPCHK(c, hipMalloc((void **)&d, b * 12));
-    HIPCHK(c, hipMalloc((void **)&di, sizeof(int) * (size_t)n));
+    HIPCHK(c, hipMalloc((void **)&di, 2 * sizeof(int) * (size_t)n));
     double *L = d, *R = d + 4ull * n, *P = d + 8ull * n, *V = d + 9ull * n;
     HIPCHK(c, hipMemcpyAsync(L, left, b * 4, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(R, right, b * 4, hipMemcpyHostToDevice, c->stream));
     const dim3 g(grid_for(n, 256)), bl(256);
     const int ref = riemann_refine64(c);
-    if (precision == SPHGPU_PAIR_FP64) {
+    const MLPDev m = mlp_dev(c);
+    if (precision != SPHGPU_PAIR_FP64 && solver == SPHGPU_RIEMANN_ITERATIVE &&
+        c->prm.riemann_guess == SPHGPU_GUESS_LEARNED && m.w) {
+        hipLaunchKernelGGL(k_riemann_batch_learned, g, bl, 0, c->stream, n, (const double *)L, (const double *)R,
+                           (float)c->prm.gamma, ref, m.w, P, V, di, di + n);
+    } else if (precision == SPHGPU_PAIR_FP64) {
         if (solver == SPHGPU_RIEMANN_HLL)
             hipLaunchKernelGGL((k_riemann_batch<double, SPHGPU_RIEMANN_HLL>), g, bl, 0, c->stream, n, L, R, c->prm.gamma, ref, P, V, di);
         else
@@ -1065,6 +1109,13 @@ int riemann_batch(sphgpu_ctx *c, int solver, int precision, int n, const double 
     HIPCHK(c, hipMemcpyAsync(pstar, P, b, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipMemcpyAsync(vstar, V, b, hipMemcpyDeviceToHost, c->stream));
     if (iters) HIPCHK(c, hipMemcpyAsync(iters, di, sizeof(int) * (size_t)n, hipMemcpyDeviceToHost, c->stream));
+    if (learned_used) {
+        if (precision != SPHGPU_PAIR_FP64 && solver == SPHGPU_RIEMANN_ITERATIVE &&
+            c->prm.riemann_guess == SPHGPU_GUESS_LEARNED && m.w)
+            HIPCHK(c, hipMemcpyAsync(learned_used, di + n, sizeof(int) * (size_t)n, hipMemcpyDeviceToHost, c->stream));
+        else
+            std::memset(learned_used, 0, sizeof(int32_t) * (size_t)n);
+    }
     HIPCHK(c, hipStreamSynchronize(c->stream));
     hipFree(d);
     hipFree(di);

@@ -108,7 +108,7 @@ def load_library() -> C.CDLL:
         "sphgpu_reset_stats": (ip, [vp]),
         "sphgpu_neighbor_sets": (ip, [vp, dp, ip, ip, i32p, i32p]),
         "sphgpu_tree_dump": (ip, [vp, i64p, i32p, i32p, i32p, i32p, dp, dp, dp, dp, dp, i32p]),
-        "sphgpu_riemann_batch": (ip, [vp, ip, ip, ip, dp, dp, dp, dp, i32p]),
+        "sphgpu_riemann_batch": (ip, [vp, ip, ip, ip, dp, dp, dp, dp, i32p, i32p]),
         "sphgpu_riemann_log_enable": (ip, [vp, C.c_int64, ip]),
         "sphgpu_riemann_log_read": (ip, [vp, fp, C.c_int64, i64p]),
         "sphgpu_mlp_init": (ip, [vp, C.c_uint64]),
@@ -265,7 +265,8 @@ class Context:
         out["com"] = out["com"].reshape(m, 3)
         return out
 
-    def riemann_batch(self, solver: int, precision: int, left: np.ndarray, right: np.ndarray):
+    def riemann_batch(self, solver: int, precision: int, left: np.ndarray, right: np.ndarray,
+                      with_usage: bool = False):
         left = np.ascontiguousarray(left, dtype=np.float64)
         right = np.ascontiguousarray(right, dtype=np.float64)
         n = left.shape[0]
@@ -273,10 +274,12 @@ class Context:
         ps = np.zeros(n)
         vs = np.zeros(n)
         it = np.zeros(n, dtype=np.int32)
+        used = np.zeros(n, dtype=np.int32)
         self._check(self.lib.sphgpu_riemann_batch(
             self.h, int(solver), int(precision), n, _dp(left), _dp(right), _dp(ps), _dp(vs),
-            it.ctypes.data_as(C.POINTER(C.c_int32))), "riemann_batch")
-        return ps, vs, it
+            it.ctypes.data_as(C.POINTER(C.c_int32)), used.ctypes.data_as(C.POINTER(C.c_int32))),
+            "riemann_batch")
+        return (ps, vs, it, used) if with_usage else (ps, vs, it)
 
     # ---- learned Riemann guess -------------------------------------------
     def riemann_log_enable(self, capacity: int, stride: int = 1):

@@ -125,3 +125,45 @@ def test_learned_guess_gains_over_plain_guess():
     print(f"learned vs plain: acc {e_acc.max():.3e} (p99.9 {np.quantile(e_acc, 0.999):.3e}) dens {e_rho.max():.3e}")
     assert np.quantile(e_acc, 0.999) < 5e-5
     assert np.quantile(e_rho, 0.999) < 5e-5
+
+
+def _states(rows, gamma):
+    L = np.stack([rows[:, 0], rows[:, 1], rows[:, 2], np.sqrt(gamma * rows[:, 2] / rows[:, 1])], 1)
+    R = np.stack([rows[:, 3], rows[:, 4], rows[:, 5], np.sqrt(gamma * rows[:, 5] / rows[:, 4])], 1)
+    return L.astype(np.float64), R.astype(np.float64)
+
+
+def test_learned_guess_cuts_iterations_on_hard_pairs():
+    """The gain where the network is used: Sedov 24^3 pair states logged by
+    the force kernel, hard pairs only (the force kernel's gate), split into
+    training and held-out rows.  On the held-out rows the learned start needs
+    fewer Newton iterations than the plain two-shock guess, is rarely
+    rejected, and p* agrees with the plain-guess solution to the FP32 bar."""
+    import learned
+    p, f = ics.sedov_3d(n1=24)
+    gam = p["gamma"]
+    rows = learned.hard_rows(learned.log_pairs(p, f, steps=3, capacity=4_000_000), gam)
+    rows = rows[np.random.default_rng(0).permutation(rows.shape[0])]
+    n_test = rows.shape[0] // 5
+    test, train = rows[:n_test], rows[n_test:]
+    w, info = learned.train(p, train, epochs=20, batch=512, lr=2e-3)
+    L, R = _states(test, gam)
+    q = dict(p)
+    c = sphgpu.Context(q, 64)
+    ps0, vs0, it0 = c.riemann_batch(ics.RIEMANN_ITERATIVE, ics.PAIR_FP32, L, R)
+    c.close()
+    q["riemann_guess"] = ics.GUESS_LEARNED
+    c = sphgpu.Context(q, 64)
+    c.mlp_set_params(w)
+    ps1, vs1, it1, used = c.riemann_batch(ics.RIEMANN_ITERATIVE, ics.PAIR_FP32, L, R, with_usage=True)
+    c.close()
+    n_used = int((used > 0).sum())
+    fb = int((used == 2).sum())
+    print(f"held-out hard pairs {n_test} (trained on {train.shape[0]}, loss {info['loss_initial']:.3f} -> "
+          f"{info['loss_final']:.3f}): iterations plain {it0.mean():.3f} learned {it1.mean():.3f}; "
+          f"learned start used {n_used}, rejected {fb}")
+    assert n_used == n_test
+    assert it1.mean() < 0.8 * it0.mean()
+    assert fb <= 0.1 * n_used
+    scale = np.maximum(np.abs(ps0), np.maximum(L[:, 2], R[:, 2]))
+    assert (np.abs(ps1 - ps0) / scale).max() < 1e-5

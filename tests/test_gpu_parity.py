@@ -252,3 +252,32 @@ def test_riemann_batch_matches_oracle():
         assert ev < 1e-12
         assert e32 < 1e-5
         assert ev32 < 1e-5
+
+
+@pytest.mark.parametrize("prec", [ics.PAIR_FP64, ics.PAIR_FP32])
+def test_sedov_energy_drift_gate(prec):
+    """The accuracy gate of the benchmark workload at a size the oracle runs
+    in seconds: 3D GSPH Sedov 32^3, 30 steps; the total-energy drift of the
+    GPU run stays within 1e-4 of the oracle's (FP32 pairs) or 1e-9 (FP64
+    pairs), and the total momentum (zero initially) stays at the rounding
+    level of the pair precision."""
+    p, f = ics.sedov_3d(n1=32)
+    steps = 30
+    o = Oracle(p, f)
+    o.initialize()
+    e0 = o.energy()
+    o.integrate(steps)
+    eo = o.energy()
+    g = gpu_ctx(p, f, pair_precision=prec)
+    g.initialize()
+    g0 = g.energy_momentum()
+    g.integrate(steps)
+    eg = g.energy_momentum()
+    drift_o = (eo[3] - e0[3]) / abs(e0[3])
+    drift_g = (eg[3] - g0[3]) / abs(g0[3])
+    print(f"prec={prec}: t oracle {o.time:.6e} gpu {g.time:.6e}; drift oracle {drift_o:.6e} gpu {drift_g:.6e} "
+          f"|diff| {abs(drift_g - drift_o):.3e}; momentum {eg[4]:.3e} {eg[5]:.3e} {eg[6]:.3e}")
+    assert abs(drift_g - drift_o) <= (1e-9 if prec == ics.PAIR_FP64 else 1e-4)
+    # sum_i m_i |v_i| <= sqrt(2 K M) (total mass M = 1): momentum relative to it
+    pscale = np.sqrt(2.0 * eg[0] * f[ics.F_MASS].sum())
+    assert np.all(np.abs(eg[4:7]) <= (1e-12 if prec == ics.PAIR_FP64 else 1e-5) * pscale)
