@@ -71,7 +71,7 @@ def train_learned(log_workload: str = "sedov_3d_gsph_262k", log_steps: int = 4) 
     t0 = time.perf_counter()
     rows = learned.log_pairs(p, f, log_steps, capacity=12_000_000)
     t1 = time.perf_counter()
-    w, info = learned.train(p, rows, epochs=4, batch=4096, lr=2e-3)
+    w, info = learned.train(p, rows, epochs=20, batch=512, lr=2e-3, holdout=0.2)
     info.update(log_workload=log_workload, log_steps=log_steps, log_s=round(t1 - t0, 2))
     return w, info
 

@@ -13,6 +13,7 @@ import time
 
 import numpy as np
 
+import ics
 import sphgpu
 
 
@@ -32,7 +33,7 @@ def log_pairs(params: dict, fields: np.ndarray, steps: int, capacity: int, strid
     """Run `steps` steps of the given problem with the plain guess and return
     the logged pair rows."""
     p = dict(params)
-    p["riemann_guess"] = 0
+    p["riemann_guess"] = ics.GUESS_PLAIN
     ctx = sphgpu.Context(p, fields.shape[1])
     try:
         ctx.upload(fields)
@@ -44,14 +45,54 @@ def log_pairs(params: dict, fields: np.ndarray, steps: int, capacity: int, strid
         ctx.close()
 
 
+def states(rows: np.ndarray, gamma: float) -> tuple[np.ndarray, np.ndarray]:
+    """Left and right states [u, rho, p, c] of log rows."""
+    r = rows.astype(np.float64)
+    L = np.stack([r[:, 0], r[:, 1], r[:, 2], np.sqrt(gamma * r[:, 2] / r[:, 1])], 1)
+    R = np.stack([r[:, 3], r[:, 4], r[:, 5], np.sqrt(gamma * r[:, 5] / r[:, 4])], 1)
+    return L, R
+
+
+def evaluate(params: dict, rows: np.ndarray, w: np.ndarray) -> dict:
+    """Newton iterations of the FP32 iterative solver on `rows` from the plain
+    two-shock guess and from the learned guess (network w), the share of
+    learned starts rejected, and the largest p* difference on the pair's
+    pressure scale."""
+    L, R = states(rows, params["gamma"])
+    q = dict(params)
+    q["riemann_guess"] = ics.GUESS_PLAIN
+    ctx = sphgpu.Context(q, 64)
+    try:
+        ps0, _, it0 = ctx.riemann_batch(ics.RIEMANN_ITERATIVE, ics.PAIR_FP32, L, R)
+    finally:
+        ctx.close()
+    q["riemann_guess"] = ics.GUESS_LEARNED
+    ctx = sphgpu.Context(q, 64)
+    try:
+        ctx.mlp_set_params(w)
+        ps1, _, it1, used = ctx.riemann_batch(ics.RIEMANN_ITERATIVE, ics.PAIR_FP32, L, R, with_usage=True)
+    finally:
+        ctx.close()
+    scale = np.maximum(np.abs(ps0), np.maximum(L[:, 2], R[:, 2]))
+    n_used = int((used > 0).sum())
+    return {"pairs": int(rows.shape[0]), "iters_plain": round(float(it0.mean()), 4),
+            "iters_learned": round(float(it1.mean()), 4), "learned_used": n_used,
+            "learned_rejected": int((used == 2).sum()),
+            "max_pstar_diff": float((np.abs(ps1 - ps0) / scale).max())}
+
+
 def train(params: dict, rows: np.ndarray, epochs: int = 4, batch: int = 4096, lr: float = 2e-3,
-          seed: int = 7) -> tuple[np.ndarray, dict]:
+          seed: int = 7, holdout: float = 0.0) -> tuple[np.ndarray, dict]:
     """Train the network on the hard rows; returns the FP32 parameter block
-    and a summary (initial and final loss, row counts, seconds)."""
+    and a summary (initial and final loss, row counts, seconds).  With
+    holdout > 0 that share of the hard rows is kept out of training and the
+    summary gains the iteration comparison on it (`evaluate`)."""
     hard = hard_rows(rows, params["gamma"])
     if hard.shape[0] == 0:
         raise ValueError("no hard pairs in the log")
     hard = hard[np.random.default_rng(seed).permutation(hard.shape[0])]
+    n_hold = int(holdout * hard.shape[0])
+    held, hard = hard[:n_hold], hard[n_hold:]
     ctx = sphgpu.Context(params, 64)  # the network needs no particle state
     try:
         ctx.mlp_init(seed)
@@ -65,4 +106,6 @@ def train(params: dict, rows: np.ndarray, epochs: int = 4, batch: int = 4096, lr
     info = {"pairs_logged": int(rows.shape[0]), "hard_pairs_trained": int(hard.shape[0]), "epochs": epochs,
             "batch": batch, "lr": lr, "loss_initial": round(float(loss0), 5), "loss_final": round(float(loss), 5),
             "train_s": round(t1 - t0, 2)}
+    if n_hold > 0:
+        info["held_out"] = evaluate(params, held, w)
     return w, info
