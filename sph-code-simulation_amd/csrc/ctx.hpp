@@ -71,6 +71,11 @@ struct Tree {
     float4 *nbox = nullptr;
     float2 *nhalf = nullptr;
     int4 *nlink = nullptr;
+    // children of each node stored contiguously (breadth-first walks):
+    // nchild[k] = (offset into child, count); leaves have count 0
+    int2 *nchild = nullptr;
+    int *child = nullptr;
+    unsigned int *ccount = nullptr;   // child counts, scanned into offsets
     // hierarchy of block reductions over sorted particles (64 and 4096)
     double *bmax1 = nullptr, *bmax2 = nullptr;
     double *bsum1 = nullptr, *bsum2 = nullptr;  // [4][nb] : m, mx, my, mz
@@ -89,7 +94,7 @@ struct Tree {
 // inside a block.  A wavefront's slot-q loads are one 256-byte segment, and
 // the list block a wavefront writes during the search is contiguous
 // (cap * 256 bytes), which keeps its scattered appends on few pages.
-SPH_DEV __host__ inline size_t list_slot(int i, int q, int cap)
+__device__ __host__ inline size_t list_slot(int i, int q, int cap)
 {
     return ((size_t)(i >> 6) * (size_t)cap + (size_t)q) * 64 + (size_t)(i & 63);
 }

@@ -43,6 +43,8 @@ struct SearchArgs {
     const float4 *nbox;       // FP32 node boxes: centre, half extent x (rounded up)
     const float2 *nhalf;      // half extents y, z (rounded up)
     const int4 *nlink;        // start, end, skip, leaf flag | kernel_size
+    const int2 *nchild;       // (offset into child, number of children)
+    const int *child;         // children of each node, contiguous
     Box box;
     int *idx;                 // [cap][n]
     int *cnt;
@@ -65,6 +67,8 @@ template <int D> SPH_DEV double r2_exact(const double xi[3], const double xj[3],
 
 // particles per "chunk" node: a wavefront walks once per chunk it touches
 constexpr int kChunk = 128;
+// nodes a breadth-first frontier can hold (per wavefront, in LDS)
+constexpr int kFrontier = 512;
 // relative widening of FP32 reach radii (covers rounding of radii and extents)
 constexpr float kReachRel = 1.000003814697265625f;
 
@@ -189,6 +193,9 @@ __global__ __launch_bounds__(256) void k_search(SearchArgs a)
     __shared__ int s_idx[4][kWave];
     float4 *srec = s_rec[(threadIdx.x / kWave) & 3];
     int *sidx = s_idx[(threadIdx.x / kWave) & 3];
+    // per-wave breadth-first frontiers (current and next level)
+    __shared__ int s_fr[4][2][kFrontier];
+    int *sfr[2] = {s_fr[(threadIdx.x / kWave) & 3][0], s_fr[(threadIdx.x / kWave) & 3][1]};
     // per-piece state used by flush: group centre, and whether candidates can
     // be pre-shifted to the minimum image seen from the group centre
     float g[3] = {0, 0, 0};
@@ -319,83 +326,159 @@ __global__ __launch_bounds__(256) void k_search(SearchArgs a)
             shift = __builtin_amdgcn_readfirstlane(ok ? 1 : 0);  // wave-uniform by construction
         }
 
-        // Stackless pre-order walk evaluated 64 nodes at a time: lane q tests
-        // node w0 + q; the walk then advances through the window on the
-        // wave-uniform reach mask (descend = +1, prune = skip) and loads the
-        // next window where it leaves this one.
-        int w0 = 0;
-        while (w0 < a.n_nodes) {
-            ++windows;
-            const int k = w0 + lane;
-            int4 lk = make_int4(0, 0, a.n_nodes, 0);
-            float4 bx = make_float4(0.f, 0.f, 0.f, 0.f);
-            float2 bh = make_float2(0.f, 0.f);
-            bool reach = false;
-            if (k < a.n_nodes) {
-                bx = a.nbox[k];
-                bh = a.nhalf[k];
-                lk = a.nlink[k];
-                float rn = Rg;
-                if (IS_IJ) rn = fmaxf(rn, fabsf(__int_as_float(lk.w)));
-                const float bc[3] = {bx.x, bx.y, bx.z};
-                const float he[3] = {bx.w, bh.x, bh.y};
-                reach = true;
+        // reach test of node k against this piece's box (lane-parallel)
+        auto reach_node = [&](const float4 &bx, const float2 &bh, const int4 &lk) -> bool {
+            float rn = Rg;
+            if (IS_IJ) rn = fmaxf(rn, fabsf(__int_as_float(lk.w)));
+            const float bc[3] = {bx.x, bx.y, bx.z};
+            const float he[3] = {bx.w, bh.x, bh.y};
+            bool r = true;
 #pragma unroll
-                for (int q = 0; q < D; ++q) {
-                    const float d = abs_min_image_f(bc[q] - g[q], a.box.periodic, Lf[q]) - hb[q];
-                    reach = reach && (d <= (he[q] + rn) * kReachRel + padX);
+            for (int q = 0; q < D; ++q) {
+                const float d = abs_min_image_f(bc[q] - g[q], a.box.periodic, Lf[q]) - hb[q];
+                r = r && (d <= (he[q] + rn) * kReachRel + padX);
+            }
+            return r;
+        };
+        // an entered leaf held by lane o: queue its members if some lane's
+        // own sphere reaches the leaf's box
+        auto visit_leaf = [&](int o, const float4 &bx, const float2 &bh, const int4 &lk) {
+            const float lc3[3] = {rdl(bx.x, o), rdl(bx.y, o), rdl(bx.z, o)};
+            const float lh[3] = {rdl(bx.w, o), rdl(bh.x, o), rdl(bh.y, o)};
+            float rl = Rf;
+            if (IS_IJ) rl = fmaxf(rl, fabsf(__int_as_float(rdl(lk.w, o))));
+            rl = rl * kReachRel + padX;
+            float dist2 = 0.f;
+#pragma unroll
+            for (int q = 0; q < D; ++q) {
+                const float e = fmaxf(abs_min_image_f(lc3[q] - xf[q], a.box.periodic, Lf[q]) - lh[q], 0.f);
+                dist2 = (q == 0) ? e * e : dist2 + e * e;
+            }
+            if (__ballot(mine && dist2 <= rl * rl)) {
+                ++leaves;
+                int s = rdl(lk.x, o);
+                const int e = rdl(lk.y, o);
+                while (s < e) {
+                    const int take = min(e - s, kWave - nb);
+                    if (lane >= nb && lane < nb + take) cand = s + (lane - nb);
+                    nb += take;
+                    s += take;
+                    if (nb == kWave) flush(mine);
                 }
             }
-            const unsigned long long mask = __ballot(reach);
-            // events: pruned nodes and entered leaves; entered internal nodes
-            // between events are simply descended through
-            const unsigned long long evt = __ballot(k < a.n_nodes && (!reach || lk.w < 0));
-            int node = w0;
-            const int wend = min(w0 + kWave, a.n_nodes);
-            while (node < wend) {
-                const int off = node - w0;
-                const unsigned long long m = evt >> off;
-                if (m == 0ull) {
-                    visits += wend - node;
-                    node = wend;
-                    break;
+        };
+        // Depth-first walk of the pre-order node range [wb, we) (a whole
+        // subtree), evaluated 64 nodes at a time: lane q tests node w0 + q; the
+        // walk advances through the window on the wave-uniform reach mask
+        // (descend = +1, prune = skip) and loads the next window where it
+        // leaves this one.
+        auto dfs = [&](int wb, int we) {
+            int w0 = wb;
+            while (w0 < we) {
+                ++windows;
+                const int k = w0 + lane;
+                int4 lk = make_int4(0, 0, we, 0);
+                float4 bx = make_float4(0.f, 0.f, 0.f, 0.f);
+                float2 bh = make_float2(0.f, 0.f);
+                bool reach = false;
+                if (k < we) {
+                    bx = a.nbox[k];
+                    bh = a.nhalf[k];
+                    lk = a.nlink[k];
+                    reach = reach_node(bx, bh, lk);
                 }
-                const int step = __ffsll((long long)m) - 1;
-                visits += step + 1;
-                const int o = off + step;
-                node = w0 + o;
-                if ((mask >> o) & 1ull) {
-                    // entered leaf: queue its members if some lane's own
-                    // sphere reaches the leaf's box
-                    const float lc[3] = {rdl(bx.x, o), rdl(bx.y, o), rdl(bx.z, o)};
-                    const float lh[3] = {rdl(bx.w, o), rdl(bh.x, o), rdl(bh.y, o)};
-                    float rl = Rf;
-                    if (IS_IJ) rl = fmaxf(rl, fabsf(__int_as_float(rdl(lk.w, o))));
-                    rl = rl * kReachRel + padX;
-                    float dist2 = 0.f;
-#pragma unroll
-                    for (int q = 0; q < D; ++q) {
-                        const float e = fmaxf(abs_min_image_f(lc[q] - xf[q], a.box.periodic, Lf[q]) - lh[q], 0.f);
-                        dist2 = (q == 0) ? e * e : dist2 + e * e;
+                const unsigned long long mask = __ballot(reach);
+                // events: pruned nodes and entered leaves; entered internal
+                // nodes between events are simply descended through
+                const unsigned long long evt = __ballot(k < we && (!reach || lk.w < 0));
+                int node = w0;
+                const int wend = min(w0 + kWave, we);
+                while (node < wend) {
+                    const int off = node - w0;
+                    const unsigned long long m = evt >> off;
+                    if (m == 0ull) {
+                        visits += wend - node;
+                        node = wend;
+                        break;
                     }
-                    if (__ballot(mine && dist2 <= rl * rl)) {
-                        ++leaves;
-                        int s = rdl(lk.x, o);
-                        const int e = rdl(lk.y, o);
-                        while (s < e) {
-                            const int take = min(e - s, kWave - nb);
-                            if (lane >= nb && lane < nb + take) cand = s + (lane - nb);
-                            nb += take;
-                            s += take;
-                            if (nb == kWave) flush(mine);
-                        }
+                    const int step = __ffsll((long long)m) - 1;
+                    visits += step + 1;
+                    const int o = off + step;
+                    node = w0 + o;
+                    if ((mask >> o) & 1ull) {
+                        visit_leaf(o, bx, bh, lk);
+                        ++node;
+                    } else {
+                        node = rdl(lk.z, o);
                     }
-                    ++node;
+                }
+                w0 = node;
+            }
+        };
+
+        // Breadth-first walk: the frontier of nodes to test (children of the
+        // nodes entered on the previous level) is tested 64 at a time, one node
+        // per lane; entered leaves are visited, entered internal nodes append
+        // their children (stored contiguously) to the next frontier.  If the
+        // next frontier would overflow its LDS buffer, the subtrees of that
+        // batch's entered internal nodes are walked depth-first instead.
+        int *cur = sfr[0], *nxt = sfr[1];
+        if (lane == 0) cur[0] = 0;  // the root
+        int nf = 1;
+        wave_lds_fence();
+        while (nf > 0) {
+            int nn = 0;
+            for (int base = 0; base < nf; base += kWave) {
+                ++windows;
+                const int q = base + lane;
+                const bool act = q < nf;
+                const int node = act ? cur[q] : 0;
+                int4 lk = make_int4(0, 0, 0, 0);
+                float4 bx = make_float4(0.f, 0.f, 0.f, 0.f);
+                float2 bh = make_float2(0.f, 0.f);
+                bool reach = false;
+                if (act) {
+                    bx = a.nbox[node];
+                    bh = a.nhalf[node];
+                    lk = a.nlink[node];
+                    reach = reach_node(bx, bh, lk);
+                }
+                visits += min(kWave, nf - base);
+                const bool is_leaf = lk.w < 0;
+                unsigned long long lm = __ballot(act && reach && is_leaf);
+                while (lm) {
+                    const int o = __ffsll((long long)lm) - 1;
+                    lm &= lm - 1;
+                    visit_leaf(o, bx, bh, lk);
+                }
+                const bool expand = act && reach && !is_leaf;
+                int2 ch = make_int2(0, 0);
+                if (expand) ch = a.nchild[node];
+                const int cnt = expand ? ch.y : 0;
+                int incl = cnt;
+                for (int off = 1; off < kWave; off <<= 1) {
+                    const int t = __shfl_up(incl, off, kWave);
+                    if (lane >= off) incl += t;
+                }
+                const int total = __shfl(incl, kWave - 1, kWave);
+                if (nn + total <= kFrontier) {
+                    const int pos = nn + incl - cnt;
+                    for (int c = 0; c < cnt; ++c) nxt[pos + c] = a.child[ch.x + c];
+                    nn += total;
                 } else {
-                    node = rdl(lk.z, o);
+                    unsigned long long em = __ballot(expand);
+                    while (em) {
+                        const int o = __ffsll((long long)em) - 1;
+                        em &= em - 1;
+                        dfs(rdl(node, o) + 1, rdl(lk.z, o));
+                    }
                 }
             }
-            w0 = node;
+            wave_lds_fence();
+            int *t = cur;
+            cur = nxt;
+            nxt = t;
+            nf = nn;
         }
         if (nb) flush(mine);
         sub = sub_end;
@@ -447,7 +530,7 @@ int search(sphgpu_ctx *c, Lists &l, int is_ij, const double *radius)
         a.sml = c->sc[S_SML];
         a.posf = c->rec32[R_POSH];
         a.root = t.root;
-        a.nbox = t.nbox; a.nhalf = t.nhalf; a.nlink = t.nlink;
+        a.nbox = t.nbox; a.nhalf = t.nhalf; a.nlink = t.nlink; a.nchild = t.nchild; a.child = t.child;
         a.box = c->box;
         a.idx = l.idx; a.cnt = l.cnt; a.cap = l.cap;
         a.overflow = c->dflag;

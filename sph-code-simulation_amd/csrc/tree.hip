@@ -571,6 +571,32 @@ __global__ void k_node_records(int total, int dim, const int *__restrict__ nstar
     nlink[idx] = make_int4(s, e, nskip[idx], leaf ? (int)0x80000000 : 0);
 }
 
+// child lists: count the direct children of each internal node (its
+// sibling chain k+1, skip[k+1], ... below skip[k]) ...
+__global__ void k_child_count(int total, const int *__restrict__ nskip, const int *__restrict__ nlevel,
+                              unsigned int *__restrict__ ccount)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= total) return;
+    unsigned int c = 0;
+    if (nlevel[k] > 0)
+        for (int ch = k + 1; ch < nskip[k]; ch = nskip[ch]) ++c;
+    ccount[k] = c;
+}
+
+// ... and write them at the scanned offsets
+__global__ void k_child_write(int total, const int *__restrict__ nskip, const int *__restrict__ nlevel,
+                              const unsigned int *__restrict__ coff, int2 *__restrict__ nchild, int *__restrict__ child)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= total) return;
+    const int off = (int)coff[k];
+    int c = 0;
+    if (nlevel[k] > 0)
+        for (int ch = k + 1; ch < nskip[k]; ch = nskip[ch]) child[off + c++] = ch;
+    nchild[k] = make_int2(off, c);
+}
+
 struct SumTables {
     const double *m, *x, *y, *z;   // particles
     const double *b1, *b2;         // [4][nb1], [4][nb2]
@@ -642,8 +668,9 @@ int tree_alloc(sphgpu_ctx *c)
               al((void **)&t.bsum1, 32ull * nb1) && al((void **)&t.bsum2, 32ull * nb2) &&
               al((void **)&t.root, 8ull * 8) && al((void **)&t.hist, 4ull * 256 * t.hist_blocks) &&
               al((void **)&t.nbox, 16ull * t.node_cap) && al((void **)&t.nlink, 16ull * t.node_cap) &&
-              al((void **)&t.nhalf, 8ull * t.node_cap);
-    const long long scan_max = std::max<long long>(n + 1, 256ll * t.hist_blocks);
+              al((void **)&t.nhalf, 8ull * t.node_cap) && al((void **)&t.nchild, 8ull * t.node_cap) &&
+              al((void **)&t.child, 4ull * t.node_cap) && al((void **)&t.ccount, 4ull * t.node_cap);
+    const long long scan_max = std::max<long long>(std::max<long long>(n + 1, 256ll * t.hist_blocks), t.node_cap);
     ok = ok && al((void **)&t.scan_tmp, 4ull * ((scan_max + kScanTile - 1) / kScanTile + 1));
     if (!ok) return SPHGPU_E_NOMEM;
     if (c->prm.periodic) {
@@ -665,7 +692,7 @@ void tree_free(sphgpu_ctx *c)
     Tree &t = c->tree;
     void *ps[] = {t.key, t.key_alt, t.perm, t.perm_alt, t.kdig, t.lam, t.leaf_end, t.ncount, t.nbase, t.start,
                   t.end, t.skip, t.level, t.center, t.edge, t.ks, t.mass, t.com, t.bmax1, t.bmax2, t.bsum1,
-                  t.bsum2, t.root, t.hist, t.nbox, t.nlink, t.scan_tmp, t.nhalf};
+                  t.bsum2, t.root, t.hist, t.nbox, t.nlink, t.scan_tmp, t.nhalf, t.nchild, t.child, t.ccount};
     for (void *p : ps)
         if (p) hipFree(p);
     t = Tree();
@@ -775,6 +802,11 @@ int tree_build(sphgpu_ctx *c)
                        (const double *)t.center, (const double *)t.edge, t.node_cap, (const double *)c->pos[0],
                        (const double *)(c->dim > 1 ? c->pos[1] : c->pos[0]),
                        (const double *)(c->dim > 2 ? c->pos[2] : c->pos[0]), t.nbox, t.nhalf, t.nlink);
+    hipLaunchKernelGGL(k_child_count, dim3(grid_for(total, 256)), dim3(256), 0, c->stream, total,
+                       (const int *)t.skip, (const int *)t.level, t.ccount);
+    scan_counts(c->stream, (const unsigned *)t.ccount, t.ccount, total, t.scan_tmp, false);
+    hipLaunchKernelGGL(k_child_write, dim3(grid_for(total, 256)), dim3(256), 0, c->stream, total,
+                       (const int *)t.skip, (const int *)t.level, (const unsigned *)t.ccount, t.nchild, t.child);
     HIPCHK(c, hipGetLastError());
     // mass and centre of mass per node (needed by the gravity walk); the
     // positions are read from the permuted (key-ordered) arrays
