@@ -140,11 +140,13 @@ SPH_DEV int cand_class(float r2f, const float4 &c, float hhi, float R2lo, float 
 // FP32 test of the broadcast candidates [t0, t1) (at most 32) against this
 // lane's particle: bit t - t0 of km is set when the candidate is a member for
 // certain; und is set when some candidate falls inside the rounding band.
+// For is_ij, near-zero FP32 distances (the particle itself, or a coincident
+// pair) are neither kept nor flagged here but reported in kz.
 template <int D, int IS_IJ, bool SHIFTED>
 SPH_DEV void test_range(const float4 *srec, const float *shi, int t0, int t1, const float xf[3], const float Lf[3],
-                        float R2lo, float R2hi, float z2, unsigned &km, bool &und)
+                        float R2lo, float R2hi, float z2, unsigned &km, unsigned &kz, bool &und)
 {
-    unsigned kk = 0;
+    unsigned kk = 0, zz = 0;
     bool uu = false;
 #pragma unroll 4
     for (int t = t0; t < t1; ++t) {
@@ -157,12 +159,14 @@ SPH_DEV void test_range(const float4 *srec, const float *shi, int t0, int t1, co
             far = far && r2f > shi[t];
             const bool n0 = !(r2f > z2);
             keep = keep && !n0;
-            far = far && !n0;
+            far = far || n0;
+            zz |= n0 ? (1u << (t - t0)) : 0u;
         }
         kk |= keep ? (1u << (t - t0)) : 0u;
         uu = uu || !(keep || far);
     }
     km = kk;
+    kz = zz;
     und = uu;
 }
 
@@ -273,19 +277,31 @@ __global__ __launch_bounds__(256) void k_search(SearchArgs a)
             sidx[lane] = cand;
         }
         wave_lds_fence();
-        unsigned km0 = 0, km1 = 0;
+        unsigned km0 = 0, km1 = 0, kz0 = 0, kz1 = 0;
         bool u0 = false, u1 = false;
         const int n0 = min(nb, 32);
         const bool shifted = !a.box.periodic || shift;
         if (shifted) {
-            test_range<D, IS_IJ, true>(srec, shi, 0, n0, xf, Lf, R2lo, R2hi, z2, km0, u0);
-            if (nb > 32) test_range<D, IS_IJ, true>(srec, shi, 32, nb, xf, Lf, R2lo, R2hi, z2, km1, u1);
+            test_range<D, IS_IJ, true>(srec, shi, 0, n0, xf, Lf, R2lo, R2hi, z2, km0, kz0, u0);
+            if (nb > 32) test_range<D, IS_IJ, true>(srec, shi, 32, nb, xf, Lf, R2lo, R2hi, z2, km1, kz1, u1);
         } else {
-            test_range<D, IS_IJ, false>(srec, shi, 0, n0, xf, Lf, R2lo, R2hi, z2, km0, u0);
-            if (nb > 32) test_range<D, IS_IJ, false>(srec, shi, 32, nb, xf, Lf, R2lo, R2hi, z2, km1, u1);
+            test_range<D, IS_IJ, false>(srec, shi, 0, n0, xf, Lf, R2lo, R2hi, z2, km0, kz0, u0);
+            if (nb > 32) test_range<D, IS_IJ, false>(srec, shi, 32, nb, xf, Lf, R2lo, R2hi, z2, km1, kz1, u1);
         }
         unsigned long long km = ((unsigned long long)km1 << 32) | km0;
         bool und = u0 || u1;
+        if (IS_IJ) {
+            // near-zero FP32 distances: the particle itself is excluded (r_ij =
+            // 0), any other such pair is decided by the FP64 predicate
+            unsigned long long kz = ((unsigned long long)kz1 << 32) | kz0;
+            if (!mine || !fast) kz = 0;
+            while (kz) {
+                const int t = __ffsll((long long)kz) - 1;
+                kz &= kz - 1;
+                const int j = sidx[t];
+                if (j != i && exact(j)) km |= 1ull << t;
+            }
+        }
         if (!fast) {  // no FP32 band for this lane: FP64 decides every candidate
             km = 0;
             und = true;
@@ -299,7 +315,8 @@ __global__ __launch_bounds__(256) void k_search(SearchArgs a)
             for (int t = 0; t < nb; ++t) {
                 const float4 c = srec[t];
                 const float r2f = shifted ? cand_r2<D, true>(c, xf, Lf) : cand_r2<D, false>(c, xf, Lf);
-                const int cl = fast ? cand_class<IS_IJ>(r2f, c, IS_IJ ? shi[t] : 0.f, R2lo, R2hi, z2) : 0;
+                int cl = fast ? cand_class<IS_IJ>(r2f, c, IS_IJ ? shi[t] : 0.f, R2lo, R2hi, z2) : 0;
+                if (IS_IJ && fast && !(r2f > z2)) cl = -1;  // settled above
                 if (cl == 0 && exact(sidx[t])) km |= 1ull << t;
             }
         }
