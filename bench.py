@@ -223,6 +223,7 @@ def main() -> None:
         "search_member_tests_per_wave": round(st["search_member_tests"] * 64 / (2 * res["n"] * args.steps), 1),
         "search_windows_per_wave": round(st["search_windows"] * 64 / (2 * res["n"] * args.steps), 1),
         "search_leaves_per_wave": round(st["search_leaves"] * 64 / (2 * res["n"] * args.steps), 1),
+        "search_fp64_decisions_per_particle": round(st["search_exact"] / (res["n"] * args.steps), 4),
     }
     if train_info is not None:
         out["training"] = train_info

@@ -151,6 +151,7 @@ typedef struct sphgpu_stats {
     int32_t pad;
     int64_t search_windows;      /* 64-node windows loaded by search wavefronts */
     int64_t search_leaves;       /* leaves whose members a wavefront tested   */
+    int64_t search_exact;        /* candidates decided by the FP64 predicate  */
 } sphgpu_stats;
 
 /* ---- lifecycle ------------------------------------------------------------ */

@@ -443,6 +443,7 @@ int sphgpu_get_stats(sphgpu_ctx *c, sphgpu_stats *out)
     out->search_member_tests = h[14];
     out->search_windows = h[15];
     out->search_leaves = h[16];
+    out->search_exact = h[17];
     return 0;
 }
 

@@ -96,82 +96,54 @@ SPH_DEV float abs_min_image_f(float d, int periodic, float L)
 // certain, of ku when the FP32 distance falls inside the rounding band (or,
 // for is_ij, is near zero) and the FP64 predicate has to decide.  SHIFTED:
 // candidates were moved to their image nearest the group centre.
-// FP32 distance of broadcast candidate c from this lane's particle
-template <int D, bool SHIFTED>
-SPH_DEV float cand_r2(const float4 &c, const float xf[3], const float Lf[3])
-{
-    if (SHIFTED) {
-        const float dx = xf[0] - c.x;
-        const float dy = D > 1 ? xf[1] - c.y : 0.f;
-        const float dz = D > 2 ? xf[2] - c.z : 0.f;
-        return __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
-    }
-    const float cj[3] = {c.x, c.y, c.z};
-    float r2f = 0.f;
-#pragma unroll
-    for (int k = 0; k < D; ++k) {
-        const float dk = abs_min_image_f(xf[k] - cj[k], 1, Lf[k]);
-        r2f = (k == 0) ? dk * dk : r2f + dk * dk;
-    }
-    return r2f;
-}
-
-// FP32 classification of a candidate at FP32 distance r2f: +1 member for
-// certain, -1 outside for certain, 0 inside the rounding band (FP64 decides).
-// For is_ij the candidate record carries h_j^2 (1 - delta_w) in c.w and
-// h_j^2 (1 + delta_w) in hhi, with delta_w the wave's widest band.
-template <int IS_IJ>
-SPH_DEV int cand_class(float r2f, const float4 &c, float hhi, float R2lo, float R2hi, float z2)
-{
-    bool keep = r2f < R2lo;
-    bool far = r2f > R2hi;
-    if (IS_IJ) {
-        keep = keep || r2f < c.w;
-        far = far && r2f > hhi;
-        // r_ij != 0: the pre-shift need not reproduce a coincident position
-        // exactly, so near-zero FP32 distances go to FP64
-        const bool n0 = !(r2f > z2);
-        keep = keep && !n0;
-        far = far && !n0;
-    }
-    return keep ? 1 : (far ? -1 : 0);
-}
-
-// FP32 test of the broadcast candidates [t0, t1) (at most 32) against this
-// lane's particle: bit t - t0 of km is set when the candidate is a member for
-// certain; und is set when some candidate falls inside the rounding band.
-// For is_ij, near-zero FP32 distances (the particle itself, or a coincident
-// pair) are neither kept nor flagged here but reported in kz.
 template <int D, int IS_IJ, bool SHIFTED>
-SPH_DEV void test_range(const float4 *srec, const float *shi, int t0, int t1, const float xf[3], const float Lf[3],
-                        float R2lo, float R2hi, float z2, unsigned &km, unsigned &kz, bool &und)
+SPH_DEV void test_range(const float4 *srec, int t0, int t1, const float xf[3], const float Lf[3], float Rf,
+                        float R2lo, float R2hi, float blo, float bhi, float z2, unsigned &km, unsigned &ku)
 {
-    unsigned kk = 0, zz = 0;
-    bool uu = false;
+    unsigned kk = 0, uu = 0;
 #pragma unroll 4
     for (int t = t0; t < t1; ++t) {
         const float4 c = srec[t];
-        const float r2f = cand_r2<D, SHIFTED>(c, xf, Lf);
-        bool keep = r2f < R2lo;
-        bool far = r2f > R2hi;
-        if (IS_IJ) {
-            keep = keep || r2f < c.w;
-            far = far && r2f > shi[t];
-            const bool n0 = !(r2f > z2);
-            keep = keep && !n0;
-            far = far || n0;
-            zz |= n0 ? (1u << (t - t0)) : 0u;
+        float r2f;
+        if (SHIFTED) {
+            const float dx = xf[0] - c.x;
+            const float dy = D > 1 ? xf[1] - c.y : 0.f;
+            const float dz = D > 2 ? xf[2] - c.z : 0.f;
+            r2f = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+        } else {
+            const float cj[3] = {c.x, c.y, c.z};
+            r2f = 0.f;
+#pragma unroll
+            for (int k = 0; k < D; ++k) {
+                const float dk = abs_min_image_f(xf[k] - cj[k], 1, Lf[k]);
+                r2f = (k == 0) ? dk * dk : r2f + dk * dk;
+            }
         }
-        kk |= keep ? (1u << (t - t0)) : 0u;
-        uu = uu || !(keep || far);
+        float lo2 = R2lo, hi2 = R2hi;
+        if (IS_IJ) {
+            const float Hf = fmaxf(Rf, c.w);
+            lo2 = Hf * Hf * blo;
+            hi2 = Hf * Hf * bhi;
+        }
+        const unsigned bit = 1u << (t - t0);
+        bool keep = r2f < lo2;
+        bool und = !keep && !(r2f > hi2);
+        if (IS_IJ) {
+            // r_ij != 0: the pre-shift need not reproduce a coincident
+            // position exactly, so near-zero FP32 distances go to FP64
+            const bool n0 = !(r2f > z2);
+            und = und || n0;
+            keep = keep && !n0;
+        }
+        kk |= keep ? bit : 0u;
+        uu |= und ? bit : 0u;
     }
     km = kk;
-    kz = zz;
-    und = uu;
+    ku = uu;
 }
 
 template <int D, int IS_IJ>
-__global__ __launch_bounds__(256) void k_search(SearchArgs a)
+__global__ __launch_bounds__(256, 4) void k_search(SearchArgs a)
 {
     const int lane = threadIdx.x & (kWave - 1);
     const int group = (xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x) / kWave;
@@ -197,11 +169,6 @@ __global__ __launch_bounds__(256) void k_search(SearchArgs a)
     const bool fast = (Rf * Rf > 1e-30f) && (delta < 0.5f);
     const float blo = 1.0f - delta, bhi = 1.0f + delta;
     const float R2lo = Rf * Rf * blo, R2hi = Rf * Rf * bhi;
-    // the widest band over the wave's lanes that have one (for h_j tests
-    // shared by all lanes: h_j > h_i has a narrower band than h_i's)
-    float dw = fast ? delta : 0.f;
-    for (int off = kWave / 2; off > 0; off >>= 1) dw = fmaxf(dw, __shfl_xor(dw, off, kWave));
-    const float blo_w = 1.0f - dw, bhi_w = 1.0f + dw;
     float Lf[3] = {1.f, 1.f, 1.f};
 #pragma unroll
     for (int k = 0; k < D; ++k) Lf[k] = a.box.rangef[k];
@@ -224,10 +191,8 @@ __global__ __launch_bounds__(256) void k_search(SearchArgs a)
     // per-wave LDS broadcast buffer of the pending candidates
     __shared__ float4 s_rec[4][kWave];
     __shared__ int s_idx[4][kWave];
-    __shared__ float s_hhi[4][kWave];
     float4 *srec = s_rec[(threadIdx.x / kWave) & 3];
     int *sidx = s_idx[(threadIdx.x / kWave) & 3];
-    float *shi = s_hhi[(threadIdx.x / kWave) & 3];
     // per-wave breadth-first frontiers (current and next level)
     __shared__ int s_fr[4][2][kFrontier];
     int *sfr[2] = {s_fr[(threadIdx.x / kWave) & 3][0], s_fr[(threadIdx.x / kWave) & 3][1]};
@@ -237,7 +202,9 @@ __global__ __launch_bounds__(256) void k_search(SearchArgs a)
     int shift = 1;
 
     // the FP64 predicate (the reference's) for candidate j
+    long long nexact = 0;
     auto exact = [&](int j) -> bool {
+        ++nexact;
         double xj64[3] = {0, 0, 0};
 #pragma unroll
         for (int k = 0; k < D; ++k) xj64[k] = a.x[k][j];
@@ -268,56 +235,32 @@ __global__ __launch_bounds__(256) void k_search(SearchArgs a)
                 }
                 pj.x = q[0]; pj.y = q[1]; pj.z = q[2];
             }
-            if (IS_IJ) {
-                const float h2 = pj.w * pj.w;
-                shi[lane] = h2 * bhi_w;
-                pj.w = h2 * blo_w;
-            }
             srec[lane] = pj;
             sidx[lane] = cand;
         }
         wave_lds_fence();
-        unsigned km0 = 0, km1 = 0, kz0 = 0, kz1 = 0;
-        bool u0 = false, u1 = false;
+        unsigned km0 = 0, km1 = 0, ku0 = 0, ku1 = 0;
         const int n0 = min(nb, 32);
-        const bool shifted = !a.box.periodic || shift;
-        if (shifted) {
-            test_range<D, IS_IJ, true>(srec, shi, 0, n0, xf, Lf, R2lo, R2hi, z2, km0, kz0, u0);
-            if (nb > 32) test_range<D, IS_IJ, true>(srec, shi, 32, nb, xf, Lf, R2lo, R2hi, z2, km1, kz1, u1);
+        if (!a.box.periodic || shift) {
+            test_range<D, IS_IJ, true>(srec, 0, n0, xf, Lf, Rf, R2lo, R2hi, blo, bhi, z2, km0, ku0);
+            if (nb > 32) test_range<D, IS_IJ, true>(srec, 32, nb, xf, Lf, Rf, R2lo, R2hi, blo, bhi, z2, km1, ku1);
         } else {
-            test_range<D, IS_IJ, false>(srec, shi, 0, n0, xf, Lf, R2lo, R2hi, z2, km0, kz0, u0);
-            if (nb > 32) test_range<D, IS_IJ, false>(srec, shi, 32, nb, xf, Lf, R2lo, R2hi, z2, km1, kz1, u1);
+            test_range<D, IS_IJ, false>(srec, 0, n0, xf, Lf, Rf, R2lo, R2hi, blo, bhi, z2, km0, ku0);
+            if (nb > 32) test_range<D, IS_IJ, false>(srec, 32, nb, xf, Lf, Rf, R2lo, R2hi, blo, bhi, z2, km1, ku1);
         }
         unsigned long long km = ((unsigned long long)km1 << 32) | km0;
-        bool und = u0 || u1;
-        if (IS_IJ) {
-            // near-zero FP32 distances: the particle itself is excluded (r_ij =
-            // 0), any other such pair is decided by the FP64 predicate
-            unsigned long long kz = ((unsigned long long)kz1 << 32) | kz0;
-            if (!mine || !fast) kz = 0;
-            while (kz) {
-                const int t = __ffsll((long long)kz) - 1;
-                kz &= kz - 1;
-                const int j = sidx[t];
-                if (j != i && exact(j)) km |= 1ull << t;
-            }
-        }
+        unsigned long long ku = ((unsigned long long)ku1 << 32) | ku0;
         if (!fast) {  // no FP32 band for this lane: FP64 decides every candidate
             km = 0;
-            und = true;
+            ku = (nb == 64) ? ~0ull : ((1ull << nb) - 1ull);
         }
-        if (!mine) {
-            km = 0;
-            und = false;
-        }
-        if (__ballot(und) != 0ull && und) {
+        if (!mine) km = ku = 0;
+        if (__ballot(ku != 0ull) != 0ull) {
             // inside the rounding band: the reference's FP64 predicate decides
-            for (int t = 0; t < nb; ++t) {
-                const float4 c = srec[t];
-                const float r2f = shifted ? cand_r2<D, true>(c, xf, Lf) : cand_r2<D, false>(c, xf, Lf);
-                int cl = fast ? cand_class<IS_IJ>(r2f, c, IS_IJ ? shi[t] : 0.f, R2lo, R2hi, z2) : 0;
-                if (IS_IJ && fast && !(r2f > z2)) cl = -1;  // settled above
-                if (cl == 0 && exact(sidx[t])) km |= 1ull << t;
+            while (ku) {
+                const int t = __ffsll((long long)ku) - 1;
+                ku &= ku - 1;
+                if (exact(sidx[t])) km |= 1ull << t;
             }
         }
         // append the kept candidates in candidate order; one store
@@ -547,7 +490,9 @@ __global__ __launch_bounds__(256) void k_search(SearchArgs a)
     // candidate statistics
     long long c = valid ? count : 0;
     for (int off = kWave / 2; off > 0; off >>= 1) c += __shfl_xor(c, off, kWave);
+    for (int off = kWave / 2; off > 0; off >>= 1) nexact += __shfl_xor(nexact, off, kWave);
     if (lane == 0) {
+        atomicAdd((unsigned long long *)&a.stats[17], (unsigned long long)nexact);
         atomicAdd((unsigned long long *)&a.stats[IS_IJ ? 4 : 0], (unsigned long long)c);
         atomicAdd((unsigned long long *)&a.stats[13], (unsigned long long)visits);
         atomicAdd((unsigned long long *)&a.stats[14], (unsigned long long)tests);
