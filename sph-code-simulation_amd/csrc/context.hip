@@ -223,6 +223,7 @@ int sphgpu_create(const sphgpu_params *p, int n, sphgpu_ctx **out)
         c->box.hi[k] = p->range_max[k];
         c->box.range[k] = p->range_max[k] - p->range_min[k];
         c->box.rangef[k] = (float)c->box.range[k];
+        c->box.rinvf[k] = c->box.range[k] > 0.0 ? (float)(1.0 / c->box.range[k]) : 0.f;
     }
     auto alloc = [&](void **ptr, size_t bytes) -> bool {
         if (hipMalloc(ptr, bytes) != hipSuccess) return false;

@@ -62,17 +62,28 @@ struct Box {
     double lo[3];
     double hi[3];
     float rangef[3];
+    float rinvf[3];   // 1 / rangef
 };
 
 template <typename T> SPH_DEV T box_range(const Box &b, int k);
 template <> SPH_DEV double box_range<double>(const Box &b, int k) { return b.range[k]; }
 template <> SPH_DEV float box_range<float>(const Box &b, int k) { return b.rangef[k]; }
 
-// r_i - r_j with the reference's minimum-image choice.
+// r_i - r_j with the reference's minimum-image choice (FP64).
 template <typename T> SPH_DEV T rij_axis(T xi, T xj, const Box &b, int k)
 {
     const T d = xi - xj;
     return b.periodic ? min_image(d, box_range<T>(b, k)) : d;
+}
+
+// FP32: d - L rint(d / L), three instructions.  It differs from the
+// reference's choice only at |d| = L/2 (the tie order decides the sign there),
+// and r^2 not at all; membership tests near such ties go through the FP64
+// predicate anyway.
+template <> SPH_DEV float rij_axis<float>(float xi, float xj, const Box &b, int k)
+{
+    const float d = xi - xj;
+    return b.periodic ? __builtin_fmaf(-b.rangef[k], __builtin_rintf(d * b.rinvf[k]), d) : d;
 }
 
 // include/defines.hpp:28-32
@@ -312,14 +323,16 @@ SPH_DEV double refine_step64(const float l[4], const float r[4], double gamma, d
 SPH_DEV bool r_less(double r2, double h) { return __builtin_sqrt(r2) < h; }
 
 // FP32 prefilter for the FP64 membership tests r^2 < h^2.  With positions
-// bounded by X (|x_k| <= X) and u = 2^-24, rounding the positions and the
-// periodic shift to FP32 moves each r_ij component by at most 4uX + 2u|r_ij|,
-// and the FP32 sum of squares adds (D+1)u r^2, so near r = h
-//   |r2_fp32 - r2_fp64| <= 8 sqrt(3) u X h + 11 u h^2   (first order).
+// bounded by X (|x_k| <= X, and the box range L <= X) and u = 2^-24,
+// rounding the two positions (2uX), their difference (2uX), the box range
+// (uX) and the periodic correction (u|r_ij|) moves each r_ij component by at
+// most 5uX + u|r_ij|, and the FP32 sum of squares adds (D+1)u r^2, so near
+// r = h
+//   |r2_fp32 - r2_fp64| <= 10 sqrt(3) u X h + 6 u h^2   (first order).
 // The band half-width used, delta * h^2 with delta = k / h + c,
-//   k = 16 sqrt(3) u X,  c = 32 u,
-// is twice that bound (and also covers rounding h, h^2 and h^2 (1 +- delta)
-// in FP32).  Outside the band the FP32 comparison decides exactly as the
+//   k = 20 sqrt(3) u X,  c = 32 u,
+// is at least twice that bound (and also covers rounding h, h^2 and
+// h^2 (1 +- delta) in FP32).  Outside the band the FP32 comparison decides exactly as the
 // FP64 one would; inside it the caller evaluates the FP64 predicate.
 // root = tree root (center[3], edge); X is taken as max |center_k| + edge.
 SPH_DEV float fp32_margin_k(const double *root, int D)
@@ -331,7 +344,7 @@ SPH_DEV float fp32_margin_k(const double *root, int D)
     }
     X += root[3];
     // beyond 1e18 the FP32 squares could overflow: no band (always FP64)
-    return X < 1e18 ? (float)(16.0 * 1.7320508075688772 * 5.9604644775390625e-08 * X) : __builtin_inff();
+    return X < 1e18 ? (float)(20.0 * 1.7320508075688772 * 5.9604644775390625e-08 * X) : __builtin_inff();
 }
 constexpr float kMarginC = 32.0f * 5.9604644775390625e-08f;
 

@@ -160,7 +160,7 @@ __global__ __launch_bounds__(256, 4) void k_search(SearchArgs a)
     const float Rf = (float)R;
     // three times the pair kernels' band: pre-shifting a candidate to the
     // image nearest the group centre adds up to 5 u X of rounding to each
-    // component (9 u X + u |r| in all, against the 4 u X + 2 u |r| the band
+    // component (10 u X + u |r| in all, against the 5 u X + u |r| the band
     // is sized for with a factor 2)
     const float mk = 3.0f * fp32_margin_k(a.root, D);
     // candidate band of this lane (for is_ij the band of H = max(h_i, h_j) >= h_i
