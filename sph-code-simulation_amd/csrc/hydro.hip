@@ -590,7 +590,7 @@ SPH_DEV int solve_pair(const T l[4], const T r[4], T gamma, int refine64, const 
 template <int D, int K, typename T, int SOLVER, bool LEARNED>
 SPH_DEV void force_gsph_slots(const ForceArgs<T> &a, int i, int nc, int nmax, const T xi[3], const T vi[3],
                               const T gdi[3], const T gpi[3], const T gvi[3][3], T hi, T rhoi, T presi, T ci,
-                              T gamma, T rho2_inv_i, T dt, Acc<T> ac[3], Acc<T> &de, const MlpWeightsLDS *wl,
+                              T gamma, T rho2_inv_i, T dt, Acc<T> ac[3], Acc<T> &de, const MlpWeightsBF16 *wl,
                               MlpWaveLDS *wsc, bool valid)
 {
     typedef typename V4T<T>::type V4;
@@ -791,13 +791,11 @@ __global__ __launch_bounds__(256) void k_force_gsph(ForceArgs<T> a)
         nmax = o > nmax ? o : nmax;
     }
     if constexpr (LEARNED) {
-        // the learned guess needs the network's weights in LDS (whole block)
-        __shared__ MlpWeightsLDS wl;
+        // the network's bf16 image is read from global memory (L2-resident);
+        // LDS holds each wavefront's activation tiles
         __shared__ MlpWaveLDS wsc[4];
-        if (a.mlp.w) mlp_load_weights(wl, a.mlp.w);
-        __syncthreads();
         force_gsph_slots<D, K, T, SOLVER, true>(a, i, nc, nmax, xi, vi, gdi, gpi, gvi, hi, rhoi, presi, ci, gamma,
-                                               rho2_inv_i, dt, ac, de, &wl, &wsc[(threadIdx.x >> 6) & 3], valid);
+                                               rho2_inv_i, dt, ac, de, a.mlp.img, &wsc[(threadIdx.x >> 6) & 3], valid);
     } else {
         force_gsph_slots<D, K, T, SOLVER, false>(a, i, nc, nmax, xi, vi, gdi, gpi, gvi, hi, rhoi, presi, ci, gamma,
                                                 rho2_inv_i, dt, ac, de, nullptr, nullptr, valid);
@@ -830,14 +828,11 @@ __global__ void k_riemann_batch(int n, const double *__restrict__ L, const doubl
 // same gate and wave-cooperative MFMA inference as the GSPH force kernel
 __global__ __launch_bounds__(256) void k_riemann_batch_learned(int n, const double *__restrict__ L,
                                                                const double *__restrict__ R, float gamma,
-                                                               int refine64, const float *w, double *__restrict__ ps,
-                                                               double *__restrict__ vs, int *__restrict__ it,
-                                                               int *__restrict__ used)
+                                                               int refine64, const MlpWeightsBF16 *wl,
+                                                               double *__restrict__ ps, double *__restrict__ vs,
+                                                               int *__restrict__ it, int *__restrict__ used)
 {
-    __shared__ MlpWeightsLDS wl;
     __shared__ MlpWaveLDS wsc[4];
-    mlp_load_weights(wl, w);
-    __syncthreads();
     const int k0 = blockIdx.x * blockDim.x + threadIdx.x;
     const bool act = k0 < n;
     const int k = act ? k0 : n - 1;
@@ -851,7 +846,7 @@ __global__ __launch_bounds__(256) void k_riemann_batch_learned(int n, const doub
     if (__ballot(hard)) {
         float x[6] = {0, 0, 0, 0, 0, 0}, pm = 1.f;
         if (hard) mlp_features(gamma, l[0], l[1], l[2], r[0], r[1], r[2], x, pm);
-        const float y = mlp_wave_infer(wl, wsc[(threadIdx.x >> 6) & 3], x);
+        const float y = mlp_wave_infer(*wl, wsc[(threadIdx.x >> 6) & 3], x);
         guess = pm * __expf(y);
     }
     if (!act) return;
@@ -979,7 +974,10 @@ template <typename T> ForceArgs<T> force_args(sphgpu_ctx *c)
     a.refine64 = riemann_refine64(c);
     a.guess = c->prm.riemann_guess;
     a.mlp = mlp_dev(c);
-    if (c->prm.riemann_guess != SPHGPU_GUESS_LEARNED) a.mlp.w = nullptr;  // plain guess
+    if (c->prm.riemann_guess != SPHGPU_GUESS_LEARNED) {  // plain guess
+        a.mlp.w = nullptr;
+        a.mlp.img = nullptr;
+    }
     a.rlog = c->rlog_on ? c->rlog : nullptr;
     a.rlog_cap = c->rlog_cap;
     a.rlog_stride = c->rlog_stride;
@@ -1093,7 +1091,7 @@ int riemann_batch(sphgpu_ctx *c, int solver, int precision, int n, const double 
     if (precision != SPHGPU_PAIR_FP64 && solver == SPHGPU_RIEMANN_ITERATIVE &&
         c->prm.riemann_guess == SPHGPU_GUESS_LEARNED && m.w) {
         hipLaunchKernelGGL(k_riemann_batch_learned, g, bl, 0, c->stream, n, (const double *)L, (const double *)R,
-                           (float)c->prm.gamma, ref, m.w, P, V, di, di + n);
+                           (float)c->prm.gamma, ref, m.img, P, V, di, di + n);
     } else if (precision == SPHGPU_PAIR_FP64) {
         if (solver == SPHGPU_RIEMANN_HLL)
             hipLaunchKernelGGL((k_riemann_batch<double, SPHGPU_RIEMANN_HLL>), g, bl, 0, c->stream, n, L, R, c->prm.gamma, ref, P, V, di);

@@ -15,6 +15,8 @@ namespace sphgpu {
 
 struct MLP {
     float *w = nullptr;      // FP32 master parameters (device)
+    MlpWeightsBF16 *img = nullptr;  // bf16 image used by the inference
+    bool img_dirty = true;   // the image lags w
     float *m = nullptr, *v = nullptr;   // Adam moments
     float *grad = nullptr;   // reduced gradient + loss
     float *gpart = nullptr, *lpart = nullptr;  // per-tile partials
@@ -28,14 +30,20 @@ int mlp_create(sphgpu_ctx *c)
     c->mlp = new MLP;
     if (hipMalloc((void **)&c->mlp->w, sizeof(float) * kMlpParams) != hipSuccess) return SPHGPU_E_NOMEM;
     if (hipMemset(c->mlp->w, 0, sizeof(float) * kMlpParams) != hipSuccess) return SPHGPU_E_HIP;
+    if (hipMalloc((void **)&c->mlp->img, sizeof(MlpWeightsBF16)) != hipSuccess) return SPHGPU_E_NOMEM;
     return 0;
+}
+
+__global__ __launch_bounds__(256) void k_mlp_image(const float *__restrict__ w, MlpWeightsBF16 *img)
+{
+    mlp_load_weights(*img, w);
 }
 
 void mlp_destroy(sphgpu_ctx *c)
 {
     if (!c->mlp) return;
     MLP *M = c->mlp;
-    void *ps[] = {M->w, M->m, M->v, M->grad, M->gpart, M->lpart};
+    void *ps[] = {M->w, M->img, M->m, M->v, M->grad, M->gpart, M->lpart};
     for (void *q : ps)
         if (q) hipFree(q);
     delete c->mlp;
@@ -46,6 +54,14 @@ MLPDev mlp_dev(sphgpu_ctx *c)
 {
     MLPDev m;
     m.w = (c->mlp && c->mlp->trained) ? c->mlp->w : nullptr;
+    m.img = nullptr;
+    if (m.w) {
+        if (c->mlp->img_dirty) {
+            hipLaunchKernelGGL(k_mlp_image, dim3(1), dim3(256), 0, c->stream, (const float *)c->mlp->w, c->mlp->img);
+            c->mlp->img_dirty = false;
+        }
+        m.img = c->mlp->img;
+    }
     m.gamma = (float)c->prm.gamma;
     return m;
 }
@@ -279,6 +295,7 @@ int sphgpu_mlp_init(sphgpu_ctx *c, uint64_t seed)
     for (int i = 0; i < kMlpH; ++i) w[kOffW3 + i] = n3(rng);
     HIPCHK(c, hipMemcpy(c->mlp->w, w.data(), sizeof(float) * kMlpParams, hipMemcpyHostToDevice));
     c->mlp->trained = true;
+    c->mlp->img_dirty = true;
     c->mlp->step = 0;
     if (c->mlp->m) {
         HIPCHK(c, hipMemset(c->mlp->m, 0, sizeof(float) * kMlpParams));
@@ -292,6 +309,7 @@ int sphgpu_mlp_set_params(sphgpu_ctx *c, const float *p, int64_t n)
     if (!c || !p || n != kMlpParams) return SPHGPU_E_ARG;
     HIPCHK(c, hipMemcpy(c->mlp->w, p, sizeof(float) * kMlpParams, hipMemcpyHostToDevice));
     c->mlp->trained = true;
+    c->mlp->img_dirty = true;
     return 0;
 }
 
@@ -396,6 +414,7 @@ int sphgpu_mlp_train(sphgpu_ctx *c, const float *rows, int64_t n_rows, int epoch
         }
     }
     M->trained = true;
+    M->img_dirty = true;
     if (!rc && final_loss) {
         // loss over the whole set with the final parameters
         rc = mlp_pass(c, d, n_rows, 0, n_rows, M->grad, M->grad + kMlpParams, false, 0.f, 1);

@@ -27,8 +27,11 @@ constexpr int kOffW3 = kOffB2 + kMlpH;
 constexpr int kOffB3 = kOffW3 + kMlpH;
 constexpr int kMlpParams = kOffB3 + 1;
 
+struct MlpWeightsBF16;
+
 struct MLPDev {
-    const float *w;    // FP32 parameters (device), or null when untrained
+    const float *w;                // FP32 parameters (device), or null when untrained
+    const MlpWeightsBF16 *img;     // their bf16 image for the MFMA inference
     float gamma;
 };
 
@@ -95,7 +98,7 @@ constexpr int kInfK1 = 32;
 constexpr int kInfLd1 = kInfK1 + 8;
 constexpr int kInfLd = kMlpH + 8;
 
-struct MlpWeightsLDS {            // shared by the waves of a workgroup
+struct MlpWeightsBF16 {           // bf16 image of the parameters (global memory)
     mlp_bf16 W1[kMlpH][kInfLd1];  // [out][in], inputs padded to 32
     mlp_bf16 W2[kMlpH][kInfLd];   // [out][in]
     mlp_bf16 W3[16][kInfLd];      // row 0 = w3, rows 1..15 zero
@@ -109,8 +112,8 @@ struct MlpWaveLDS {               // per wavefront scratch
     float out[64];
 };
 
-// cooperative load of the FP32 parameters into bf16 LDS (whole workgroup)
-SPH_DEV void mlp_load_weights(MlpWeightsLDS &L, const float *w)
+// conversion of the FP32 parameters into the bf16 image (whole workgroup)
+SPH_DEV void mlp_load_weights(MlpWeightsBF16 &L, const float *w)
 {
     for (int e = threadIdx.x; e < kMlpH * kInfK1; e += blockDim.x) {
         const int o = e / kInfK1, k = e % kInfK1;
@@ -140,7 +143,7 @@ SPH_DEV void wave_lds_sync()
 }
 
 // whole wavefront: returns the network output (log(p*/p_m)) for this lane's row
-SPH_DEV float mlp_wave_infer(const MlpWeightsLDS &L, MlpWaveLDS &S, const float x[6])
+SPH_DEV float mlp_wave_infer(const MlpWeightsBF16 &L, MlpWaveLDS &S, const float x[6])
 {
     const int lane = threadIdx.x & 63;
     for (int k = 0; k < kInfK1; ++k) S.X[lane][k] = (mlp_bf16)(k < kMlpIn ? x[k] : 0.f);
