@@ -693,7 +693,7 @@ SPH_DEV void force_gsph_slots(const ForceArgs<T> &a, int i, int nc, int nmax, co
                 if (hard)
                     mlp_features((float)gamma, (float)left[0], (float)left[1], pl, (float)right[0],
                                  (float)right[1], pr, x, pm);
-                const float y = mlp_wave_infer(*wl, *wsc, x);
+                const float y = mlp_wave_infer(*wl, *wsc, x, hard);
                 if (hard) {
                     guess = (T)(pm * __expf(y));
                     use_guess = true;
@@ -846,7 +846,7 @@ __global__ __launch_bounds__(256) void k_riemann_batch_learned(int n, const doub
     if (__ballot(hard)) {
         float x[6] = {0, 0, 0, 0, 0, 0}, pm = 1.f;
         if (hard) mlp_features(gamma, l[0], l[1], l[2], r[0], r[1], r[2], x, pm);
-        const float y = mlp_wave_infer(*wl, wsc[(threadIdx.x >> 6) & 3], x);
+        const float y = mlp_wave_infer(*wl, wsc[(threadIdx.x >> 6) & 3], x, hard);
         guess = pm * __expf(y);
     }
     if (!act) return;

@@ -142,13 +142,20 @@ SPH_DEV void wave_lds_sync()
     __builtin_amdgcn_wave_barrier();
 }
 
-// whole wavefront: returns the network output (log(p*/p_m)) for this lane's row
-SPH_DEV float mlp_wave_infer(const MlpWeightsBF16 &L, MlpWaveLDS &S, const float x[6])
+// whole wavefront: the lanes with `use` set (at most 64) are packed into the
+// first rows, only the 16-row chunks that hold them are evaluated, and each
+// such lane gets the network output (log(p*/p_m)) for its features
+SPH_DEV float mlp_wave_infer(const MlpWeightsBF16 &L, MlpWaveLDS &S, const float x[6], bool use)
 {
     const int lane = threadIdx.x & 63;
-    for (int k = 0; k < kInfK1; ++k) S.X[lane][k] = (mlp_bf16)(k < kMlpIn ? x[k] : 0.f);
+    const unsigned long long um = __ballot(use);
+    const int row = (int)__builtin_amdgcn_mbcnt_hi((unsigned)(um >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)um, 0u));
+    const int nrows = __popcll(um);
+    if (use)
+        for (int k = 0; k < kInfK1; ++k) S.X[row][k] = (mlp_bf16)(k < kMlpIn ? x[k] : 0.f);
     const int r16 = lane & 15, kq = 8 * (lane >> 4);
-    for (int chunk = 0; chunk < 4; ++chunk) {
+    const int nchunks = (nrows + 15) >> 4;
+    for (int chunk = 0; chunk < nchunks; ++chunk) {
         wave_lds_sync();
         // layer 1: z1 (16 x 64) = X[chunk] (16 x 32) W1^T
         const mlp_bf16x8 a1 = *reinterpret_cast<const mlp_bf16x8 *>(&S.X[chunk * 16 + r16][kq]);
@@ -187,7 +194,7 @@ SPH_DEV float mlp_wave_infer(const MlpWeightsBF16 &L, MlpWaveLDS &S, const float
             for (int i = 0; i < 4; ++i) S.out[chunk * 16 + 4 * (lane >> 4) + i] = acc3[i] + L.b3;
     }
     wave_lds_sync();
-    const float y = S.out[lane];
+    const float y = use ? S.out[row] : 0.f;
     wave_lds_sync();
     return y;
 }
