@@ -61,6 +61,9 @@
 #undef class
 
 #include "../include/sphgpu.h"
+#ifdef SPHGPU_DROPIN
+#include "gpu_modules.hpp"
+#endif
 
 namespace sph {
 std::string Logger::dir_name;
@@ -150,6 +153,14 @@ void *ref_create(const sphgpu_params *prm, int n)
     r->sim->set_particle_num(n);
     for (int i = 0; i < n; ++i) r->sim->get_particles()[i].id = i;
     /* module wiring, src/solver.cpp:686-724 */
+#ifdef SPHGPU_DROPIN
+    /* the drop-in: the GPU Module adapters in place of the reference modules,
+     * the change INTEGRATION.md describes for Solver::initialize */
+    r->ts = std::make_shared<sph::gpu::TimeStep>();
+    r->pre = std::make_shared<sph::gpu::PreInteraction>();
+    r->force = std::make_shared<sph::gpu::FluidForce>();
+    if (p->type == sph::SPHType::GSPH) {
+#else
     r->ts = std::make_shared<sph::TimeStep>();
     if (p->type == sph::SPHType::SSPH) {
         r->pre = std::make_shared<sph::PreInteraction>();
@@ -160,6 +171,7 @@ void *ref_create(const sphgpu_params *prm, int n)
     } else {
         r->pre = std::make_shared<sph::gsph::PreInteraction>();
         r->force = std::make_shared<sph::gsph::FluidForce>();
+#endif
         std::vector<std::string> names = {"grad_density", "grad_pressure", "grad_velocity_0"};
 #if DIM >= 2
         names.push_back("grad_velocity_1");
@@ -169,7 +181,11 @@ void *ref_create(const sphgpu_params *prm, int n)
 #endif
         r->sim->add_vector_array(names);
     }
+#ifdef SPHGPU_DROPIN
+    r->grav = std::make_shared<sph::gpu::GravityForce>();
+#else
     r->grav = std::make_shared<sph::GravityForce>();
+#endif
     r->ts->initialize(p);
     r->pre->initialize(p);
     r->force->initialize(p);

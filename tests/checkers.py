@@ -192,6 +192,26 @@ class Reference(_Checker):
         super().__init__(_ref_libs[d], params, fields)
 
 
+_dropin_libs: dict = {}
+
+
+def dropin_available(dim: int) -> bool:
+    return (REF_DIR / f"libsphdropin_d{dim}.so").exists()
+
+
+class DropIn(_Checker):
+    """The reference's modules and driver with the GPU Module adapters of
+    sph-code-simulation_amd/dropin wired in (oracle/_ref/libsphdropin_dD.so);
+    needs a GPU."""
+    prefix = "ref_"
+
+    def __init__(self, params, fields):
+        d = params["dim"]
+        if d not in _dropin_libs:
+            _dropin_libs[d] = C.CDLL(str(REF_DIR / f"libsphdropin_d{d}.so"), mode=os.RTLD_LOCAL)
+        super().__init__(_dropin_libs[d], params, fields)
+
+
 def oracle_riemann(solver: int, gamma: float, left, right):
     """One Riemann problem through the oracle's restated solvers."""
     global _oracle_lib
