@@ -18,7 +18,7 @@ import pytest
 
 import ics
 from checkers import DropIn, Reference, dropin_available, ref_available
-from parity_util import SMALL, rel_err
+from parity_util import SMALL, r2_ref, rel_err
 
 pytestmark = pytest.mark.gpu
 
@@ -53,6 +53,13 @@ def test_dropin_modules_match_reference_modules(case):
     er, eg = r.energy(), g.energy()
     assert abs(eg[3] - er[3]) <= 1e-5 * abs(er[3])
     # neighbour counts are exact for a given h; h itself carries the FP32
-    # error, so a count may differ where a neighbour sits at r = h
-    mism = int((gf[ics.F_NEIGHBOR] != rf[ics.F_NEIGHBOR]).sum())
-    assert mism <= max(1, int(1e-3 * f.shape[1])), mism
+    # error, so a count may differ only by neighbours sitting at r = h within
+    # that error (lattice ties)
+    mism = np.nonzero(gf[ics.F_NEIGHBOR] != rf[ics.F_NEIGHBOR])[0]
+    allj = np.arange(f.shape[1])
+    for i in mism:
+        r = np.sqrt(r2_ref(p, rf, int(i), allj))
+        h = rf[ics.F_SML, i]
+        ties = int((np.abs(r - h) <= 1e-5 * h).sum())
+        assert abs(int(gf[ics.F_NEIGHBOR, i]) - int(rf[ics.F_NEIGHBOR, i])) <= ties, (int(i), ties)
+    print(f"  neighbour counts: {len(mism)} differ, all by ties at r = h")
