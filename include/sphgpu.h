@@ -199,7 +199,10 @@ int sphgpu_reset_stats(sphgpu_ctx *ctx);
 /* BHTree::neighbor_search for every particle with caller-supplied smoothing
  * lengths h[n] (index = id).  is_ij selects the FluidForce variant.  The sets
  * are returned in id space, unsorted: counts[n], lists[n*cap].  For is_ij the
- * set is the one the force loop accepts (r < max(h_i,h_j), r != 0). */
+ * set is the one the force loop accepts (r < max(h_i,h_j), r != 0).
+ * is_ij = 2 returns, without searching, the force pair sets the last
+ * sphgpu_pre_interaction derived from its own lists (h is ignored; error
+ * SPHGPU_E_STATE when none is current). */
 int sphgpu_neighbor_sets(sphgpu_ctx *ctx, const double *h, int is_ij, int cap,
                          int32_t *counts, int32_t *lists);
 /* Tree topology after sphgpu_build_tree: per node (pre-order) level, particle

@@ -246,8 +246,8 @@ int sphgpu_create(const sphgpu_params *p, int n, sphgpu_ctx **out)
     ok = ok && alloc((void **)&c->hs, dn) && alloc((void **)&c->stage, dn * SPHGPU_NFIELDS);
     c->red_blocks = 1024;
     ok = ok && alloc((void **)&c->red, sizeof(double) * 8 * c->red_blocks);
-    ok = ok && alloc((void **)&c->dscal, sizeof(double) * 16) && alloc((void **)&c->dmin, sizeof(unsigned long long) * 8) &&
-         alloc((void **)&c->dstats, sizeof(long long) * 32) && alloc((void **)&c->dflag, sizeof(int) * 8);
+    ok = ok && alloc((void **)&c->dscal, sizeof(double) * 16) && alloc((void **)&c->dmin, sizeof(unsigned long long) * kStatCopies) &&
+         alloc((void **)&c->dstats, sizeof(long long) * kStatSlots * kStatCopies) && alloc((void **)&c->dflag, sizeof(int) * 8);
     ok = ok && hipHostMalloc((void **)&c->hscal, sizeof(double) * 16, hipHostMallocDefault) == hipSuccess;
     ok = ok && tree_alloc(c) == 0;
     // neighbour lists: about 2.5x the expected candidate count nn * 1.2^D
@@ -422,9 +422,16 @@ int sphgpu_energy_momentum(sphgpu_ctx *c, double out[7])
 int sphgpu_get_stats(sphgpu_ctx *c, sphgpu_stats *out)
 {
     if (!c || !out) return SPHGPU_E_ARG;
-    long long h[32];
-    HIPCHK(c, hipMemcpyAsync(h, c->dstats, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+    std::vector<long long> all((size_t)kStatSlots * kStatCopies);
+    HIPCHK(c, hipMemcpyAsync(all.data(), c->dstats, sizeof(long long) * all.size(), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    // sum the copies (slot 12, the tree depth, is a maximum)
+    long long h[kStatSlots] = {};
+    for (int k = 0; k < kStatCopies; ++k)
+        for (int s = 0; s < kStatSlots; ++s) {
+            const long long v = all[(size_t)k * kStatSlots + s];
+            h[s] = (s == 12) ? std::max(h[s], v) : h[s] + v;
+        }
     std::memset(out, 0, sizeof(*out));
     out->pre_candidates = h[0];
     out->pre_newton_iters = h[1];
@@ -451,25 +458,31 @@ int sphgpu_get_stats(sphgpu_ctx *c, sphgpu_stats *out)
 int sphgpu_reset_stats(sphgpu_ctx *c)
 {
     if (!c) return SPHGPU_E_ARG;
-    HIPCHK(c, hipMemsetAsync(c->dstats, 0, sizeof(long long) * 32, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->dstats, 0, sizeof(long long) * kStatSlots * kStatCopies, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return 0;
 }
 
 int sphgpu_neighbor_sets(sphgpu_ctx *c, const double *h, int is_ij, int cap, int32_t *counts, int32_t *lists)
 {
-    if (!c || !h || !counts || !lists || cap <= 0) return SPHGPU_E_ARG;
+    if (!c || !counts || !lists || cap <= 0 || is_ij < 0 || is_ij > 2 || (is_ij != 2 && !h)) return SPHGPU_E_ARG;
     if (!c->tree.valid) return fail(c, SPHGPU_E_STATE, "neighbor_sets needs a built tree");
-    // smoothing lengths in id order -> sorted order
-    std::vector<double> stage((size_t)c->n * SPHGPU_NFIELDS, 0.0);
-    std::memcpy(stage.data() + (size_t)SPHGPU_F_SML * c->n, h, sizeof(double) * c->n);
-    int rc = sphgpu_upload(c, stage.data(), SPHGPU_MASK(SPHGPU_F_SML));
-    if (rc) return rc;
-    if ((rc = tree_set_kernel(c))) return rc;
-    // pre-interaction radius: h itself (r2 < h^2, src/bhtree.cpp:280-299)
-    HIPCHK(c, hipMemcpyAsync(c->hs, c->sc[S_SML], sizeof(double) * c->n, hipMemcpyDeviceToDevice, c->stream));
+    int rc;
+    if (is_ij == 2) {
+        // the force pair lists the last pre-interaction derived
+        if ((rc = flist_check(c)) < 0) return rc;
+        if (rc) return fail(c, SPHGPU_E_STATE, "neighbor_sets: no force pair list from a pre-interaction");
+    } else {
+        // smoothing lengths in id order -> sorted order
+        std::vector<double> stage((size_t)c->n * SPHGPU_NFIELDS, 0.0);
+        std::memcpy(stage.data() + (size_t)SPHGPU_F_SML * c->n, h, sizeof(double) * c->n);
+        if ((rc = sphgpu_upload(c, stage.data(), SPHGPU_MASK(SPHGPU_F_SML)))) return rc;
+        if ((rc = tree_set_kernel(c))) return rc;
+        // pre-interaction radius: h itself (r2 < h^2, src/bhtree.cpp:280-299)
+        HIPCHK(c, hipMemcpyAsync(c->hs, c->sc[S_SML], sizeof(double) * c->n, hipMemcpyDeviceToDevice, c->stream));
+        if ((rc = search(c, is_ij ? c->lforce : c->lpre, is_ij, c->hs))) return rc;
+    }
     Lists &l = is_ij ? c->lforce : c->lpre;
-    if ((rc = search(c, l, is_ij, c->hs))) return rc;
     const size_t list_len = (size_t)l.cap * (((size_t)c->n + 63) / 64) * 64;
     std::vector<int> ids(c->n), cnt(c->n), idx(list_len);
     HIPCHK(c, hipMemcpyAsync(ids.data(), c->id, sizeof(int) * c->n, hipMemcpyDeviceToHost, c->stream));

@@ -102,6 +102,8 @@ __device__ __host__ inline size_t list_slot(int i, int q, int cap)
 struct Lists {
     int *idx = nullptr;   // see list_slot; ceil(n/64) * cap * 64 entries
     int *cnt = nullptr;   // [n]
+    int *cnt_a = nullptr; // [n] force lists built from the pre-interaction:
+                          // entries [0, cnt_a) have r < h_i, the rest r < h_j only
     int cap = 0;
 };
 
@@ -154,6 +156,9 @@ struct sphgpu_ctx {
     bool mirror_dirty = true;    // pair records lag the FP64 masters
     bool pre_first = true;       // PreInteraction::m_first
     bool have_pre = false;
+    // force pair lists derived from the last pre-interaction (no is_ij
+    // search): 0 none, 1 built but not yet checked, 2 valid
+    int flist = 0;
 
     // timing
     bool timing = false;
@@ -203,6 +208,8 @@ int lists_alloc(sphgpu_ctx *c, Lists &l, int cap);
 void lists_free(Lists &l);
 // is_ij = 0: radius hs[i] (r2 < hs^2);  is_ij = 1: sqrt(r2) < max(h_i, h_j), r2 != 0
 int search(sphgpu_ctx *c, Lists &l, int is_ij, const double *radius);
+// settles a pending force list (flist == 1) from the device flag: 0 = use it
+int flist_check(sphgpu_ctx *c);
 
 // hydro.hip
 int pre_interaction(sphgpu_ctx *c);

@@ -19,6 +19,15 @@ namespace sphgpu {
 
 constexpr int kWave = 64;
 
+// Counters: kStatCopies copies of kStatSlots slots.  A launch's per-wave
+// atomics go to the copy of their workgroup, which spreads them over many
+// addresses instead of serialising them on one; the host sums the copies.
+constexpr int kStatSlots = 32, kStatCopies = 64;
+SPH_DEV unsigned long long *stat_ptr(long long *stats, int slot)
+{
+    return (unsigned long long *)&stats[kStatSlots * (blockIdx.x & (kStatCopies - 1)) + slot];
+}
+
 // Workgroups are dispatched round-robin over the 8 XCDs of an MI355X, each
 // with its own L2.  Particle kernels remap the workgroup index so that XCD x
 // runs one contiguous range of logical workgroups: particles consecutive in

@@ -127,7 +127,7 @@ __global__ __launch_bounds__(256) void k_gravity(GravArgs<T> a)
         for (int k = 0; k < D; ++k) a.acc[k][i] += (double)ac[k].get();
     }
     for (int off = kWave / 2; off > 0; off >>= 1) inter += __shfl_xor(inter, off, kWave);
-    if ((threadIdx.x & (kWave - 1)) == 0) atomicAdd((unsigned long long *)&a.stats[10], (unsigned long long)inter);
+    if ((threadIdx.x & (kWave - 1)) == 0) atomicAdd(stat_ptr(a.stats, 10), (unsigned long long)inter);
 }
 
 template <typename T> void launch_gravity(sphgpu_ctx *c)

@@ -51,6 +51,10 @@ template <typename T> struct PreArgs {
     int nn, iterative, balsara_on, td_av, gsph2;
     double gamma, kappa, alpha_min, alpha_max, eps_av;
     const double *root;           // tree root geometry (bounds |x| for the FP32 prefilter)
+    // force pair list, first part (r < h_i, r != 0), written when fl_idx is set
+    int *fl_idx, *fl_cnt, *fl_cnt_a;
+    int fl_cap;
+    int *fl_flag;                 // set when the derived list cannot be used
 };
 
 template <int D, typename T>
@@ -78,6 +82,7 @@ SPH_DEV double r2_64(const double *const x[3], const double xi[3], int j, const 
 
 SPH_DEV void wave_min_atomic(unsigned long long *dst, double v)
 {
+    dst += blockIdx.x & (kStatCopies - 1);  // one of kStatCopies copies
     for (int off = kWave / 2; off > 0; off >>= 1) {
         const double o = __shfl_xor(v, off, kWave);
         v = o < v ? o : v;
@@ -85,10 +90,10 @@ SPH_DEV void wave_min_atomic(unsigned long long *dst, double v)
     if ((threadIdx.x & (kWave - 1)) == 0) atomicMin(dst, dbits(v));
 }
 
-SPH_DEV void wave_add_atomic(long long *dst, long long v)
+SPH_DEV void wave_add_atomic(long long *stats, int slot, long long v)
 {
     for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
-    if ((threadIdx.x & (kWave - 1)) == 0) atomicAdd((unsigned long long *)dst, (unsigned long long)v);
+    if ((threadIdx.x & (kWave - 1)) == 0) atomicAdd(stat_ptr(stats, slot), (unsigned long long)v);
 }
 
 template <int D, int K, int TYPE, typename T>
@@ -164,6 +169,21 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
     Acc<T> dens_a, dhd_a, n_a, pres_a, dhp_a, dhn_a;
     T vsig = ci * T(2.0);
     int w = 0;
+    // members other than coincident ones start the force pair list: with
+    // h_i below the search radius they are exactly {j : r_ij < h_i, r_ij != 0}
+    const bool fl = a.fl_idx != nullptr;
+    int na = 0;
+    // this particle's column of the force list: entry q at fcol[64 q]
+    int *const fcol = fl ? a.fl_idx + list_slot(i, 0, a.fl_cap) : nullptr;
+    // FP32 r^2 at or below z2 may come from coincident positions (their
+    // rounding moves each component by a few u X): (2^-18 X)^2 leaves room
+    float z2 = 0.f;
+    if (fl && sizeof(T) == 4) {
+        double Xb = 0.0;
+        for (int k = 0; k < D; ++k) Xb = fmax(Xb, fabs(a.root[k]));
+        const float padX = (float)((Xb + a.root[3]) * 3.814697265625e-06);
+        z2 = padX * padX;
+    }
     for (int q = 0; q < nc; ++q) {
         const int j = a.idx[list_slot(i, q, a.cap)];
         T d[3];
@@ -173,6 +193,16 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
 #pragma unroll
         for (int b = 0; b < 4; ++b) mb[b] |= ((q >> 6) == b) ? bit : 0ull;
         ++w;
+        if (fl && j != i) {
+            // r_ij != 0 as the reference decides it (FP64); an FP32 r^2 above
+            // z2 cannot come from coincident positions
+            bool nz = r2 != T(0);
+            if (sizeof(T) == 4 && !((float)r2 > z2)) nz = r2_64(a.x64, xi64, j, a.box, D) != 0.0;
+            if (nz) {
+                if (na < a.fl_cap) fcol[na * kWave] = j;
+                ++na;
+            }
+        }
         const T r = tsqrt(r2);
         const T wk = Kern<D, K, T>::w(r, h);
         const V4 vj = RV[j];
@@ -225,10 +255,17 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
         a.rec_w[R_THERMO][4 * (size_t)i + 1] = pres;
         if (TYPE != SPHGPU_GSPH) a.rec_w[R_AUX][4 * (size_t)i + 0] = gradh;
         if (a.posf_w) a.posf_w[4 * (size_t)i + 3] = (float)h;
+        if (fl) {
+            a.fl_cnt_a[i] = min(na, a.fl_cap);
+            a.fl_cnt[i] = min(na, a.fl_cap);
+            // complete only when every j with r < h_i was a candidate
+            // (r^2 < hs^2): h_i must stay below hs_i with room for rounding
+            if (na > a.fl_cap || !(h64 < hs64 * (1.0 - 9.094947017729282e-13))) atomicOr(a.fl_flag, 1);
+        }
     }
     wave_min_atomic(a.hvs_min, valid ? (double)(h / vsig) : 1.7976931348623157e308);
-    wave_add_atomic(&a.stats[1], valid ? newton_iters : 0);
-    wave_add_atomic(&a.stats[2], valid ? newton_fail : 0);
+    wave_add_atomic(a.stats, 1, valid ? newton_iters : 0);
+    wave_add_atomic(a.stats, 2, valid ? newton_fail : 0);
     if (!valid) return;
 
     if (TYPE == SPHGPU_GSPH) {
@@ -360,6 +397,92 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
     }
 }
 
+// ---- force pair lists from the pre-interaction lists ----------------------------
+// The FluidForce pair set of i is {j : r_ij < max(h_i, h_j), r_ij != 0}
+// (src/fluid_force.cpp:43-62, src/gsph/g_fluid_force.cpp:93): the members
+// with r < h_i, which k_pre wrote, plus every j with h_j > r_ij >= h_i.  The
+// latter pairs are exactly the transposes of k_pre's entries (i in the list
+// of j) that fail r < h_i; since the FP64 r_ij^2 is symmetric in i and j,
+// the same predicate decides them.  This pass appends them; the next one
+// puts each appended run in index order so the lists do not depend on the
+// order of the atomics.
+struct FlistArgs {
+    int n;
+    int *idx, *cnt;
+    const int *cnt_a;
+    int cap;
+    const float4 *posf;       // FP32 (x, y, z, h)
+    const double *x64[3];
+    const double *sml;
+    const double *root;
+    Box box;
+    int *flag;
+    long long *stats;
+};
+
+template <int D, bool FP64>
+__global__ __launch_bounds__(256) void k_flist_scatter(FlistArgs a)
+{
+    const int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    const int na = a.cnt_a[i];
+    const float4 pi = a.posf[i];
+    const double hi = a.sml[i];
+    double xi64[3] = {0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < D; ++k) xi64[k] = a.x64[k][i];
+    const float xf[3] = {pi.x, pi.y, pi.z};
+    const float mk = fp32_margin_k(a.root, D);
+    for (int q = 0; q < na; ++q) {
+        const int j = a.idx[list_slot(i, q, a.cap)];
+        const float4 pj = a.posf[j];
+        // FP32 rounding is monotone: h_j > h_i when the rounded values say so,
+        // and then r < h_i <= h_j puts i in the list of j already
+        if (pj.w > pi.w) continue;
+        const double hj = a.sml[j];
+        if (!(hj < hi)) continue;
+        bool in_j;  // r_ij < h_j
+        if (FP64) {
+            in_j = r_less(r2_64(a.x64, xi64, j, a.box, D), hj);
+        } else {
+            float d[3];
+            const float r2f = rec_r2<D, float>(pj, xf, a.box, d);
+            const int band = fp32_band(r2f, pj.w, mk);
+            in_j = band > 0 || (band == 0 && r_less(r2_64(a.x64, xi64, j, a.box, D), hj));
+        }
+        if (in_j) continue;
+        const int s = atomicAdd(&a.cnt[j], 1);
+        if (s < a.cap) a.idx[list_slot(j, s, a.cap)] = i;
+        else atomicOr(a.flag, 1);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_flist_order(FlistArgs a)
+{
+    const int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    int nt = 0;
+    if (i < a.n) {
+        const int na = a.cnt_a[i];
+        nt = min(a.cnt[i], a.cap);
+        // insertion sort of the appended run (a few entries)
+        for (int q = na + 1; q < nt; ++q) {
+            const int v = a.idx[list_slot(i, q, a.cap)];
+            int p = q - 1;
+            int u = a.idx[list_slot(i, p, a.cap)];
+            while (p >= na && u > v) {
+                a.idx[list_slot(i, p + 1, a.cap)] = u;
+                --p;
+                if (p >= na) u = a.idx[list_slot(i, p, a.cap)];
+            }
+            a.idx[list_slot(i, p + 1, a.cap)] = v;
+        }
+        a.cnt[i] = nt;
+    }
+    long long c = nt;
+    for (int off = kWave / 2; off > 0; off >>= 1) c += __shfl_xor(c, off, kWave);
+    if ((threadIdx.x & (kWave - 1)) == 0) atomicAdd(stat_ptr(a.stats, 4), (unsigned long long)c);
+}
+
 // ---- initial_smoothing (pre_interaction.cpp:171-215) ----------------------------
 template <int D, int K>
 __global__ __launch_bounds__(256) void k_initial_density(int n, int cap, const int *__restrict__ idx, const int *__restrict__ cnt,
@@ -396,10 +519,15 @@ __global__ void k_hguess(int n, const double *__restrict__ mass, const double *_
 
 __global__ void k_minbits_init(unsigned long long *m)
 {
-    m[threadIdx.x] = 0x7fefffffffffffffull;  // DBL_MAX
+    if (threadIdx.x < kStatCopies) m[threadIdx.x] = 0x7fefffffffffffffull;  // DBL_MAX
 }
 
-__global__ void k_store_hvs(const unsigned long long *m, double *dscal) { dscal[2] = bitsd(m[0]); }
+__global__ void k_store_hvs(const unsigned long long *m, double *dscal)
+{
+    unsigned long long v = m[0];
+    for (int k = 1; k < kStatCopies; ++k) v = m[k] < v ? m[k] : v;
+    dscal[2] = bitsd(v);
+}
 
 // ---- FluidForce --------------------------------------------------------------------
 template <typename T> struct ForceArgs {
@@ -745,13 +873,13 @@ SPH_DEV void force_gsph_slots(const ForceArgs<T> &a, int i, int nc, int nmax, co
             wave_max_sum += m;
         }
     }
-    wave_add_atomic(&a.stats[3], valid ? nc : 0);
+    wave_add_atomic(a.stats, 3, valid ? nc : 0);
     if (SOLVER == SPHGPU_RIEMANN_ITERATIVE) {
-        wave_add_atomic(&a.stats[5], iters);
-        if ((threadIdx.x & (kWave - 1)) == 0) atomicAdd((unsigned long long *)&a.stats[6], (unsigned long long)wave_max_sum);
-        wave_add_atomic(&a.stats[7], calls);
-        wave_add_atomic(&a.stats[8], fallbacks);
-        wave_add_atomic(&a.stats[9], learned);
+        wave_add_atomic(a.stats, 5, iters);
+        if ((threadIdx.x & (kWave - 1)) == 0) atomicAdd(stat_ptr(a.stats, 6), (unsigned long long)wave_max_sum);
+        wave_add_atomic(a.stats, 7, calls);
+        wave_add_atomic(a.stats, 8, fallbacks);
+        wave_add_atomic(a.stats, 9, learned);
     }
 }
 
@@ -892,7 +1020,41 @@ template <int D, int K, int TYPE, typename T> void launch_pre(sphgpu_ctx *c)
     a.alpha_min = c->prm.alpha_min;
     a.alpha_max = c->prm.alpha_max;
     a.eps_av = c->prm.epsilon_av;
+    // with the iterative h the final h stays below the search radius, so the
+    // force pair lists follow from these lists (non-iterative: h = hs, and
+    // r < h need not imply r^2 < hs^2 at the last bit: search instead)
+    const bool fl = c->prm.iterative_sml != 0;
+    a.fl_idx = fl ? c->lforce.idx : nullptr;
+    a.fl_cnt = c->lforce.cnt;
+    a.fl_cnt_a = c->lforce.cnt_a;
+    a.fl_cap = c->lforce.cap;
+    a.fl_flag = c->dflag + 2;
     hipLaunchKernelGGL((k_pre<D, K, TYPE, T>), dim3(grid_for(c->n, 256)), dim3(256), 0, c->stream, a);
+}
+
+template <int D> void launch_flist(sphgpu_ctx *c)
+{
+    FlistArgs a;
+    a.n = c->n;
+    a.idx = c->lforce.idx;
+    a.cnt = c->lforce.cnt;
+    a.cnt_a = c->lforce.cnt_a;
+    a.cap = c->lforce.cap;
+    a.posf = c->rec32[R_POSH];
+    for (int k = 0; k < 3; ++k) a.x64[k] = c->pos[k < c->dim ? k : 0];
+    a.sml = c->sc[S_SML];
+    a.root = c->tree.root;
+    a.box = c->box;
+    a.flag = c->dflag + 2;
+    a.stats = c->dstats;
+    const dim3 g(grid_for(c->n, 256)), b(256);
+    time_begin(c, "force_lists");
+    if (c->prm.pair_precision == SPHGPU_PAIR_FP64) hipLaunchKernelGGL((k_flist_scatter<D, true>), g, b, 0, c->stream, a);
+    else hipLaunchKernelGGL((k_flist_scatter<D, false>), g, b, 0, c->stream, a);
+    time_end(c);
+    time_begin(c, "force_lists_order");
+    hipLaunchKernelGGL(k_flist_order, g, b, 0, c->stream, a);
+    time_end(c);
 }
 
 template <int D, int K, typename T> void dispatch_pre_type(sphgpu_ctx *c)
@@ -1049,13 +1211,22 @@ int pre_interaction(sphgpu_ctx *c)
     hguess(c, kappa, c->hs);
     time_end(c);
     if ((rc = search(c, c->lpre, 0, c->hs))) return rc;
-    hipLaunchKernelGGL(k_minbits_init, dim3(1), dim3(8), 0, c->stream, c->dmin);
+    hipLaunchKernelGGL(k_minbits_init, dim3(1), dim3(kStatCopies), 0, c->stream, c->dmin);
+    const bool fl = c->prm.iterative_sml != 0;
+    if (fl) HIPCHK(c, hipMemsetAsync(c->dflag + 2, 0, sizeof(int), c->stream));
     time_begin(c, "pre_interaction");
     if (c->prm.pair_precision == SPHGPU_PAIR_FP64) dispatch_pre<double>(c);
     else dispatch_pre<float>(c);
     time_end(c);
     hipLaunchKernelGGL(k_store_hvs, dim3(1), dim3(1), 0, c->stream, (const unsigned long long *)c->dmin, c->dscal);
     HIPCHK(c, hipGetLastError());
+    if (fl) {
+        if (c->dim == 1) launch_flist<1>(c);
+        else if (c->dim == 2) launch_flist<2>(c);
+        else launch_flist<3>(c);
+        HIPCHK(c, hipGetLastError());
+        c->flist = 1;  // checked (one flag read) by the first fluid_force
+    }
     c->have_pre = true;
     return tree_set_kernel(c);
 }
@@ -1065,7 +1236,11 @@ int fluid_force(sphgpu_ctx *c)
     if (!c->tree.valid) return fail(c, SPHGPU_E_STATE, "fluid_force needs a built tree");
     int rc;
     if (c->mirror_dirty && (rc = refresh_mirrors(c))) return rc;
-    if ((rc = search(c, c->lforce, 1, c->sc[S_SML]))) return rc;
+    // the pair lists derived by the pre-interaction when they are complete,
+    // the is_ij tree search otherwise
+    const int fc = flist_check(c);
+    if (fc < 0) return fc;
+    if (fc && (rc = search(c, c->lforce, 1, c->sc[S_SML]))) return rc;
     time_begin(c, "fluid_force");
     if (c->prm.pair_precision == SPHGPU_PAIR_FP64) dispatch_force<double>(c);
     else dispatch_force<float>(c);

@@ -285,6 +285,7 @@ int energy_momentum(sphgpu_ctx *c, double out[7])
 
 int refresh_mirrors(sphgpu_ctx *c)
 {
+    c->flist = 0;  // records change: a force list derived earlier is stale
     RecArgs a;
     a.n = c->n;
     const bool gsph2 = c->prm.sph_type == SPHGPU_GSPH && c->prm.gsph_2nd_order;

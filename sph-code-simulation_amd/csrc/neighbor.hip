@@ -492,12 +492,12 @@ __global__ __launch_bounds__(256, 4) void k_search(SearchArgs a)
     for (int off = kWave / 2; off > 0; off >>= 1) c += __shfl_xor(c, off, kWave);
     for (int off = kWave / 2; off > 0; off >>= 1) nexact += __shfl_xor(nexact, off, kWave);
     if (lane == 0) {
-        atomicAdd((unsigned long long *)&a.stats[17], (unsigned long long)nexact);
-        atomicAdd((unsigned long long *)&a.stats[IS_IJ ? 4 : 0], (unsigned long long)c);
-        atomicAdd((unsigned long long *)&a.stats[13], (unsigned long long)visits);
-        atomicAdd((unsigned long long *)&a.stats[14], (unsigned long long)tests);
-        atomicAdd((unsigned long long *)&a.stats[15], (unsigned long long)windows);
-        atomicAdd((unsigned long long *)&a.stats[16], (unsigned long long)leaves);
+        atomicAdd(stat_ptr(a.stats, 17), (unsigned long long)nexact);
+        atomicAdd(stat_ptr(a.stats, IS_IJ ? 4 : 0), (unsigned long long)c);
+        atomicAdd(stat_ptr(a.stats, 13), (unsigned long long)visits);
+        atomicAdd(stat_ptr(a.stats, 14), (unsigned long long)tests);
+        atomicAdd(stat_ptr(a.stats, 15), (unsigned long long)windows);
+        atomicAdd(stat_ptr(a.stats, 16), (unsigned long long)leaves);
     }
 }
 
@@ -510,6 +510,7 @@ int lists_alloc(sphgpu_ctx *c, Lists &l, int cap)
     const size_t blocks = ((size_t)c->n + 63) / 64;
     if (hipMalloc((void **)&l.idx, sizeof(int) * (size_t)cap * blocks * 64) != hipSuccess) return SPHGPU_E_NOMEM;
     if (hipMalloc((void **)&l.cnt, sizeof(int) * (size_t)c->n) != hipSuccess) return SPHGPU_E_NOMEM;
+    if (hipMalloc((void **)&l.cnt_a, sizeof(int) * (size_t)c->n) != hipSuccess) return SPHGPU_E_NOMEM;
     if (hipMemset(l.cnt, 0, sizeof(int) * (size_t)c->n) != hipSuccess) return SPHGPU_E_HIP;
     return 0;
 }
@@ -518,13 +519,26 @@ void lists_free(Lists &l)
 {
     if (l.idx) hipFree(l.idx);
     if (l.cnt) hipFree(l.cnt);
+    if (l.cnt_a) hipFree(l.cnt_a);
     l = Lists();
+}
+
+int flist_check(sphgpu_ctx *c)
+{
+    if (c->flist == 1) {
+        int bad = 1;
+        HIPCHK(c, hipMemcpyAsync(&bad, c->dflag + 2, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        c->flist = bad ? 0 : 2;
+    }
+    return c->flist == 2 ? 0 : 1;
 }
 
 int search(sphgpu_ctx *c, Lists &l, int is_ij, const double *radius)
 {
     Tree &t = c->tree;
     if (!t.valid) return fail(c, SPHGPU_E_STATE, "neighbour search without a tree");
+    if (&l == &c->lforce) c->flist = 0;  // the derived force list is overwritten
     for (int attempt = 0; attempt < 6; ++attempt) {
         SearchArgs a;
         a.n = c->n;

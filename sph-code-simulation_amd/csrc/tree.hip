@@ -450,8 +450,8 @@ __global__ void k_fill(const unsigned long long *__restrict__ key, const unsigne
         }
         e = e * 0.5;
     }
-    if (leaves) atomicAdd((unsigned long long *)&stats[11], (unsigned long long)leaves);
-    atomicMax((unsigned long long *)&stats[12], (unsigned long long)L1);
+    if (leaves) atomicAdd(stat_ptr(stats, 11), (unsigned long long)leaves);
+    atomicMax(stat_ptr(stats, 12), (unsigned long long)L1);
 }
 
 // ---- 7. block hierarchy and node reductions ------------------------------------------
@@ -828,6 +828,7 @@ int tree_build(sphgpu_ctx *c)
     HIPCHK(c, hipGetLastError());
     t.valid = true;
     c->sorted_valid = true;
+    c->flist = 0;  // particles were reordered
     // the inputs of the pair loops moved: refresh their FP32 copies
     return refresh_mirrors(c);
 }

@@ -238,12 +238,17 @@ class Context:
 
     def reset_stats(self): self._check(self.lib.sphgpu_reset_stats(self.h), "reset_stats")
 
-    def neighbor_sets(self, h: np.ndarray, is_ij: bool, cap: int = 512):
-        h = np.ascontiguousarray(h, dtype=np.float64)
+    def neighbor_sets(self, h, is_ij, cap: int = 512):
+        """is_ij 0/1: search with the given h; is_ij 2: the force pair sets
+        derived by the last pre_interaction (h unused, may be None)."""
+        hp = None
+        if h is not None:
+            h = np.ascontiguousarray(h, dtype=np.float64)
+            hp = _dp(h)
         counts = np.zeros(self.n, dtype=np.int32)
         lists = np.zeros(self.n * cap, dtype=np.int32)
         self._check(self.lib.sphgpu_neighbor_sets(
-            self.h, _dp(h), int(is_ij), int(cap),
+            self.h, hp, int(is_ij), int(cap),
             counts.ctypes.data_as(C.POINTER(C.c_int32)),
             lists.ctypes.data_as(C.POINTER(C.c_int32))), "neighbor_sets")
         return counts, lists.reshape(self.n, cap)

@@ -124,6 +124,46 @@ def test_neighbor_sets_exact(case, is_ij):
     print(f"{case} is_ij={is_ij}: {len(idx)} particles, {total} neighbours, all sets identical")
 
 
+@pytest.mark.parametrize("case", list(SMALL))
+@pytest.mark.parametrize("prec", [ics.PAIR_FP64, ics.PAIR_FP32])
+def test_force_lists_from_pre_exact(case, prec):
+    """The force pair lists the pre-interaction derives (members r < h_i plus
+    the transposed pairs r < h_j) are the FluidForce sets of
+    src/fluid_force.cpp:43-62, checked by brute force over all j with the
+    final h, and equal to what the is_ij tree search returns."""
+    p, f = SMALL[case]()
+    g = gpu_ctx(p, f, pair_precision=prec)
+    g.initialize()
+    g.integrate(3)
+    g.build_tree()
+    g.pre_interaction()
+    g.fluid_force()
+    # still current after the force call: the force used it (a fallback
+    # search would have replaced it)
+    counts, lists = g.neighbor_sets(None, 2, cap=1024)
+    gf = g.download()
+    n = gf.shape[1]
+    allj = np.arange(n)
+    idx = np.arange(n) if n <= 5000 else np.random.default_rng(2).choice(n, 5000, replace=False)
+    total = extra = 0
+    h = gf[ics.F_SML]
+    for i in idx:
+        ref = force_pairs(p, gf, int(i), allj)
+        got = lists[i, :counts[i]]
+        assert counts[i] == len(ref), (case, i, counts[i], len(ref))
+        np.testing.assert_array_equal(np.sort(got), np.sort(ref))
+        total += len(ref)
+        extra += int((h[ref] > h[i]).sum())
+    c2, l2 = g.neighbor_sets(h, True, cap=1024)
+    np.testing.assert_array_equal(c2, counts)
+    for i in idx:
+        np.testing.assert_array_equal(np.sort(l2[i, :c2[i]]), np.sort(lists[i, :counts[i]]))
+    with pytest.raises(sphgpu.SphGpuError):
+        g.neighbor_sets(None, 2, cap=1024)  # the search replaced the derived lists
+    print(f"{case} prec={prec}: {len(idx)} particles, {total} pairs ({extra} with h_j > h_i), "
+          "all sets identical to brute force and to the is_ij search")
+
+
 def force_errors(gf, of, d):
     """Acceleration and du/dt errors per particle, normalised by the size of
     one pair's pressure term, P/(rho h) (and P c/(rho h) for du/dt), plus the
