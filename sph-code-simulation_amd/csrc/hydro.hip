@@ -51,8 +51,11 @@ template <typename T> struct PreArgs {
     int nn, iterative, balsara_on, td_av, gsph2;
     double gamma, kappa, alpha_min, alpha_max, eps_av;
     const double *root;           // tree root geometry (bounds |x| for the FP32 prefilter)
-    // force pair list, first part (r < h_i, r != 0), written when fl_idx is set
-    int *fl_idx, *fl_cnt, *fl_cnt_a;
+    // force pair list, first part: membership masks over the first 256
+    // candidates ([4][n], bit q = candidate q has r < h_i and r != 0) and
+    // counts, written when fl_mask is set
+    unsigned long long *fl_mask;
+    int *fl_cnt, *fl_cnt_a;
     int fl_cap;
     int *fl_flag;                 // set when the derived list cannot be used
 };
@@ -170,11 +173,13 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
     T vsig = ci * T(2.0);
     int w = 0;
     // members other than coincident ones start the force pair list: with
-    // h_i below the search radius they are exactly {j : r_ij < h_i, r_ij != 0}
-    const bool fl = a.fl_idx != nullptr;
-    int na = 0;
-    // this particle's column of the force list: entry q at fcol[64 q]
-    int *const fcol = fl ? a.fl_idx + list_slot(i, 0, a.fl_cap) : nullptr;
+    // h_i below the search radius they are exactly {j : r_ij < h_i, r_ij != 0}.
+    // They are passed on as the membership bits without the particle itself;
+    // a coincident pair j != i (r_ij = 0) or more than 256 candidates make
+    // the derived list unusable for this step (the force searches instead).
+    const bool fl = a.fl_mask != nullptr;
+    int qself = -1;
+    bool coincident = false;
     // FP32 r^2 at or below z2 may come from coincident positions (their
     // rounding moves each component by a few u X): (2^-18 X)^2 leaves room
     float z2 = 0.f;
@@ -193,14 +198,15 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
 #pragma unroll
         for (int b = 0; b < 4; ++b) mb[b] |= ((q >> 6) == b) ? bit : 0ull;
         ++w;
-        if (fl && j != i) {
-            // r_ij != 0 as the reference decides it (FP64); an FP32 r^2 above
-            // z2 cannot come from coincident positions
-            bool nz = r2 != T(0);
-            if (sizeof(T) == 4 && !((float)r2 > z2)) nz = r2_64(a.x64, xi64, j, a.box, D) != 0.0;
-            if (nz) {
-                if (na < a.fl_cap) fcol[na * kWave] = j;
-                ++na;
+        if (fl) {
+            if (j == i) {
+                qself = q;
+            } else {
+                // r_ij != 0 as the reference decides it (FP64); an FP32 r^2
+                // above z2 cannot come from coincident positions
+                bool nz = r2 != T(0);
+                if (sizeof(T) == 4 && !((float)r2 > z2)) nz = r2_64(a.x64, xi64, j, a.box, D) != 0.0;
+                coincident = coincident || !nz;
             }
         }
         const T r = tsqrt(r2);
@@ -256,11 +262,20 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
         if (TYPE != SPHGPU_GSPH) a.rec_w[R_AUX][4 * (size_t)i + 0] = gradh;
         if (a.posf_w) a.posf_w[4 * (size_t)i + 3] = (float)h;
         if (fl) {
+            int na = 0;
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const unsigned long long am = mb[b] & ~(((qself >> 6) == b) ? 1ull << (qself & 63) : 0ull);
+                a.fl_mask[(size_t)b * a.n + i] = am;
+                na += __popcll(am);
+            }
             a.fl_cnt_a[i] = min(na, a.fl_cap);
             a.fl_cnt[i] = min(na, a.fl_cap);
             // complete only when every j with r < h_i was a candidate
             // (r^2 < hs^2): h_i must stay below hs_i with room for rounding
-            if (na > a.fl_cap || !(h64 < hs64 * (1.0 - 9.094947017729282e-13))) atomicOr(a.fl_flag, 1);
+            if (nc > 256 || qself < 0 || coincident || na > a.fl_cap ||
+                !(h64 < hs64 * (1.0 - 9.094947017729282e-13)))
+                atomicOr(a.fl_flag, 1);
         }
     }
     wave_min_atomic(a.hvs_min, valid ? (double)(h / vsig) : 1.7976931348623157e308);
@@ -400,10 +415,12 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
 // ---- force pair lists from the pre-interaction lists ----------------------------
 // The FluidForce pair set of i is {j : r_ij < max(h_i, h_j), r_ij != 0}
 // (src/fluid_force.cpp:43-62, src/gsph/g_fluid_force.cpp:93): the members
-// with r < h_i, which k_pre wrote, plus every j with h_j > r_ij >= h_i.  The
-// latter pairs are exactly the transposes of k_pre's entries (i in the list
+// with r < h_i, which k_pre marked, plus every j with h_j > r_ij >= h_i.  The
+// latter pairs are exactly the transposes of k_pre's members (i in the list
 // of j) that fail r < h_i; since the FP64 r_ij^2 is symmetric in i and j,
-// the same predicate decides them.  This pass appends them; the next one
+// the same predicate decides them.  This pass writes each particle's members
+// (one list row per member rank, so a wavefront's stores are contiguous) and
+// appends the transposed pairs to the other particle's list; the next one
 // puts each appended run in index order so the lists do not depend on the
 // order of the atomics.
 struct FlistArgs {
@@ -411,6 +428,9 @@ struct FlistArgs {
     int *idx, *cnt;
     const int *cnt_a;
     int cap;
+    const int *pidx;          // pre-interaction candidate lists
+    int pcap;
+    const unsigned long long *mask;  // [4][n] member bits over the candidates
     const float4 *posf;       // FP32 (x, y, z, h)
     const double *x64[3];
     const double *sml;
@@ -433,8 +453,19 @@ __global__ __launch_bounds__(256) void k_flist_scatter(FlistArgs a)
     for (int k = 0; k < D; ++k) xi64[k] = a.x64[k][i];
     const float xf[3] = {pi.x, pi.y, pi.z};
     const float mk = fp32_margin_k(a.root, D);
+    int wd = 0;
+    unsigned long long m = a.mask[i];
     for (int q = 0; q < na; ++q) {
-        const int j = a.idx[list_slot(i, q, a.cap)];
+        // next member: lowest remaining bit of the candidate masks
+        while (m == 0ull && wd < 3) {
+            ++wd;
+            m = a.mask[(size_t)wd * a.n + i];
+        }
+        if (m == 0ull) break;
+        const int qc = (wd << 6) + __ffsll((long long)m) - 1;
+        m &= m - 1ull;
+        const int j = a.pidx[list_slot(i, qc, a.pcap)];
+        a.idx[list_slot(i, q, a.cap)] = j;
         const float4 pj = a.posf[j];
         // FP32 rounding is monotone: h_j > h_i when the rounded values say so,
         // and then r < h_i <= h_j puts i in the list of j already
@@ -1024,7 +1055,7 @@ template <int D, int K, int TYPE, typename T> void launch_pre(sphgpu_ctx *c)
     // force pair lists follow from these lists (non-iterative: h = hs, and
     // r < h need not imply r^2 < hs^2 at the last bit: search instead)
     const bool fl = c->prm.iterative_sml != 0;
-    a.fl_idx = fl ? c->lforce.idx : nullptr;
+    a.fl_mask = fl ? c->amask : nullptr;
     a.fl_cnt = c->lforce.cnt;
     a.fl_cnt_a = c->lforce.cnt_a;
     a.fl_cap = c->lforce.cap;
@@ -1040,6 +1071,9 @@ template <int D> void launch_flist(sphgpu_ctx *c)
     a.cnt = c->lforce.cnt;
     a.cnt_a = c->lforce.cnt_a;
     a.cap = c->lforce.cap;
+    a.pidx = c->lpre.idx;
+    a.pcap = c->lpre.cap;
+    a.mask = c->amask;
     a.posf = c->rec32[R_POSH];
     for (int k = 0; k < 3; ++k) a.x64[k] = c->pos[k < c->dim ? k : 0];
     a.sml = c->sc[S_SML];

@@ -151,13 +151,9 @@ __global__ __launch_bounds__(256, 4) void k_search(SearchArgs a)
     if (group * kWave >= a.n) return;
     const bool valid = i < a.n;
     const int ii = valid ? i : (a.n - 1);
-    double xi[3] = {0, 0, 0};
-#pragma unroll
-    for (int k = 0; k < D; ++k) xi[k] = a.x[k][ii];
-    const double R = a.radius[ii];
     const float4 pfi = a.posf[ii];
     const float xf[3] = {pfi.x, pfi.y, pfi.z};
-    const float Rf = (float)R;
+    const float Rf = (float)a.radius[ii];
     // three times the pair kernels' band: pre-shifting a candidate to the
     // image nearest the group centre adds up to 5 u X of rounding to each
     // component (10 u X + u |r| in all, against the 5 u X + u |r| the band
@@ -182,7 +178,9 @@ __global__ __launch_bounds__(256, 4) void k_search(SearchArgs a)
     const float z2 = padX * padX;
     int count = 0;
     bool over = false;
-    long long visits = 0, tests = 0, windows = 0, leaves = 0;
+    // this particle's column of the list: entry q at col[64 q]
+    int *const col = a.idx + list_slot(ii, 0, a.cap);
+    int visits = 0, tests = 0, windows = 0, leaves = 0;
     const int p_end = min(group * kWave + kWave, a.n);
     int sub = group * kWave;
     // candidate buffer: lane q holds the q-th pending candidate index
@@ -201,13 +199,19 @@ __global__ __launch_bounds__(256, 4) void k_search(SearchArgs a)
     float g[3] = {0, 0, 0};
     int shift = 1;
 
-    // the FP64 predicate (the reference's) for candidate j
-    long long nexact = 0;
+    // the FP64 predicate (the reference's) for candidate j; the particle's
+    // own FP64 position and radius are reloaded here (a rare path) rather
+    // than held in registers for the whole walk
+    int nexact = 0;
     auto exact = [&](int j) -> bool {
         ++nexact;
-        double xj64[3] = {0, 0, 0};
+        double xi[3] = {0, 0, 0}, xj64[3] = {0, 0, 0};
 #pragma unroll
-        for (int k = 0; k < D; ++k) xj64[k] = a.x[k][j];
+        for (int k = 0; k < D; ++k) {
+            xi[k] = a.x[k][ii];
+            xj64[k] = a.x[k][j];
+        }
+        const double R = a.radius[ii];
         const double r2 = r2_exact<D>(xi, xj64, a.box);
         if (IS_IJ) {
             const double hj = a.sml[j];
@@ -272,7 +276,7 @@ __global__ __launch_bounds__(256, 4) void k_search(SearchArgs a)
             const int j = sidx[t];
             if (km) {
                 km &= km - 1;
-                if (count < a.cap) a.idx[list_slot(i, count, a.cap)] = j;
+                if (count < a.cap) col[count * kWave] = j;
                 else over = true;
                 ++count;
             }
@@ -438,11 +442,14 @@ __global__ __launch_bounds__(256, 4) void k_search(SearchArgs a)
                 int4 lk = make_int4(0, 0, 0, 0);
                 float4 bx = make_float4(0.f, 0.f, 0.f, 0.f);
                 float2 bh = make_float2(0.f, 0.f);
+                int2 ch = make_int2(0, 0);
                 bool reach = false;
                 if (act) {
+                    // the child list is loaded with the node (no dependent load)
                     bx = a.nbox[node];
                     bh = a.nhalf[node];
                     lk = a.nlink[node];
+                    ch = a.nchild[node];
                     reach = reach_node(bx, bh, lk);
                 }
                 visits += min(kWave, nf - base);
@@ -454,8 +461,6 @@ __global__ __launch_bounds__(256, 4) void k_search(SearchArgs a)
                     visit_leaf(o, bx, bh, lk);
                 }
                 const bool expand = act && reach && !is_leaf;
-                int2 ch = make_int2(0, 0);
-                if (expand) ch = a.nchild[node];
                 const int cnt = expand ? ch.y : 0;
                 int incl = cnt;
                 for (int off = 1; off < kWave; off <<= 1) {
@@ -465,7 +470,16 @@ __global__ __launch_bounds__(256, 4) void k_search(SearchArgs a)
                 const int total = __shfl(incl, kWave - 1, kWave);
                 if (nn + total <= kFrontier) {
                     const int pos = nn + incl - cnt;
-                    for (int c = 0; c < cnt; ++c) nxt[pos + c] = a.child[ch.x + c];
+                    if (expand) {
+                        // at most 2^D children; the child array is padded by
+                        // 8 entries, so the 2^D loads are independent and in bounds
+                        int cv[1 << D];
+#pragma unroll
+                        for (int c = 0; c < (1 << D); ++c) cv[c] = a.child[ch.x + c];
+#pragma unroll
+                        for (int c = 0; c < (1 << D); ++c)
+                            if (c < cnt) nxt[pos + c] = cv[c];
+                    }
                     nn += total;
                 } else {
                     unsigned long long em = __ballot(expand);
@@ -488,7 +502,7 @@ __global__ __launch_bounds__(256, 4) void k_search(SearchArgs a)
     if (valid) a.cnt[i] = count < a.cap ? count : a.cap;
     if (over) atomicOr(a.overflow, 1);
     // candidate statistics
-    long long c = valid ? count : 0;
+    int c = valid ? count : 0;
     for (int off = kWave / 2; off > 0; off >>= 1) c += __shfl_xor(c, off, kWave);
     for (int off = kWave / 2; off > 0; off >>= 1) nexact += __shfl_xor(nexact, off, kWave);
     if (lane == 0) {

@@ -377,20 +377,28 @@ __global__ void k_lambda(const unsigned char *__restrict__ kdig, TreeShape sh, u
     ncount[i] = (i == 0 ? 1 : 0) + max(0, L - lmin + 1);
 }
 
-// exclusive scan of ncount -> nbase (n+1 entries), single workgroup
 // ---- 6. node fill ------------------------------------------------------------------
-__device__ int prefix_end(const unsigned long long *__restrict__ key, int n, int s, int shift)
+// first j >= lo whose key prefix (key >> shift) differs from p, searching
+// forward from lo with doubling steps, then bisecting the last step; nodes on
+// one particle's path are nested, so each level starts from the end of the
+// level below it and the search stays short
+__device__ int gallop_end(const unsigned long long *__restrict__ key, int n, int lo, unsigned long long p, int shift)
 {
-    // first j > s whose key prefix (key >> shift) differs from key[s]'s
-    const unsigned long long p = (shift >= 64) ? 0ull : (key[s] >> shift);
-    int lo = s + 1, hi = n;
-    while (lo < hi) {
+    if (lo >= n || (key[lo] >> shift) != p) return lo;
+    int step = 1;
+    int hi = lo + 1;
+    while (hi < n && (key[hi] >> shift) == p) {
+        lo = hi;
+        step <<= 1;
+        hi = min(n, lo + step);
+    }
+    // key[lo] has the prefix; the first differing position is in (lo, hi]
+    while (hi - lo > 1) {
         const int mid = (lo + hi) >> 1;
-        const unsigned long long q = (shift >= 64) ? 0ull : (key[mid] >> shift);
-        if (q == p) lo = mid + 1;
+        if ((key[mid] >> shift) == p) lo = mid;
         else hi = mid;
     }
-    return lo;
+    return hi;
 }
 
 __global__ void k_fill(const unsigned long long *__restrict__ key, const unsigned char *__restrict__ kdig,
@@ -401,57 +409,76 @@ __global__ void k_fill(const unsigned long long *__restrict__ key, const unsigne
                        long long *__restrict__ stats)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= sh.n) return;
-    int idx = nbase[s];
-    const int cnt = nbase[s + 1] - idx;
-    if (cnt == 0) return;
-    const int L1 = lam[s];
-    const int ML = sh.ml;
-    if (s == 0) {
-        nstart[idx] = 0;
-        nend[idx] = sh.n;
-        nskip[idx] = total;
-        nlevel[idx] = 1;
-        for (int k = 0; k < 3; ++k) ncenter[(size_t)k * cap + idx] = root[k];
-        nedge[idx] = root[3];
-        ++idx;
-    }
-    const int lmin = max(2, (int)kdig[s] + 1);
-    const unsigned long long kk = key[s];
-    // replay the centre recursion down to level lmin
-    double c[3] = {root[0], root[1], root[2]};
-    double e = root[3];
-    for (int d = 1; d < lmin; ++d) {
-        const unsigned dig = (unsigned)(kk >> (sh.bits * (ML - d))) & ((1u << sh.bits) - 1u);
-        for (int k = 0; k < sh.dim; ++k) {
-            const double sg = ((dig >> k) & 1u) ? 1.0 : -1.0;
-            c[k] = c[k] + sg * e * 0.25;
+    int leaves = 0, depth = 0;
+    const int idx0 = (s < sh.n) ? nbase[s] : 0;
+    const int cnt = (s < sh.n) ? nbase[s + 1] - idx0 : 0;
+    if (cnt > 0) {
+        int idx = idx0;
+        const int L1 = lam[s];
+        const int ML = sh.ml;
+        depth = L1;
+        if (s == 0) {
+            nstart[idx] = 0;
+            nend[idx] = sh.n;
+            nskip[idx] = total;
+            nlevel[idx] = 1;
+            for (int k = 0; k < 3; ++k) ncenter[(size_t)k * cap + idx] = root[k];
+            nedge[idx] = root[3];
+            ++idx;
         }
-        e = e * 0.5;
-    }
-    int leaves = 0;
-    for (int L = lmin; L <= L1; ++L, ++idx) {
-        const bool leaf = (L == L1);
-        int end;
-        if (leaf && L1 <= ML) end = leaf_end[s];
-        else end = prefix_end(key, sh.n, s, sh.bits * (ML - (L - 1)));
-        nstart[idx] = s;
-        nend[idx] = end;
-        nskip[idx] = (end < sh.n) ? nbase[end] : total;
-        nlevel[idx] = leaf ? -L : L;
-        for (int k = 0; k < 3; ++k) ncenter[(size_t)k * cap + idx] = c[k];
-        nedge[idx] = e;
-        leaves += leaf ? 1 : 0;
-        // centre of the child on this particle's path
-        const unsigned dig = (unsigned)(kk >> (sh.bits * (ML - L))) & ((1u << sh.bits) - 1u);
-        for (int k = 0; k < sh.dim; ++k) {
-            const double sg = ((dig >> k) & 1u) ? 1.0 : -1.0;
-            c[k] = c[k] + sg * e * 0.25;
+        const int lmin = max(2, (int)kdig[s] + 1);
+        const unsigned long long kk = key[s];
+        // replay the centre recursion down to level lmin
+        double c[3] = {root[0], root[1], root[2]};
+        double e = root[3];
+        for (int d = 1; d < lmin; ++d) {
+            const unsigned dig = (unsigned)(kk >> (sh.bits * (ML - d))) & ((1u << sh.bits) - 1u);
+            for (int k = 0; k < sh.dim; ++k) {
+                const double sg = ((dig >> k) & 1u) ? 1.0 : -1.0;
+                c[k] = c[k] + sg * e * 0.25;
+            }
+            e = e * 0.5;
         }
-        e = e * 0.5;
+        // top-down: start, level and geometry of the nodes on this path
+        for (int L = lmin, q = idx; L <= L1; ++L, ++q) {
+            const bool leaf = (L == L1);
+            nstart[q] = s;
+            nlevel[q] = leaf ? -L : L;
+            for (int k = 0; k < 3; ++k) ncenter[(size_t)k * cap + q] = c[k];
+            nedge[q] = e;
+            leaves += leaf ? 1 : 0;
+            // centre of the child on this particle's path
+            const unsigned dig = (unsigned)(kk >> (sh.bits * (ML - L))) & ((1u << sh.bits) - 1u);
+            for (int k = 0; k < sh.dim; ++k) {
+                const double sg = ((dig >> k) & 1u) ? 1.0 : -1.0;
+                c[k] = c[k] + sg * e * 0.25;
+            }
+            e = e * 0.5;
+        }
+        // bottom-up: the node at level L holds the particles sharing key's
+        // first L-1 digits; its end is at or after the end of the level below
+        int end_below = s + 1;
+        for (int L = L1, q = idx + (L1 - lmin); L >= lmin; --L, --q) {
+            int end;
+            if (L == L1 && L1 <= ML) end = leaf_end[s];
+            else {
+                const int shift = sh.bits * (ML - (L - 1));
+                end = (shift >= 64) ? sh.n : gallop_end(key, sh.n, end_below, kk >> shift, shift);
+            }
+            nend[q] = end;
+            nskip[q] = (end < sh.n) ? nbase[end] : total;
+            end_below = end;
+        }
     }
-    if (leaves) atomicAdd(stat_ptr(stats, 11), (unsigned long long)leaves);
-    atomicMax(stat_ptr(stats, 12), (unsigned long long)L1);
+    // counters: one atomic per wavefront
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+        leaves += __shfl_xor(leaves, off, kWave);
+        depth = max(depth, __shfl_xor(depth, off, kWave));
+    }
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        if (leaves) atomicAdd(stat_ptr(stats, 11), (unsigned long long)leaves);
+        if (depth) atomicMax(stat_ptr(stats, 12), (unsigned long long)depth);
+    }
 }
 
 // ---- 7. block hierarchy and node reductions ------------------------------------------
@@ -669,7 +696,7 @@ int tree_alloc(sphgpu_ctx *c)
               al((void **)&t.root, 8ull * 8) && al((void **)&t.hist, 4ull * 256 * t.hist_blocks) &&
               al((void **)&t.nbox, 16ull * t.node_cap) && al((void **)&t.nlink, 16ull * t.node_cap) &&
               al((void **)&t.nhalf, 8ull * t.node_cap) && al((void **)&t.nchild, 8ull * t.node_cap) &&
-              al((void **)&t.child, 4ull * t.node_cap) && al((void **)&t.ccount, 4ull * t.node_cap);
+              al((void **)&t.child, 4ull * (t.node_cap + 8)) && al((void **)&t.ccount, 4ull * t.node_cap);
     const long long scan_max = std::max<long long>(std::max<long long>(n + 1, 256ll * t.hist_blocks), t.node_cap);
     ok = ok && al((void **)&t.scan_tmp, 4ull * ((scan_max + kScanTile - 1) / kScanTile + 1));
     if (!ok) return SPHGPU_E_NOMEM;
