@@ -211,6 +211,10 @@ def main() -> None:
         "roofline": rl,
     }
     st = res["stats"]
+    # tree searches per step: the pre-interaction's, plus the FluidForce one
+    # when its pair lists could not be derived from the pre-interaction's
+    searches = sum(res["times"].get(k, (0.0, 0))[1] for k in ("search_pre", "search_force")) / args.steps
+    per_wave = 64 / (max(searches, 1.0) * res["n"] * args.steps)
     out["counters"] = {
         "riemann_iters_per_call": round(st["riemann_iters"] / max(1, st["riemann_calls"]), 4),
         "riemann_wave_max_per_slot": round(st["riemann_wave_max_iters"] * 64 / max(1, st["riemann_calls"]), 4),
@@ -219,11 +223,14 @@ def main() -> None:
         "pre_candidates_per_particle": round(st["pre_candidates"] / (res["n"] * args.steps), 3),
         "h_newton_iters_per_particle": round(st["pre_newton_iters"] / (res["n"] * args.steps), 3),
         "tree_nodes": st["nodes"],
-        "search_node_visits_per_wave": round(st["search_node_visits"] * 64 / (2 * res["n"] * args.steps), 1),
-        "search_member_tests_per_wave": round(st["search_member_tests"] * 64 / (2 * res["n"] * args.steps), 1),
-        "search_windows_per_wave": round(st["search_windows"] * 64 / (2 * res["n"] * args.steps), 1),
-        "search_leaves_per_wave": round(st["search_leaves"] * 64 / (2 * res["n"] * args.steps), 1),
+        "search_node_visits_per_wave": round(st["search_node_visits"] * per_wave, 1),
+        "search_member_tests_per_wave": round(st["search_member_tests"] * per_wave, 1),
+        "search_windows_per_wave": round(st["search_windows"] * per_wave, 1),
+        "search_leaves_per_wave": round(st["search_leaves"] * per_wave, 1),
         "search_fp64_decisions_per_particle": round(st["search_exact"] / (res["n"] * args.steps), 4),
+        "searches_per_step": round(searches, 3),
+        "force_list_searches": st["flist_searches"],
+        "h_beyond_search_radius": st["flist_h_beyond"],
     }
     if train_info is not None:
         out["training"] = train_info

@@ -152,6 +152,12 @@ typedef struct sphgpu_stats {
     int64_t search_windows;      /* 64-node windows loaded by search wavefronts */
     int64_t search_leaves;       /* leaves whose members a wavefront tested   */
     int64_t search_exact;        /* candidates decided by the FP64 predicate  */
+    int64_t flist_h_beyond;      /* particles whose h reached the pre-interaction
+                                    search radius (force lists then repaired)  */
+    int64_t flist_other;         /* particles that made a derived force list
+                                    unusable (over 256 candidates, r = 0 pairs,
+                                    list capacity)                             */
+    int64_t flist_searches;      /* FluidForce calls that ran the is_ij search */
 } sphgpu_stats;
 
 /* ---- lifecycle ------------------------------------------------------------ */

@@ -454,6 +454,9 @@ int sphgpu_get_stats(sphgpu_ctx *c, sphgpu_stats *out)
     out->search_windows = h[15];
     out->search_leaves = h[16];
     out->search_exact = h[17];
+    out->flist_h_beyond = h[18];
+    out->flist_other = h[19];
+    out->flist_searches = c->flist_searches;
     return 0;
 }
 
@@ -462,6 +465,7 @@ int sphgpu_reset_stats(sphgpu_ctx *c)
     if (!c) return SPHGPU_E_ARG;
     HIPCHK(c, hipMemsetAsync(c->dstats, 0, sizeof(long long) * kStatSlots * kStatCopies, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->flist_searches = 0;
     return 0;
 }
 

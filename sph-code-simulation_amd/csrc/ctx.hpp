@@ -160,6 +160,7 @@ struct sphgpu_ctx {
     // force pair lists derived from the last pre-interaction (no is_ij
     // search): 0 none, 1 built but not yet checked, 2 valid
     int flist = 0;
+    long long flist_searches = 0;  // FluidForce calls that searched instead
 
     // timing
     bool timing = false;

@@ -53,8 +53,10 @@ template <typename T> struct PreArgs {
     const double *root;           // tree root geometry (bounds |x| for the FP32 prefilter)
     // force pair list, first part: membership masks over the first 256
     // candidates ([4][n], bit q = candidate q has r < h_i and r != 0) and
-    // counts, written when fl_mask is set
+    // counts, written when fl_mask is set; a particle with more than 256
+    // candidates writes its members into its list rows instead
     unsigned long long *fl_mask;
+    int *fl_idx;
     int *fl_cnt, *fl_cnt_a;
     int fl_cap;
     int *fl_flag;                 // set when the derived list cannot be used
@@ -174,11 +176,14 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
     int w = 0;
     // members other than coincident ones start the force pair list: with
     // h_i below the search radius they are exactly {j : r_ij < h_i, r_ij != 0}.
-    // They are passed on as the membership bits without the particle itself;
-    // a coincident pair j != i (r_ij = 0) or more than 256 candidates make
-    // the derived list unusable for this step (the force searches instead).
+    // They are passed on as the membership bits without the particle itself
+    // (or, past 256 candidates, written out as list rows); a coincident pair
+    // j != i (r_ij = 0) makes the derived list unusable for this step (the
+    // force searches instead).
     const bool fl = a.fl_mask != nullptr;
-    int qself = -1;
+    const bool fl_rows = fl && nc > 256;
+    int *const fcol = fl_rows ? a.fl_idx + list_slot(i, 0, a.fl_cap) : nullptr;
+    int qself = -1, nrow = 0;
     bool coincident = false;
     // FP32 r^2 at or below z2 may come from coincident positions (their
     // rounding moves each component by a few u X): (2^-18 X)^2 leaves room
@@ -207,6 +212,10 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
                 bool nz = r2 != T(0);
                 if (sizeof(T) == 4 && !((float)r2 > z2)) nz = r2_64(a.x64, xi64, j, a.box, D) != 0.0;
                 coincident = coincident || !nz;
+                if (fl_rows && nz) {
+                    if (nrow < a.fl_cap) fcol[nrow * kWave] = j;
+                    ++nrow;
+                }
             }
         }
         const T r = tsqrt(r2);
@@ -269,13 +278,17 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
                 a.fl_mask[(size_t)b * a.n + i] = am;
                 na += __popcll(am);
             }
+            if (fl_rows) na = nrow;
             a.fl_cnt_a[i] = min(na, a.fl_cap);
             a.fl_cnt[i] = min(na, a.fl_cap);
             // complete only when every j with r < h_i was a candidate
             // (r^2 < hs^2): h_i must stay below hs_i with room for rounding
-            if (nc > 256 || qself < 0 || coincident || na > a.fl_cap ||
-                !(h64 < hs64 * (1.0 - 9.094947017729282e-13)))
+            const bool beyond = !(h64 < hs64 * (1.0 - 9.094947017729282e-13));
+            const bool other = (!fl_rows && qself < 0) || coincident || na > a.fl_cap;
+            if (beyond || other) {
                 atomicOr(a.fl_flag, 1);
+                atomicAdd(stat_ptr(a.stats, beyond ? 18 : 19), 1ull);
+            }
         }
     }
     wave_min_atomic(a.hvs_min, valid ? (double)(h / vsig) : 1.7976931348623157e308);
@@ -429,6 +442,7 @@ struct FlistArgs {
     const int *cnt_a;
     int cap;
     const int *pidx;          // pre-interaction candidate lists
+    const int *pcnt;
     int pcap;
     const unsigned long long *mask;  // [4][n] member bits over the candidates
     const float4 *posf;       // FP32 (x, y, z, h)
@@ -453,19 +467,26 @@ __global__ __launch_bounds__(256) void k_flist_scatter(FlistArgs a)
     for (int k = 0; k < D; ++k) xi64[k] = a.x64[k][i];
     const float xf[3] = {pi.x, pi.y, pi.z};
     const float mk = fp32_margin_k(a.root, D);
+    // past 256 candidates the pre-interaction wrote the member rows itself
+    const bool rows = a.pcnt[i] > 256;
     int wd = 0;
-    unsigned long long m = a.mask[i];
+    unsigned long long m = rows ? 0ull : a.mask[i];
     for (int q = 0; q < na; ++q) {
-        // next member: lowest remaining bit of the candidate masks
-        while (m == 0ull && wd < 3) {
-            ++wd;
-            m = a.mask[(size_t)wd * a.n + i];
+        int j;
+        if (rows) {
+            j = a.idx[list_slot(i, q, a.cap)];
+        } else {
+            // next member: lowest remaining bit of the candidate masks
+            while (m == 0ull && wd < 3) {
+                ++wd;
+                m = a.mask[(size_t)wd * a.n + i];
+            }
+            if (m == 0ull) break;
+            const int qc = (wd << 6) + __ffsll((long long)m) - 1;
+            m &= m - 1ull;
+            j = a.pidx[list_slot(i, qc, a.pcap)];
+            a.idx[list_slot(i, q, a.cap)] = j;
         }
-        if (m == 0ull) break;
-        const int qc = (wd << 6) + __ffsll((long long)m) - 1;
-        m &= m - 1ull;
-        const int j = a.pidx[list_slot(i, qc, a.pcap)];
-        a.idx[list_slot(i, q, a.cap)] = j;
         const float4 pj = a.posf[j];
         // FP32 rounding is monotone: h_j > h_i when the rounded values say so,
         // and then r < h_i <= h_j puts i in the list of j already
@@ -484,7 +505,10 @@ __global__ __launch_bounds__(256) void k_flist_scatter(FlistArgs a)
         if (in_j) continue;
         const int s = atomicAdd(&a.cnt[j], 1);
         if (s < a.cap) a.idx[list_slot(j, s, a.cap)] = i;
-        else atomicOr(a.flag, 1);
+        else {
+            atomicOr(a.flag, 1);
+            atomicAdd(stat_ptr(a.stats, 19), 1ull);
+        }
     }
 }
 
@@ -1056,6 +1080,7 @@ template <int D, int K, int TYPE, typename T> void launch_pre(sphgpu_ctx *c)
     // r < h need not imply r^2 < hs^2 at the last bit: search instead)
     const bool fl = c->prm.iterative_sml != 0;
     a.fl_mask = fl ? c->amask : nullptr;
+    a.fl_idx = c->lforce.idx;
     a.fl_cnt = c->lforce.cnt;
     a.fl_cnt_a = c->lforce.cnt_a;
     a.fl_cap = c->lforce.cap;
@@ -1072,6 +1097,7 @@ template <int D> void launch_flist(sphgpu_ctx *c)
     a.cnt_a = c->lforce.cnt_a;
     a.cap = c->lforce.cap;
     a.pidx = c->lpre.idx;
+    a.pcnt = c->lpre.cnt;
     a.pcap = c->lpre.cap;
     a.mask = c->amask;
     a.posf = c->rec32[R_POSH];
@@ -1274,7 +1300,10 @@ int fluid_force(sphgpu_ctx *c)
     // the is_ij tree search otherwise
     const int fc = flist_check(c);
     if (fc < 0) return fc;
-    if (fc && (rc = search(c, c->lforce, 1, c->sc[S_SML]))) return rc;
+    if (fc) {
+        ++c->flist_searches;
+        if ((rc = search(c, c->lforce, 1, c->sc[S_SML]))) return rc;
+    }
     time_begin(c, "fluid_force");
     if (c->prm.pair_precision == SPHGPU_PAIR_FP64) dispatch_force<double>(c);
     else dispatch_force<float>(c);

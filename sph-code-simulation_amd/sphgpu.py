@@ -45,7 +45,8 @@ class Stats(C.Structure):
         "gravity_interactions", "nodes", "leaves", "search_node_visits",
         "search_member_tests")] + [
         ("max_depth", C.c_int32), ("pad", C.c_int32),
-        ("search_windows", C.c_int64), ("search_leaves", C.c_int64), ("search_exact", C.c_int64)]
+        ("search_windows", C.c_int64), ("search_leaves", C.c_int64), ("search_exact", C.c_int64),
+        ("flist_h_beyond", C.c_int64), ("flist_other", C.c_int64), ("flist_searches", C.c_int64)]
 
     def as_dict(self) -> dict:
         return {n: getattr(self, n) for n, _ in self._fields_ if n != "pad"}

@@ -164,6 +164,39 @@ def test_force_lists_from_pre_exact(case, prec):
           "all sets identical to brute force and to the is_ij search")
 
 
+@pytest.mark.parametrize("case", ["khi_2d", "sedov_3d"])
+def test_force_lists_many_candidates(case):
+    """Particles whose previous density is far too low get search radii with
+    hundreds of candidates (beyond the 256 the member masks cover); their
+    member rows are written out directly.  The derived force pair sets must
+    still be exactly the FluidForce sets."""
+    p, f = SMALL[case]()
+    g = gpu_ctx(p, f, pair_precision=ics.PAIR_FP32)
+    g.initialize()
+    g.integrate(2)
+    st = g.download()
+    low = np.arange(0, st.shape[1], 97)
+    st[ics.F_DENS, low] *= 0.02  # h guess (nn m / (rho A))^(1/D) grows by 0.02^(-1/D)
+    g.upload(st)
+    g.build_tree()
+    g.reset_stats()
+    g.pre_interaction()
+    g.fluid_force()
+    stats = g.stats()
+    counts, lists = g.neighbor_sets(None, 2, cap=1024)
+    gf = g.download()
+    n = gf.shape[1]
+    allj = np.arange(n)
+    for i in range(n):
+        ref = force_pairs(p, gf, i, allj)
+        assert counts[i] == len(ref), (case, i, counts[i], len(ref))
+        np.testing.assert_array_equal(np.sort(lists[i, :counts[i]]), np.sort(ref))
+    pc = stats["pre_candidates"]
+    print(f"{case}: {len(low)} particles with a low density guess, {pc / n:.1f} candidates per particle, "
+          f"force searches {stats['flist_searches']}; all {n} sets identical to brute force")
+    assert stats["flist_searches"] == 0
+
+
 def force_errors(gf, of, d):
     """Acceleration and du/dt errors per particle, normalised by the size of
     one pair's pressure term, P/(rho h) (and P c/(rho h) for du/dt), plus the
