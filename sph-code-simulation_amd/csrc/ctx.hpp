@@ -152,7 +152,6 @@ struct sphgpu_ctx {
 
     sphgpu::Tree tree;
     sphgpu::Lists lpre, lforce;
-    unsigned long long *amask = nullptr;  // [4][n] pre-interaction member bits
     bool sorted_valid = false;   // state is in current tree order
     bool mirror_dirty = true;    // pair records lag the FP64 masters
     bool pre_first = true;       // PreInteraction::m_first

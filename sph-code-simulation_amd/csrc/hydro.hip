@@ -51,11 +51,8 @@ template <typename T> struct PreArgs {
     int nn, iterative, balsara_on, td_av, gsph2;
     double gamma, kappa, alpha_min, alpha_max, eps_av;
     const double *root;           // tree root geometry (bounds |x| for the FP32 prefilter)
-    // force pair list, first part: membership masks over the first 256
-    // candidates ([4][n], bit q = candidate q has r < h_i and r != 0) and
-    // counts, written when fl_mask is set; a particle with more than 256
-    // candidates writes its members into its list rows instead
-    unsigned long long *fl_mask;
+    // force pair list, first part (members with r != 0), written when
+    // fl_idx is set
     int *fl_idx;
     int *fl_cnt, *fl_cnt_a;
     int fl_cap;
@@ -176,11 +173,11 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
     int w = 0;
     // members other than coincident ones start the force pair list: with
     // h_i below the search radius they are exactly {j : r_ij < h_i, r_ij != 0}.
-    // They are passed on as the membership bits without the particle itself
-    // (or, past 256 candidates, written out as list rows); a coincident pair
-    // j != i (r_ij = 0) makes the derived list unusable for this step (the
-    // force searches instead).
-    const bool fl = a.fl_mask != nullptr;
+    // They are written to the force list after the member pass, one row per
+    // member rank, from the membership bits (past 256 candidates, during the
+    // pass); a coincident pair j != i (r_ij = 0) makes the derived list
+    // unusable for this step (the force searches instead).
+    const bool fl = a.fl_idx != nullptr;
     const bool fl_rows = fl && nc > 256;
     int *const fcol = fl_rows ? a.fl_idx + list_slot(i, 0, a.fl_cap) : nullptr;
     int qself = -1, nrow = 0;
@@ -271,14 +268,34 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
         if (TYPE != SPHGPU_GSPH) a.rec_w[R_AUX][4 * (size_t)i + 0] = gradh;
         if (a.posf_w) a.posf_w[4 * (size_t)i + 3] = (float)h;
         if (fl) {
+            unsigned long long am[4];
             int na = 0;
 #pragma unroll
             for (int b = 0; b < 4; ++b) {
-                const unsigned long long am = mb[b] & ~(((qself >> 6) == b) ? 1ull << (qself & 63) : 0ull);
-                a.fl_mask[(size_t)b * a.n + i] = am;
-                na += __popcll(am);
+                am[b] = mb[b] & ~(((qself >> 6) == b) ? 1ull << (qself & 63) : 0ull);
+                na += __popcll(am[b]);
             }
-            if (fl_rows) na = nrow;
+            if (fl_rows) {
+                na = nrow;
+            } else {
+                // row r of the force list = the r-th member: every lane of the
+                // wavefront stores into the same 256-byte row at once, and the
+                // candidate rows it reads were just loaded by the member pass
+                int *const col = a.fl_idx + list_slot(i, 0, a.fl_cap);
+                const int *const pcol = a.idx + list_slot(i, 0, a.cap);
+                const int nw = min(na, a.fl_cap);
+                int wd = 0;
+                unsigned long long m = am[0];
+                for (int r = 0; r < nw; ++r) {
+                    while (m == 0ull && wd < 3) {
+                        ++wd;
+                        m = (wd == 1) ? am[1] : (wd == 2) ? am[2] : am[3];
+                    }
+                    const int qc = (wd << 6) + __ffsll((long long)m) - 1;
+                    m &= m - 1ull;
+                    col[r * kWave] = pcol[qc * kWave];
+                }
+            }
             a.fl_cnt_a[i] = min(na, a.fl_cap);
             a.fl_cnt[i] = min(na, a.fl_cap);
             // complete only when every j with r < h_i was a candidate
@@ -428,23 +445,18 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
 // ---- force pair lists from the pre-interaction lists ----------------------------
 // The FluidForce pair set of i is {j : r_ij < max(h_i, h_j), r_ij != 0}
 // (src/fluid_force.cpp:43-62, src/gsph/g_fluid_force.cpp:93): the members
-// with r < h_i, which k_pre marked, plus every j with h_j > r_ij >= h_i.  The
-// latter pairs are exactly the transposes of k_pre's members (i in the list
+// with r < h_i, which k_pre wrote, plus every j with h_j > r_ij >= h_i.  The
+// latter pairs are exactly the transposes of k_pre's entries (i in the list
 // of j) that fail r < h_i; since the FP64 r_ij^2 is symmetric in i and j,
-// the same predicate decides them.  This pass writes each particle's members
-// (one list row per member rank, so a wavefront's stores are contiguous) and
-// appends the transposed pairs to the other particle's list; the next one
-// puts each appended run in index order so the lists do not depend on the
-// order of the atomics.
+// the same predicate decides them.  This pass appends them to the other
+// particle's list; the next one puts each appended run in index order so the
+// lists do not depend on the order of the atomics.
 struct FlistArgs {
     int n;
     int *idx, *cnt;
     const int *cnt_a;
     int cap;
-    const int *pidx;          // pre-interaction candidate lists
-    const int *pcnt;
-    int pcap;
-    const unsigned long long *mask;  // [4][n] member bits over the candidates
+
     const float4 *posf;       // FP32 (x, y, z, h)
     const double *x64[3];
     const double *sml;
@@ -467,26 +479,8 @@ __global__ __launch_bounds__(256) void k_flist_scatter(FlistArgs a)
     for (int k = 0; k < D; ++k) xi64[k] = a.x64[k][i];
     const float xf[3] = {pi.x, pi.y, pi.z};
     const float mk = fp32_margin_k(a.root, D);
-    // past 256 candidates the pre-interaction wrote the member rows itself
-    const bool rows = a.pcnt[i] > 256;
-    int wd = 0;
-    unsigned long long m = rows ? 0ull : a.mask[i];
     for (int q = 0; q < na; ++q) {
-        int j;
-        if (rows) {
-            j = a.idx[list_slot(i, q, a.cap)];
-        } else {
-            // next member: lowest remaining bit of the candidate masks
-            while (m == 0ull && wd < 3) {
-                ++wd;
-                m = a.mask[(size_t)wd * a.n + i];
-            }
-            if (m == 0ull) break;
-            const int qc = (wd << 6) + __ffsll((long long)m) - 1;
-            m &= m - 1ull;
-            j = a.pidx[list_slot(i, qc, a.pcap)];
-            a.idx[list_slot(i, q, a.cap)] = j;
-        }
+        const int j = a.idx[list_slot(i, q, a.cap)];
         const float4 pj = a.posf[j];
         // FP32 rounding is monotone: h_j > h_i when the rounded values say so,
         // and then r < h_i <= h_j puts i in the list of j already
@@ -1079,8 +1073,7 @@ template <int D, int K, int TYPE, typename T> void launch_pre(sphgpu_ctx *c)
     // force pair lists follow from these lists (non-iterative: h = hs, and
     // r < h need not imply r^2 < hs^2 at the last bit: search instead)
     const bool fl = c->prm.iterative_sml != 0;
-    a.fl_mask = fl ? c->amask : nullptr;
-    a.fl_idx = c->lforce.idx;
+    a.fl_idx = fl ? c->lforce.idx : nullptr;
     a.fl_cnt = c->lforce.cnt;
     a.fl_cnt_a = c->lforce.cnt_a;
     a.fl_cap = c->lforce.cap;
@@ -1096,10 +1089,7 @@ template <int D> void launch_flist(sphgpu_ctx *c)
     a.cnt = c->lforce.cnt;
     a.cnt_a = c->lforce.cnt_a;
     a.cap = c->lforce.cap;
-    a.pidx = c->lpre.idx;
-    a.pcnt = c->lpre.cnt;
-    a.pcap = c->lpre.cap;
-    a.mask = c->amask;
+
     a.posf = c->rec32[R_POSH];
     for (int k = 0; k < 3; ++k) a.x64[k] = c->pos[k < c->dim ? k : 0];
     a.sml = c->sc[S_SML];
