@@ -267,19 +267,29 @@ __global__ __launch_bounds__(256, 4) void k_search(SearchArgs a)
                 if (exact(sidx[t])) km |= 1ull << t;
             }
         }
-        // append the kept candidates in candidate order; one store
-        // instruction per rank instead of one per kept candidate
-        int nk = __popcll(km);
+        // append the kept candidates in candidate order, four ranks per
+        // round: the four LDS reads are issued together, so a round waits
+        // once for LDS instead of once per kept candidate
+        const int nkl = __popcll(km);
+        int nk = nkl;
         for (int off = kWave / 2; off > 0; off >>= 1) nk = max(nk, __shfl_xor(nk, off, kWave));
-        for (int r = 0; r < nk; ++r) {
-            const int t = km ? __ffsll((long long)km) - 1 : 0;
-            const int j = sidx[t];
-            if (km) {
-                km &= km - 1;
-                if (count < a.cap) col[count * kWave] = j;
-                else over = true;
-                ++count;
+        if (count + nkl > a.cap) over = true;
+        for (int r = 0; r < nk; r += 4) {
+            int t[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                t[u] = km ? __ffsll((long long)km) - 1 : 0;
+                km &= km - 1;  // no-op once empty
             }
+            int j[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) j[u] = sidx[t[u]];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int q = count + u;
+                if (r + u < nkl && q < a.cap) col[q * kWave] = j[u];
+            }
+            count += max(0, min(4, nkl - r));
         }
         wave_lds_fence();
         nb = 0;
