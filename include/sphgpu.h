@@ -165,6 +165,8 @@ void        sphgpu_params_default(sphgpu_params *p, int dim);
 int         sphgpu_create(const sphgpu_params *p, int n, sphgpu_ctx **out);
 int         sphgpu_destroy(sphgpu_ctx *ctx);
 const char *sphgpu_last_error(const sphgpu_ctx *ctx);
+/* the parameters the context was created with and its particle count */
+int         sphgpu_get_params(const sphgpu_ctx *ctx, sphgpu_params *p, int32_t *n);
 int         sphgpu_device_info(char *name, int name_len, int *cu_count, double *hbm_gb);
 
 /* ---- state exchange (fields: [SPHGPU_NFIELDS][n] FP64, index = id) ------- */
@@ -268,6 +270,72 @@ int sphgpu_enable_timing(sphgpu_ctx *ctx, int on);
  * Its HBM counter readings calibrate the FETCH_SIZE / WRITE_SIZE counters,
  * and nbytes / ms is the attainable copy bandwidth. */
 int sphgpu_copy_probe(sphgpu_ctx *ctx, int64_t nbytes, double *ms);
+
+/* ---- snapshot and energy files (the reference's on-disk layout) ----------- */
+/* OutputMetadata (include/output_metadata.hpp; src/output_metadata.cpp:8-37)
+ * as the CSV header prints it.  Strings are NUL-terminated; the unit fields
+ * carry UnitSystem's names and CGS factors (src/units.cpp). */
+typedef struct sphgpu_output_meta {
+    int32_t format_version;          /* 1                                     */
+    int32_t step;                    /* snapshot count                        */
+    char    timestamp[32];           /* "%Y-%m-%dT%H:%M:%SZ" (UTC)            */
+    char    simulation_name[64];     /* "SPH Simulation"                      */
+    char    unit_type[16];           /* "Code", "Galactic", "SI", "CGS"       */
+    char    length_unit[32], mass_unit[32], time_unit[32], velocity_unit[32], energy_unit[32];
+    double  length_to_cgs, mass_to_cgs, time_to_cgs, velocity_to_cgs;
+    double  time_code, time_physical;
+    int32_t particle_count;
+    int32_t neighbor_number;
+    int32_t sph_type;                /* SPHGPU_SSPH / _DISPH / _GSPH          */
+    int32_t kernel;                  /* SPHGPU_CUBIC_SPLINE / _WENDLAND       */
+    int32_t use_gravity, use_balsara, use_time_dependent_av;
+    int32_t is_relaxation, relaxation_step, relaxation_total_steps;
+    double  gamma, gravitational_constant;
+    double  accumulated_time, alpha_scaling, rho_center, K, R, M_total;
+    double  kinetic_energy, thermal_energy, potential_energy, total_energy;
+    int32_t precision;               /* significant digits after the point (16) */
+    int32_t pad;
+} sphgpu_output_meta;
+
+/* OutputManager::build_metadata (src/output_manager.cpp:254-300): code units,
+ * the context's parameters and time, the current UTC time, and the energies
+ * of the device state (OutputManager::compute_energies, :302-327). */
+int sphgpu_output_meta_init(sphgpu_ctx *ctx, int32_t step, sphgpu_output_meta *meta);
+/* OutputManager::generate_filename (src/output_manager.cpp:329-338):
+ * "<dir>/<prefix>_%04d<ext>" into buf (returns SPHGPU_E_ARG if it does not fit). */
+int sphgpu_snapshot_filename(const char *dir, const char *prefix, int32_t count, const char *ext,
+                             char *buf, int32_t buf_len);
+/* CSVWriter::open + write_particles (src/writers/csv_writer.cpp:17-165): the
+ * 50-line header (59 with the relaxation block), the column line and one row
+ * per particle in id order.  Axes beyond the context's dimension are written
+ * as 0 (the reference reads past its vec_t there). */
+int sphgpu_write_snapshot_csv(sphgpu_ctx *ctx, const char *path, const sphgpu_output_meta *meta);
+/* The same file from a host field block ([SPHGPU_NFIELDS][n], index = id),
+ * for callers that hold the state on the host (no context needed). */
+int sphgpu_write_csv_fields(const char *path, const sphgpu_output_meta *meta, int32_t dim, int32_t n,
+                            const double *fields);
+/* CSVWriter::read_metadata + read_particles (src/writers/csv_writer.cpp:174-371)
+ * and OutputManager::load_for_resume (src/output_manager.cpp:163-233): the
+ * header's metadata into meta (may be NULL), the 21 columns of every row into
+ * the context (ids must be 0..n-1 for the context's n; fields the file does
+ * not hold are set to 0), and the context time to "Time (code)".  A file
+ * that the reference would reject (fewer than 51 lines, a row without 21
+ * columns, a field that does not parse, no particle) returns SPHGPU_E_ARG. */
+int sphgpu_read_snapshot_csv(sphgpu_ctx *ctx, const char *path, sphgpu_output_meta *meta);
+/* The same parse into a host field block ([SPHGPU_NFIELDS][n], index = id;
+ * rows must number n), and the row count of a snapshot file. */
+int sphgpu_read_csv_fields(const char *path, sphgpu_output_meta *meta, int32_t dim, int32_t n, double *fields);
+int sphgpu_csv_row_count(const char *path, int32_t *rows);
+/* OutputManager::initialize, energy part (src/output_manager.cpp:55-73):
+ * creates/truncates path and writes the 6 comment lines. */
+int sphgpu_energy_file_begin(const char *path, const sphgpu_output_meta *meta);
+/* OutputManager::write_energy (src/output_manager.cpp:235-252): appends
+ * "time  kinetic  thermal  potential  total" (total = the sum of the three). */
+int sphgpu_energy_file_write(const char *path, double time, double kinetic, double thermal, double potential,
+                             int32_t precision);
+/* The same row for the device state: the context's time and the energies of
+ * sphgpu_energy_momentum. */
+int sphgpu_energy_file_append(sphgpu_ctx *ctx, const char *path, int32_t precision);
 
 #ifdef __cplusplus
 }

@@ -300,6 +300,14 @@ int sphgpu_destroy(sphgpu_ctx *c)
 
 const char *sphgpu_last_error(const sphgpu_ctx *c) { return c ? c->err.c_str() : "null context"; }
 
+int sphgpu_get_params(const sphgpu_ctx *c, sphgpu_params *p, int32_t *n)
+{
+    if (!c) return SPHGPU_E_ARG;
+    if (p) *p = c->prm;
+    if (n) *n = c->n;
+    return 0;
+}
+
 int sphgpu_upload(sphgpu_ctx *c, const double *fields, uint64_t mask)
 {
     if (!c || !fields) return SPHGPU_E_ARG;

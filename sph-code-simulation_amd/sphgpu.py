@@ -52,6 +52,95 @@ class Stats(C.Structure):
         return {n: getattr(self, n) for n, _ in self._fields_ if n != "pad"}
 
 
+class OutputMeta(C.Structure):
+    """sphgpu_output_meta (include/sphgpu.h): the CSV header's metadata."""
+    _fields_ = [
+        ("format_version", C.c_int32), ("step", C.c_int32),
+        ("timestamp", C.c_char * 32), ("simulation_name", C.c_char * 64), ("unit_type", C.c_char * 16),
+        ("length_unit", C.c_char * 32), ("mass_unit", C.c_char * 32), ("time_unit", C.c_char * 32),
+        ("velocity_unit", C.c_char * 32), ("energy_unit", C.c_char * 32),
+        ("length_to_cgs", C.c_double), ("mass_to_cgs", C.c_double), ("time_to_cgs", C.c_double),
+        ("velocity_to_cgs", C.c_double), ("time_code", C.c_double), ("time_physical", C.c_double),
+        ("particle_count", C.c_int32), ("neighbor_number", C.c_int32), ("sph_type", C.c_int32),
+        ("kernel", C.c_int32), ("use_gravity", C.c_int32), ("use_balsara", C.c_int32),
+        ("use_time_dependent_av", C.c_int32), ("is_relaxation", C.c_int32), ("relaxation_step", C.c_int32),
+        ("relaxation_total_steps", C.c_int32),
+        ("gamma", C.c_double), ("gravitational_constant", C.c_double),
+        ("accumulated_time", C.c_double), ("alpha_scaling", C.c_double), ("rho_center", C.c_double),
+        ("K", C.c_double), ("R", C.c_double), ("M_total", C.c_double),
+        ("kinetic_energy", C.c_double), ("thermal_energy", C.c_double), ("potential_energy", C.c_double),
+        ("total_energy", C.c_double), ("precision", C.c_int32), ("pad", C.c_int32)]
+
+    def as_dict(self) -> dict:
+        out = {}
+        for k, _ in self._fields_:
+            v = getattr(self, k)
+            out[k] = v.decode() if isinstance(v, bytes) else v
+        return out
+
+
+def output_meta(**kw) -> OutputMeta:
+    """An OutputMeta with the reference's defaults (OutputMetadata(), code
+    units, precision 16), overridden by kw."""
+    m = OutputMeta()
+    d = dict(format_version=1, timestamp="", simulation_name="SPH Simulation", unit_type="Code",
+             length_unit="code_length", mass_unit="code_mass", time_unit="code_time",
+             velocity_unit="code_velocity", energy_unit="code_energy", length_to_cgs=1.0, mass_to_cgs=1.0,
+             time_to_cgs=1.0, velocity_to_cgs=1.0, gamma=1.4, gravitational_constant=1.0, neighbor_number=32,
+             use_balsara=1, alpha_scaling=1.0, rho_center=1.0, K=1.0, R=1.0, M_total=1.0, precision=16)
+    d.update(kw)
+    for k, v in d.items():
+        setattr(m, k, v.encode() if isinstance(v, str) else v)
+    return m
+
+
+def write_csv_fields(path, meta: OutputMeta, dim: int, fields: np.ndarray):
+    """sphgpu_write_csv_fields: a snapshot CSV from a host [NFIELDS][n] block."""
+    f = np.ascontiguousarray(fields, dtype=np.float64)
+    rc = load_library().sphgpu_write_csv_fields(str(path).encode(), C.byref(meta), int(dim), f.shape[1], _dp(f))
+    if rc:
+        raise SphGpuError(f"sphgpu_write_csv_fields failed ({rc})")
+
+
+def csv_row_count(path) -> int:
+    r = C.c_int32(0)
+    rc = load_library().sphgpu_csv_row_count(str(path).encode(), C.byref(r))
+    if rc:
+        raise SphGpuError(f"sphgpu_csv_row_count failed ({rc})")
+    return r.value
+
+
+def read_csv_fields(path, dim: int, n: int):
+    """sphgpu_read_csv_fields -> (OutputMeta, [NFIELDS][n] block)."""
+    f = np.zeros((NFIELDS, n), dtype=np.float64)
+    m = OutputMeta()
+    rc = load_library().sphgpu_read_csv_fields(str(path).encode(), C.byref(m), int(dim), int(n), _dp(f))
+    if rc:
+        raise SphGpuError(f"sphgpu_read_csv_fields failed ({rc})")
+    return m, f
+
+
+def energy_file_begin(path, meta: OutputMeta):
+    rc = load_library().sphgpu_energy_file_begin(str(path).encode(), C.byref(meta))
+    if rc:
+        raise SphGpuError(f"sphgpu_energy_file_begin failed ({rc})")
+
+
+def energy_file_write(path, time: float, kinetic: float, thermal: float, potential: float, precision: int = 16):
+    rc = load_library().sphgpu_energy_file_write(str(path).encode(), time, kinetic, thermal, potential, precision)
+    if rc:
+        raise SphGpuError(f"sphgpu_energy_file_write failed ({rc})")
+
+
+def snapshot_filename(directory, prefix: str, count: int, ext: str) -> str:
+    buf = C.create_string_buffer(4096)
+    rc = load_library().sphgpu_snapshot_filename(str(directory).encode(), prefix.encode(), count, ext.encode(),
+                                                 buf, 4096)
+    if rc:
+        raise SphGpuError(f"sphgpu_snapshot_filename failed ({rc})")
+    return buf.value.decode()
+
+
 def params_struct(d: dict) -> Params:
     p = Params()
     for name, _ in Params._fields_:
@@ -122,6 +211,17 @@ def load_library() -> C.CDLL:
         "sphgpu_kernel_times": (ip, [vp, C.c_char_p, ip, dp, i64p, ip, C.POINTER(C.c_int)]),
         "sphgpu_copy_probe": (ip, [vp, C.c_int64, dp]),
         "sphgpu_enable_timing": (ip, [vp, ip]),
+        "sphgpu_get_params": (ip, [vp, C.POINTER(Params), i32p]),
+        "sphgpu_output_meta_init": (ip, [vp, C.c_int32, C.POINTER(OutputMeta)]),
+        "sphgpu_snapshot_filename": (ip, [C.c_char_p, C.c_char_p, C.c_int32, C.c_char_p, C.c_char_p, C.c_int32]),
+        "sphgpu_write_snapshot_csv": (ip, [vp, C.c_char_p, C.POINTER(OutputMeta)]),
+        "sphgpu_write_csv_fields": (ip, [C.c_char_p, C.POINTER(OutputMeta), C.c_int32, C.c_int32, dp]),
+        "sphgpu_read_snapshot_csv": (ip, [vp, C.c_char_p, C.POINTER(OutputMeta)]),
+        "sphgpu_read_csv_fields": (ip, [C.c_char_p, C.POINTER(OutputMeta), C.c_int32, C.c_int32, dp]),
+        "sphgpu_csv_row_count": (ip, [C.c_char_p, i32p]),
+        "sphgpu_energy_file_begin": (ip, [C.c_char_p, C.POINTER(OutputMeta)]),
+        "sphgpu_energy_file_write": (ip, [C.c_char_p, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int32]),
+        "sphgpu_energy_file_append": (ip, [vp, C.c_char_p, C.c_int32]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name)
@@ -144,7 +244,10 @@ EXPORTED = (
     "sphgpu_riemann_log_read", "sphgpu_mlp_init", "sphgpu_mlp_set_params",
     "sphgpu_mlp_get_params", "sphgpu_mlp_param_count", "sphgpu_mlp_train",
     "sphgpu_mlp_loss_grad", "sphgpu_mlp_predict", "sphgpu_kernel_times", "sphgpu_copy_probe",
-    "sphgpu_enable_timing",
+    "sphgpu_enable_timing", "sphgpu_get_params", "sphgpu_output_meta_init", "sphgpu_snapshot_filename",
+    "sphgpu_write_snapshot_csv", "sphgpu_write_csv_fields", "sphgpu_read_snapshot_csv",
+    "sphgpu_read_csv_fields", "sphgpu_csv_row_count", "sphgpu_energy_file_begin",
+    "sphgpu_energy_file_write", "sphgpu_energy_file_append",
 )
 
 
@@ -348,6 +451,24 @@ class Context:
         ms = C.c_double(0)
         self._check(self.lib.sphgpu_copy_probe(self.h, int(nbytes), C.byref(ms)), "copy_probe")
         return ms.value
+
+    def output_meta(self, step: int) -> OutputMeta:
+        m = OutputMeta()
+        self._check(self.lib.sphgpu_output_meta_init(self.h, int(step), C.byref(m)), "output_meta_init")
+        return m
+
+    def write_snapshot_csv(self, path, meta: OutputMeta):
+        self._check(self.lib.sphgpu_write_snapshot_csv(self.h, str(path).encode(), C.byref(meta)),
+                    "write_snapshot_csv")
+
+    def read_snapshot_csv(self, path) -> OutputMeta:
+        m = OutputMeta()
+        self._check(self.lib.sphgpu_read_snapshot_csv(self.h, str(path).encode(), C.byref(m)), "read_snapshot_csv")
+        return m
+
+    def energy_file_append(self, path, precision: int = 16):
+        self._check(self.lib.sphgpu_energy_file_append(self.h, str(path).encode(), int(precision)),
+                    "energy_file_append")
 
     def kernel_times(self) -> dict:
         buf = C.create_string_buffer(8192)
