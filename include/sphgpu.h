@@ -44,7 +44,9 @@ extern "C" {
 #endif
 
 /* ---- enums mirroring include/parameters.hpp:8-27 ------------------------ */
-enum { SPHGPU_SSPH = 0, SPHGPU_DISPH = 1, SPHGPU_GSPH = 2 };
+/* SPHType (include/parameters.hpp:9-15); GDISPH = DISPH pre-interaction and
+ * pressure-energy force with the GSPH MUSCL/HLL p* (src/gdisph/) */
+enum { SPHGPU_SSPH = 0, SPHGPU_DISPH = 1, SPHGPU_GSPH = 2, SPHGPU_GDISPH = 3 };
 enum { SPHGPU_CUBIC_SPLINE = 0, SPHGPU_WENDLAND = 1, SPHGPU_GAUSSIAN = 2 };
 enum { SPHGPU_RIEMANN_HLL = 0, SPHGPU_RIEMANN_ITERATIVE = 1 };
 /* pair-loop arithmetic: FP32 (compensated sums) or FP64 (reference-equivalent) */

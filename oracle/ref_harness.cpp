@@ -3,7 +3,8 @@
  *
  * TEST INFRASTRUCTURE ONLY.  Built by oracle/Makefile into oracle/_ref/ from
  * the reference sources, unmodified (src/{bhtree,pre_interaction,fluid_force,
- * gravity_force,timestep,simulation}.cpp, src/disph/*.cpp, src/gsph/*.cpp),
+ * gravity_force,timestep,simulation}.cpp, src/disph/*.cpp, src/gsph/*.cpp,
+ * src/gdisph/*.cpp),
  * once per DIM.  It exposes the same entry points as oracle/sph_oracle.cpp with
  * the prefix ref_ so tests can compare the restatement with the reference.
  *
@@ -56,6 +57,8 @@
 #include "disph/d_fluid_force.hpp"
 #include "gsph/g_pre_interaction.hpp"
 #include "gsph/g_fluid_force.hpp"
+#include "gdisph/gd_pre_interaction.hpp"
+#include "gdisph/gd_fluid_force.hpp"
 #undef private
 #undef protected
 #undef class
@@ -118,7 +121,8 @@ void *ref_create(const sphgpu_params *prm, int n)
     p->time.output = 1.0;
     p->time.energy = 1.0;
     p->type = prm->sph_type == SPHGPU_SSPH ? sph::SPHType::SSPH
-            : prm->sph_type == SPHGPU_DISPH ? sph::SPHType::DISPH : sph::SPHType::GSPH;
+            : prm->sph_type == SPHGPU_DISPH ? sph::SPHType::DISPH
+            : prm->sph_type == SPHGPU_GDISPH ? sph::SPHType::GDISPH : sph::SPHType::GSPH;
     p->cfl.sound = prm->cfl_sound;
     p->cfl.force = prm->cfl_force;
     p->av.alpha = prm->av_alpha;
@@ -159,7 +163,7 @@ void *ref_create(const sphgpu_params *prm, int n)
     r->ts = std::make_shared<sph::gpu::TimeStep>();
     r->pre = std::make_shared<sph::gpu::PreInteraction>();
     r->force = std::make_shared<sph::gpu::FluidForce>();
-    if (p->type == sph::SPHType::GSPH) {
+    if (p->type == sph::SPHType::GSPH || p->type == sph::SPHType::GDISPH) {
 #else
     r->ts = std::make_shared<sph::TimeStep>();
     if (p->type == sph::SPHType::SSPH) {
@@ -168,6 +172,17 @@ void *ref_create(const sphgpu_params *prm, int n)
     } else if (p->type == sph::SPHType::DISPH) {
         r->pre = std::make_shared<sph::disph::PreInteraction>();
         r->force = std::make_shared<sph::disph::FluidForce>();
+    } else if (p->type == sph::SPHType::GDISPH) {
+        r->pre = std::make_shared<sph::gdisph::PreInteraction>();
+        r->force = std::make_shared<sph::gdisph::FluidForce>();
+        std::vector<std::string> names = {"grad_density", "grad_pressure", "grad_velocity_0"};
+#if DIM >= 2
+        names.push_back("grad_velocity_1");
+#endif
+#if DIM == 3
+        names.push_back("grad_velocity_2");
+#endif
+        r->sim->add_vector_array(names);
     } else {
         r->pre = std::make_shared<sph::gsph::PreInteraction>();
         r->force = std::make_shared<sph::gsph::FluidForce>();
@@ -202,7 +217,7 @@ void ref_set_fields(void *h, const double *f, uint64_t mask)
     const int64_t n = r->num;
     auto &ps = r->sim->get_particles();
 #define ROW(fi) (f + (int64_t)(fi) * n)
-    const bool gs = r->param->type == sph::SPHType::GSPH;
+    const bool gs = r->param->type == sph::SPHType::GSPH || r->param->type == sph::SPHType::GDISPH;
     for (int i = 0; i < r->num; ++i) {
         auto &q = ps[i];
         for (int k = 0; k < DIM; ++k) {
@@ -233,7 +248,7 @@ void ref_get_fields(void *h, double *f, uint64_t mask)
     Ref *r = (Ref *)h;
     const int64_t n = r->num;
     auto &ps = r->sim->get_particles();
-    const bool gs = r->param->type == sph::SPHType::GSPH;
+    const bool gs = r->param->type == sph::SPHType::GSPH || r->param->type == sph::SPHType::GDISPH;
     for (int i = 0; i < r->num; ++i) {
         const auto &q = ps[i];
         for (int k = 0; k < 3; ++k) {

@@ -610,7 +610,8 @@ public:
             }
             if (prm.iterative_sml) {
                 int it = 0;
-                pi_.sml = newton_raphson(i, list, n_tmp, type == SPHGPU_DISPH, &it);
+                /* GDISPH uses the DISPH Newton (src/gdisph/gd_pre_interaction.cpp:214-270) */
+                pi_.sml = newton_raphson(i, list, n_tmp, type == SPHGPU_DISPH || type == SPHGPU_GDISPH, &it);
 #pragma omp atomic
                 newton_iters_total += (it < 0 ? -it : it);
                 if (it < 0) {
@@ -630,7 +631,7 @@ public:
                 const real r = norm(r_ij);
                 if (r >= pi_.sml) break;
                 ++n_nb;
-                if (type == SPHGPU_DISPH) {
+                if (type == SPHGPU_DISPH || type == SPHGPU_GDISPH) {
                     const real w_ij = kernel.w(r, pi_.sml);
                     const real dhw_ij = kernel.dhw(r, pi_.sml);
                     dens_i += pj.mass * w_ij;
@@ -652,7 +653,7 @@ public:
                 }
             }
             pi_.dens = dens_i;
-            if (type == SPHGPU_DISPH) {
+            if (type == SPHGPU_DISPH || type == SPHGPU_GDISPH) {
                 pi_.pres = (gamma - 1.0) * pres_i;
                 pi_.gradh = pi_.sml / (D * n_i) * dh_pres_i / (1.0 + pi_.sml / (D * n_i) * dh_n_i);
             } else if (type == SPHGPU_SSPH) {
@@ -665,9 +666,9 @@ public:
             const real hv = pi_.sml / v_sig_max;
             if (hpvs[tid] > hv) hpvs[tid] = hv;
 
-            if (type == SPHGPU_GSPH) {
-                /* MUSCL gradients, src/gsph/g_pre_interaction.cpp:112-136 */
-                if (!prm.gsph_2nd_order) continue;
+            if ((type == SPHGPU_GSPH || type == SPHGPU_GDISPH) && prm.gsph_2nd_order) {
+                /* MUSCL gradients, src/gsph/g_pre_interaction.cpp:112-136,
+                 * src/gdisph/gd_pre_interaction.cpp:125-148 */
                 Vec<D> dd, du;
                 Vec<D> dv[D];
                 for (int n = 0; n < n_nb; ++n) {
@@ -691,12 +692,16 @@ public:
                 const real rho_inv = 1.0 / pi_.dens;
                 for (int a = 0; a < D; ++a)
                     for (int k = 0; k < D; ++k) grad_v[a][i][k] = dv[a][k] * rho_inv;
-                continue;
             }
+            if (type == SPHGPU_GSPH) continue;
 
             /* Balsara switch and time-dependent AV,
-             * src/pre_interaction.cpp:115-161, src/disph/d_pre_interaction.cpp:106-154 */
+             * src/pre_interaction.cpp:115-161, src/disph/d_pre_interaction.cpp:106-154;
+             * GDISPH (src/gdisph/gd_pre_interaction.cpp:150-197) weights the
+             * Balsara sums by mass and divides by density, and its
+             * time-dependent AV without Balsara is the DISPH one */
             const bool disph = type == SPHGPU_DISPH;
+            const bool disph_av = type == SPHGPU_DISPH || type == SPHGPU_GDISPH;
             if (prm.use_balsara_switch && D != 1) {
                 real div_v = 0.0;
                 Vec<D> rot3;      /* D == 3 */
@@ -752,10 +757,10 @@ public:
                     Vec<D> dw, v_ij;
                     for (int k = 0; k < D; ++k) dw[k] = r_ij[k] * c;
                     for (int k = 0; k < D; ++k) v_ij[k] = pi_.vel[k] - pj.vel[k];
-                    const real wgt = disph ? pj.mass * pj.ene : pj.mass;
+                    const real wgt = disph_av ? pj.mass * pj.ene : pj.mass;
                     div_v -= wgt * inner(v_ij, dw);
                 }
-                if (disph) div_v *= (gamma - 1.0) / pi_.pres;
+                if (disph_av) div_v *= (gamma - 1.0) / pi_.pres;
                 else div_v /= pi_.dens;
                 const real tau_inv = prm.epsilon_av * pi_.sound / pi_.sml;
                 const real s_i = std::max(-div_v, (real)0.0);
@@ -829,6 +834,7 @@ public:
             }
             int64_t pairs = 0, rit = 0;
             if (type == SPHGPU_GSPH) force_gsph(i, list, n_nb, pairs, rit);
+            else if (type == SPHGPU_GDISPH) force_gdisph(i, list, n_nb, pairs);
             else force_ssph_disph(i, list, n_nb, type == SPHGPU_DISPH, pairs);
 #pragma omp atomic
             force_pairs += pairs;
@@ -889,6 +895,98 @@ public:
     }
 
     int64_t force_pairs = 0, riemann_iters = 0, newton_iters_total = 0, newton_fail = 0;
+
+    /* GSPH states of the pair (right = i, left = j): MUSCL-reconstructed
+     * (src/gsph/g_fluid_force.cpp:103-140) or the particles' own */
+    void riemann_states(int i, int j, const Vec<D> &e_ij, real r, real r_inv, real ve_i, real ve_j, real left[4],
+                        real right[4]) const
+    {
+        const Particle<D> &pi_ = p[i];
+        const Particle<D> &pj = p[j];
+        const real gamma = prm.gamma;
+        if (prm.gsph_2nd_order) {
+            const real delta_i = 0.5 * (1.0 - pi_.sound * dt * r_inv);
+            const real delta_j = 0.5 * (1.0 - pj.sound * dt * r_inv);
+            const real dv_ij = ve_i - ve_j;
+            Vec<D> dv_i, dv_j;
+            for (int k = 0; k < D; ++k) {
+                dv_i[k] = inner(grad_v[k][i], e_ij);
+                dv_j[k] = inner(grad_v[k][j], e_ij);
+            }
+            const real dve_i = inner(dv_i, e_ij) * r;
+            const real dve_j = inner(dv_j, e_ij) * r;
+            right[0] = ve_i - limiter(dv_ij, dve_i) * delta_i;
+            left[0] = ve_j + limiter(dv_ij, dve_j) * delta_j;
+            const real dd_ij = pi_.dens - pj.dens;
+            const real dd_i = inner(grad_d[i], e_ij) * r;
+            const real dd_j = inner(grad_d[j], e_ij) * r;
+            right[1] = pi_.dens - limiter(dd_ij, dd_i) * delta_i;
+            left[1] = pj.dens + limiter(dd_ij, dd_j) * delta_j;
+            const real dp_ij = pi_.pres - pj.pres;
+            const real dp_i = inner(grad_p[i], e_ij) * r;
+            const real dp_j = inner(grad_p[j], e_ij) * r;
+            right[2] = pi_.pres - limiter(dp_ij, dp_i) * delta_i;
+            left[2] = pj.pres + limiter(dp_ij, dp_j) * delta_j;
+            right[3] = std::sqrt(gamma * right[2] / right[1]);
+            left[3] = std::sqrt(gamma * left[2] / left[1]);
+        } else {
+            right[0] = ve_i; right[1] = pi_.dens; right[2] = pi_.pres; right[3] = pi_.sound;
+            left[0] = ve_j; left[1] = pj.dens; left[2] = pj.pres; left[3] = pj.sound;
+        }
+    }
+
+    /* src/gdisph/gd_fluid_force.cpp:40-169: the HLL p* of the (reconstructed)
+     * pair states replaces the pressures of the DISPH force, with AV/AC */
+    void force_gdisph(int i, const std::vector<int> &list, int n_nb, int64_t &pairs)
+    {
+        Particle<D> &pi_ = p[i];
+        const real gamma = prm.gamma;
+        const Vec<D> &r_i = pi_.pos;
+        const Vec<D> &v_i = pi_.vel;
+        const real h_i = pi_.sml;
+        const real gamma2_u_i = sqr(gamma - 1.0) * pi_.ene;
+        const real gamma2_u_per_pres_i = gamma2_u_i / pi_.pres;
+        const real m_u_inv = 1.0 / (pi_.mass * pi_.ene);
+        Vec<D> acc;
+        real dene = 0.0;
+        for (int n = 0; n < n_nb; ++n) {
+            const int j = list[n];
+            const Particle<D> &pj = p[j];
+            const Vec<D> r_ij = periodic.calc_r_ij(r_i, pj.pos);
+            const real r = norm(r_ij);
+            if (r >= std::max(h_i, pj.sml) || r == 0.0) continue;
+            ++pairs;
+            const real r_inv = 1.0 / r;
+            Vec<D> e_ij;
+            for (int k = 0; k < D; ++k) e_ij[k] = r_ij[k] * r_inv;
+            const real ve_i = inner(v_i, e_ij);
+            const real ve_j = inner(pj.vel, e_ij);
+            real right[4], left[4], pstar, vstar;
+            riemann_states(i, j, e_ij, r, r_inv, ve_i, ve_j, left, right);
+            hll_solver(left, right, pstar, vstar);
+            const real ci = kernel.dwc(r, h_i);
+            const real cj = kernel.dwc(r, pj.sml);
+            Vec<D> dw_i, dw_j, dw_ij, v_ij;
+            for (int k = 0; k < D; ++k) dw_i[k] = r_ij[k] * ci;
+            for (int k = 0; k < D; ++k) dw_j[k] = r_ij[k] * cj;
+            for (int k = 0; k < D; ++k) dw_ij[k] = (dw_i[k] + dw_j[k]) * 0.5;
+            for (int k = 0; k < D; ++k) v_ij[k] = v_i[k] - pj.vel[k];
+            const real f_ij = 1.0 - pi_.gradh / (pj.mass * pj.ene);
+            const real f_ji = 1.0 - pj.gradh * m_u_inv;
+            const real u_per_pres_j = pj.ene / pj.pres;
+            const real pi_ij = artificial_viscosity(pi_, pj, r_ij);
+            const real dene_ac = prm.use_ac ? artificial_conductivity(pi_, pj, r_ij, dw_ij) : 0.0;
+            const real pstar_term_i = pstar / pi_.pres;
+            const real pstar_term_j = pstar / pj.pres;
+            const real s1 = pj.mass * (gamma2_u_per_pres_i * pj.ene * f_ij * pstar_term_i + 0.5 * pi_ij);
+            const real s2 = pj.mass * (gamma2_u_i * u_per_pres_j * f_ji * pstar_term_j + 0.5 * pi_ij);
+            for (int k = 0; k < D; ++k) acc[k] -= dw_i[k] * s1 + dw_j[k] * s2;
+            dene += pj.mass * gamma2_u_per_pres_i * pj.ene * f_ij * pstar_term_i * inner(v_ij, dw_i) +
+                    0.5 * pj.mass * pi_ij * inner(v_ij, dw_ij) + dene_ac;
+        }
+        pi_.acc = acc;
+        pi_.dene = dene;
+    }
 
     /* src/gsph/g_fluid_force.cpp:44-171 */
     void force_gsph(int i, const std::vector<int> &list, int n_nb, int64_t &pairs, int64_t &rit)

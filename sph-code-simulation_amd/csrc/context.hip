@@ -198,7 +198,7 @@ int sphgpu_device_info(char *name, int name_len, int *cu_count, double *hbm_gb)
 static int check_params(const sphgpu_params *p, std::string &why)
 {
     if (p->dim < 1 || p->dim > 3) { why = "dim must be 1, 2 or 3"; return 0; }
-    if (p->sph_type < SPHGPU_SSPH || p->sph_type > SPHGPU_GSPH) { why = "unsupported SPHType"; return 0; }
+    if (p->sph_type < SPHGPU_SSPH || p->sph_type > SPHGPU_GDISPH) { why = "unsupported SPHType"; return 0; }
     if (p->kernel < SPHGPU_CUBIC_SPLINE || p->kernel > SPHGPU_GAUSSIAN) { why = "unknown kernel"; return 0; }
     if (p->kernel == SPHGPU_WENDLAND && p->dim == 1) { why = "Wendland C4 is undefined in 1D"; return 0; }
     const int bits = p->dim;

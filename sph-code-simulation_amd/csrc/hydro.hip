@@ -124,7 +124,7 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
     T h;
     if (a.iterative) {
         h = T(hs64 / a.kappa);
-        const T b = (TYPE == SPHGPU_DISPH) ? T(a.nn / A) : T(a.mass64[i] * a.nn / A);
+        const T b = (TYPE == SPHGPU_DISPH || TYPE == SPHGPU_GDISPH) ? T(a.nn / A) : T(a.mass64[i] * a.nn / A);
         bool conv = false;
         int it = 0;
         for (; it < 10;) {
@@ -135,7 +135,7 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
                 T d[3];
                 const T r = tsqrt(rec_r2<D, T>(RP[j], xi, a.box, d));
                 if (r >= h) continue;
-                const T wt = (TYPE == SPHGPU_DISPH) ? T(1) : RV[j].w;
+                const T wt = (TYPE == SPHGPU_DISPH || TYPE == SPHGPU_GDISPH) ? T(1) : RV[j].w;
                 dn.add(wt * Kern<D, K, T>::w(r, h));
                 ddn.add(wt * Kern<D, K, T>::dhw(r, h));
             }
@@ -219,7 +219,7 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
         const T wk = Kern<D, K, T>::w(r, h);
         const V4 vj = RV[j];
         const T mj = vj.w;
-        if (TYPE == SPHGPU_DISPH) {
+        if (TYPE == SPHGPU_DISPH || TYPE == SPHGPU_GDISPH) {
             const T dh = Kern<D, K, T>::dhw(r, h);
             const T uj = RT[j].w;
             dens_a.add(mj * wk);
@@ -245,7 +245,7 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
     const T dens = dens_a.get();
     const T gm1 = T(a.gamma - 1.0);
     T pres, gradh = T(0);
-    if (TYPE == SPHGPU_DISPH) {
+    if (TYPE == SPHGPU_DISPH || TYPE == SPHGPU_GDISPH) {
         const T ni = n_a.get();
         pres = gm1 * pres_a.get();
         gradh = tdiv(tdiv(h, T(D) * ni) * dhp_a.get(), T(1.0) + tdiv(h, T(D) * ni) * dhn_a.get());
@@ -313,9 +313,9 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
     wave_add_atomic(a.stats, 2, valid ? newton_fail : 0);
     if (!valid) return;
 
-    if (TYPE == SPHGPU_GSPH) {
-        // MUSCL gradients, g_pre_interaction.cpp:112-136
-        if (!a.gsph2) return;
+    if ((TYPE == SPHGPU_GSPH || TYPE == SPHGPU_GDISPH) && a.gsph2) {
+        // MUSCL gradients, g_pre_interaction.cpp:112-136 (GDISPH:
+        // gd_pre_interaction.cpp:125-148)
         T dd[3] = {0, 0, 0}, du[3] = {0, 0, 0}, dv[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
         for (int q = 0; q < nc; ++q) {
             if (q < 256) {
@@ -366,13 +366,16 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
         }
 #pragma unroll
         for (int q = 0; q < 16; ++q) a.rec_w[R_G0 + (q >> 2)][4 * (size_t)i + (q & 3)] = g[q];
-        return;
     }
+    if (TYPE == SPHGPU_GSPH) return;
 
-    // ---- Balsara switch and time-dependent AV (pre_interaction.cpp:115-161) ----
+    // ---- Balsara switch and time-dependent AV (pre_interaction.cpp:115-161;
+    // GDISPH, gd_pre_interaction.cpp:150-197: Balsara sums weighted by mass
+    // and divided by density, the AV without Balsara as in DISPH) ----
     const T dt = T(a.dscal[0]);
     T alpha = a.rv.r[R_AUX][i].z;
     const bool do_bal = a.balsara_on && D != 1;
+    const bool disph_w = (TYPE == SPHGPU_DISPH) || (TYPE == SPHGPU_GDISPH && !do_bal);
     if (do_bal || a.td_av) {
         T div = T(0), rot[3] = {0, 0, 0};
         for (int q = 0; q < nc; ++q) {
@@ -396,7 +399,7 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
                 dw[k] = d[k] * cw;
                 vij[k] = vi[k] - vjv[k];
             }
-            const T wt = (TYPE == SPHGPU_DISPH) ? vj.w * RT[j].w : vj.w;
+            const T wt = disph_w ? vj.w * RT[j].w : vj.w;
             T dot = T(0);
 #pragma unroll
             for (int k = 0; k < D; ++k) dot = (k == 0) ? vij[k] * dw[k] : dot + vij[k] * dw[k];
@@ -410,7 +413,7 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
             }
         }
         T rot_abs;
-        if (TYPE == SPHGPU_DISPH) {
+        if (disph_w) {
             const T p_inv = gm1 / pres;
             div *= p_inv;
             for (int k = 0; k < 3; ++k) rot[k] *= p_inv;
@@ -616,6 +619,69 @@ SPH_DEV T av_pi(const T vij[3], const T d[3], T r, T ai, T aj, T bi, T bj, T ci,
     return T(0);
 }
 
+// GSPH Riemann states of a pair, right = i and left = j: MUSCL-reconstructed
+// with the van Leer limiter (src/gsph/g_fluid_force.cpp:103-140, the same in
+// src/gdisph/gd_fluid_force.cpp:101-138) or the particles' own.  d = r_ij,
+// g = (grad rho, grad p, grad v) flattened as in the gradient records.
+template <int D, typename T>
+SPH_DEV void pair_states(bool second, T gamma, T dt, const T d[3], T r, const T vi[3],
+                         const typename V4T<T>::type &vj, const typename V4T<T>::type &ti,
+                         const typename V4T<T>::type &tj, const T gi[16], const T gj[16], T left[4], T right[4])
+{
+    const T r_inv = tdiv(T(1.0), r);
+    T e[3] = {0, 0, 0};
+    T ve_i = T(0), ve_j = T(0);
+    const T vjv[3] = {vj.x, vj.y, vj.z};
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        e[k] = d[k] * r_inv;
+        ve_i = (k == 0) ? vi[k] * e[k] : ve_i + vi[k] * e[k];
+        ve_j = (k == 0) ? vjv[k] * e[k] : ve_j + vjv[k] * e[k];
+    }
+    const T rhoi = ti.x, presi = ti.y, ci = ti.z;
+    const T rhoj = tj.x, presj = tj.y, cj = tj.z;
+    if (!second) {
+        right[0] = ve_i; right[1] = rhoi; right[2] = presi; right[3] = ci;
+        left[0] = ve_j; left[1] = rhoj; left[2] = presj; left[3] = cj;
+        return;
+    }
+    const T delta_i = T(0.5) * (T(1.0) - ci * dt * r_inv);
+    const T delta_j = T(0.5) * (T(1.0) - cj * dt * r_inv);
+    T dve_i = T(0), dve_j = T(0), dd_i = T(0), dd_j = T(0), dp_i = T(0), dp_j = T(0);
+#pragma unroll
+    for (int b = 0; b < D; ++b) {
+        T si = T(0), sj = T(0);
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            si = (k == 0) ? gi[6 + 3 * b + k] * e[k] : si + gi[6 + 3 * b + k] * e[k];
+            sj = (k == 0) ? gj[6 + 3 * b + k] * e[k] : sj + gj[6 + 3 * b + k] * e[k];
+        }
+        dve_i = (b == 0) ? si * e[b] : dve_i + si * e[b];
+        dve_j = (b == 0) ? sj * e[b] : dve_j + sj * e[b];
+    }
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        dd_i = (k == 0) ? gi[k] * e[k] : dd_i + gi[k] * e[k];
+        dd_j = (k == 0) ? gj[k] * e[k] : dd_j + gj[k] * e[k];
+        dp_i = (k == 0) ? gi[3 + k] * e[k] : dp_i + gi[3 + k] * e[k];
+        dp_j = (k == 0) ? gj[3 + k] * e[k] : dp_j + gj[3 + k] * e[k];
+    }
+    dve_i = dve_i * r; dve_j = dve_j * r;
+    dd_i = dd_i * r; dd_j = dd_j * r;
+    dp_i = dp_i * r; dp_j = dp_j * r;
+    const T dv_ij = ve_i - ve_j;
+    right[0] = ve_i - limiter(dv_ij, dve_i) * delta_i;
+    left[0] = ve_j + limiter(dv_ij, dve_j) * delta_j;
+    const T dd_ij = rhoi - rhoj;
+    right[1] = rhoi - limiter(dd_ij, dd_i) * delta_i;
+    left[1] = rhoj + limiter(dd_ij, dd_j) * delta_j;
+    const T dp_ij = presi - presj;
+    right[2] = presi - limiter(dp_ij, dp_i) * delta_i;
+    left[2] = presj + limiter(dp_ij, dp_j) * delta_j;
+    right[3] = tsqrt(gamma * right[2] / right[1]);
+    left[3] = tsqrt(gamma * left[2] / left[1]);
+}
+
 template <int D, int K, int TYPE, typename T>
 __global__ __launch_bounds__(256) void k_force_sph(ForceArgs<T> a)
 {
@@ -636,6 +702,19 @@ __global__ __launch_bounds__(256) void k_force_sph(ForceArgs<T> a)
     const T g2u_i = sqr(gm1) * ui;
     const T g2upp_i = tdiv(g2u_i, presi);
     const T m_u_inv = tdiv(T(1.0), mi * ui);
+    // GDISPH: the particle's own MUSCL gradients (gd, gp, gv records)
+    T gi[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) gi[q] = T(0);
+    if (TYPE == SPHGPU_GDISPH && a.gsph2) {
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const V4 g = a.rv.r[R_G0 + b][i];
+            gi[4 * b] = g.x; gi[4 * b + 1] = g.y; gi[4 * b + 2] = g.z; gi[4 * b + 3] = g.w;
+        }
+    }
+    const T dt = (TYPE == SPHGPU_GDISPH) ? T(a.dscal[0]) : T(0);
+    const T gamma = T(a.gamma);
     Acc<T> ac[3], de;
     const int nc = a.cnt[i];
     for (int q = 0; q < nc; ++q) {
@@ -643,6 +722,23 @@ __global__ __launch_bounds__(256) void k_force_sph(ForceArgs<T> a)
         const V4 pj = RP[j], vj = RV[j], tj = RT[j], aj = RA[j];
         T d[3];
         const T r = tsqrt(rec_r2<D, T>(pj, xi, a.box, d));
+        // GDISPH: HLL p* of the pair (right = i, left = j)
+        T pstar = T(0);
+        if (TYPE == SPHGPU_GDISPH) {
+            T gj[16];
+#pragma unroll
+            for (int q2 = 0; q2 < 16; ++q2) gj[q2] = T(0);
+            if (a.gsph2) {
+#pragma unroll
+                for (int b = 0; b < 4; ++b) {
+                    const V4 g = a.rv.r[R_G0 + b][j];
+                    gj[4 * b] = g.x; gj[4 * b + 1] = g.y; gj[4 * b + 2] = g.z; gj[4 * b + 3] = g.w;
+                }
+            }
+            T left[4], right[4], vstar;
+            pair_states<D, T>(a.gsph2 != 0, gamma, dt, d, r, vi, vj, ti_, tj, gi, gj, left, right);
+            hll_solver<T>(left, right, pstar, vstar);
+        }
         const T hj = pj.w, rhoj = tj.x, presj = tj.y, mj = vj.w, uj = tj.w;
         const T cwi = Kern<D, K, T>::dwc(r, hi);
         const T cwj = Kern<D, K, T>::dwc(r, hj);
@@ -684,13 +780,23 @@ __global__ __launch_bounds__(256) void k_force_sph(ForceArgs<T> a)
             s1 = mj * (p_per_rho2_i * gradhi + T(0.5) * pi_ij);
             s2 = mj * (tdiv(presj, sqr(rhoj)) * aj.x + T(0.5) * pi_ij);
             e1 = mj * p_per_rho2_i * gradhi * vdwi + T(0.5) * mj * pi_ij * vdwij + dene_ac;
-        } else {
+        } else if (TYPE == SPHGPU_DISPH) {
             const T f_ij = T(1.0) - tdiv(gradhi, mj * uj);
             const T f_ji = T(1.0) - aj.x * m_u_inv;
             const T u_per_pres_j = tdiv(uj, presj);
             s1 = mj * (g2upp_i * uj * f_ij + T(0.5) * pi_ij);
             s2 = mj * (g2u_i * u_per_pres_j * f_ji + T(0.5) * pi_ij);
             e1 = mj * g2upp_i * uj * f_ij * vdwi + T(0.5) * mj * pi_ij * vdwij + dene_ac;
+        } else {
+            // GDISPH, gd_fluid_force.cpp:158-185: the DISPH terms scaled by
+            // p* / P_i and p* / P_j
+            const T f_ij = T(1.0) - tdiv(gradhi, mj * uj);
+            const T f_ji = T(1.0) - aj.x * m_u_inv;
+            const T u_per_pres_j = tdiv(uj, presj);
+            const T pst_i = tdiv(pstar, presi), pst_j = tdiv(pstar, presj);
+            s1 = mj * (g2upp_i * uj * f_ij * pst_i + T(0.5) * pi_ij);
+            s2 = mj * (g2u_i * u_per_pres_j * f_ji * pst_j + T(0.5) * pi_ij);
+            e1 = mj * g2upp_i * uj * f_ij * pst_i * vdwi + T(0.5) * mj * pi_ij * vdwij + dene_ac;
         }
 #pragma unroll
         for (int k = 0; k < D; ++k) ac[k].add(-(dwi[k] * s1 + dwj[k] * s2));
@@ -1112,6 +1218,7 @@ template <int D, int K, typename T> void dispatch_pre_type(sphgpu_ctx *c)
     switch (c->prm.sph_type) {
     case SPHGPU_SSPH: launch_pre<D, K, SPHGPU_SSPH, T>(c); break;
     case SPHGPU_DISPH: launch_pre<D, K, SPHGPU_DISPH, T>(c); break;
+    case SPHGPU_GDISPH: launch_pre<D, K, SPHGPU_GDISPH, T>(c); break;
     default: launch_pre<D, K, SPHGPU_GSPH, T>(c); break;
     }
 }
@@ -1205,6 +1312,8 @@ template <int D, int K, typename T> void launch_force(sphgpu_ctx *c)
         hipLaunchKernelGGL((k_force_sph<D, K, SPHGPU_SSPH, T>), g, b, 0, c->stream, a);
     } else if (c->prm.sph_type == SPHGPU_DISPH) {
         hipLaunchKernelGGL((k_force_sph<D, K, SPHGPU_DISPH, T>), g, b, 0, c->stream, a);
+    } else if (c->prm.sph_type == SPHGPU_GDISPH) {
+        hipLaunchKernelGGL((k_force_sph<D, K, SPHGPU_GDISPH, T>), g, b, 0, c->stream, a);
     } else if (c->prm.riemann_solver == SPHGPU_RIEMANN_HLL) {
         hipLaunchKernelGGL((k_force_gsph<D, K, T, SPHGPU_RIEMANN_HLL, 0>), g, b, 0, c->stream, a);
     } else {

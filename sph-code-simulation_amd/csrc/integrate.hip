@@ -288,7 +288,7 @@ int refresh_mirrors(sphgpu_ctx *c)
     c->flist = 0;  // records change: a force list derived earlier is stale
     RecArgs a;
     a.n = c->n;
-    const bool gsph2 = c->prm.sph_type == SPHGPU_GSPH && c->prm.gsph_2nd_order;
+    const bool gsph2 = (c->prm.sph_type == SPHGPU_GSPH || c->prm.sph_type == SPHGPU_GDISPH) && c->prm.gsph_2nd_order;
     a.nrec = gsph2 ? R_NREC : R_AUX + 1;
     for (int f = 0; f < 4 * R_NREC; ++f) a.src[f] = nullptr;
     for (int k = 0; k < c->dim; ++k) {

@@ -737,7 +737,7 @@ static int permute_state(sphgpu_ctx *c)
         fields.push_back(&c->acc[k]);
     }
     for (int s = 0; s < S_NSCALAR; ++s) fields.push_back(&c->sc[s]);
-    if (c->prm.sph_type == SPHGPU_GSPH) {
+    if (c->prm.sph_type == SPHGPU_GSPH || c->prm.sph_type == SPHGPU_GDISPH) {
         for (int k = 0; k < c->dim; ++k) {
             fields.push_back(&c->grad[k]);
             fields.push_back(&c->grad[3 + k]);

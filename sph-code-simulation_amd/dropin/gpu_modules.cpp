@@ -41,7 +41,9 @@ sphgpu_params to_sphgpu_params(const SPHParameters &p)
 {
     sphgpu_params q;
     sphgpu_params_default(&q, DIM);
-    q.sph_type = p.type == SPHType::SSPH ? SPHGPU_SSPH : p.type == SPHType::DISPH ? SPHGPU_DISPH : SPHGPU_GSPH;
+    q.sph_type = p.type == SPHType::SSPH ? SPHGPU_SSPH
+               : p.type == SPHType::DISPH ? SPHGPU_DISPH
+               : p.type == SPHType::GDISPH ? SPHGPU_GDISPH : SPHGPU_GSPH;
     q.kernel = p.kernel == KernelType::CUBIC_SPLINE ? SPHGPU_CUBIC_SPLINE
              : p.kernel == KernelType::WENDLAND ? SPHGPU_WENDLAND : SPHGPU_GAUSSIAN;
     q.riemann_solver = p.gsph.riemann_solver == RiemannSolverType::ITERATIVE ? SPHGPU_RIEMANN_ITERATIVE
@@ -82,7 +84,7 @@ std::shared_ptr<Device> Device::attach(const std::shared_ptr<Simulation> &sim, c
     }
     std::shared_ptr<Device> d(new Device());
     d->m_num = sim->get_particle_num();
-    d->m_gsph = param.type == SPHType::GSPH;
+    d->m_gsph = param.type == SPHType::GSPH || param.type == SPHType::GDISPH;
     const sphgpu_params q = to_sphgpu_params(param);
     const int rc = sphgpu_create(&q, d->m_num, &d->m_ctx);
     if (rc != 0) THROW_ERROR("sphgpu_create failed (", rc, ")");

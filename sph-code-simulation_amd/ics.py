@@ -24,7 +24,7 @@ F_SML, F_SOUND, F_BALSARA, F_ALPHA, F_GRADH, F_PHI = 18, 19, 20, 21, 22, 23
 F_ID, F_NEIGHBOR, F_GRAD_D, F_GRAD_P, F_GRAD_V = 24, 25, 26, 29, 32
 NFIELDS = 41
 
-SSPH, DISPH, GSPH = 0, 1, 2
+SSPH, DISPH, GSPH, GDISPH = 0, 1, 2, 3
 CUBIC_SPLINE, WENDLAND, GAUSSIAN = 0, 1, 2
 RIEMANN_HLL, RIEMANN_ITERATIVE = 0, 1
 PAIR_FP32, PAIR_FP64 = 0, 1

@@ -61,6 +61,22 @@ CASES["sedov_3d_gsph_first_order"] = (sedov_first_order, 3)
 CASES["sedov_3d_ssph_tdav_ac"] = (ssph_3d_td_av, 3)
 
 
+def khi_gdisph():
+    p, f = ics.khi_2d(nx=16)
+    p.update(sph_type=ics.GDISPH, gsph_2nd_order=1)
+    return p, f
+
+
+def sedov_gdisph_first_order():
+    p, f = ics.sedov_3d(n1=8)
+    p.update(sph_type=ics.GDISPH, gsph_2nd_order=0)
+    return p, f
+
+
+CASES["khi_2d_gdisph"] = (khi_gdisph, 5)
+CASES["sedov_3d_gdisph_first_order"] = (sedov_gdisph_first_order, 3)
+
+
 def param_vector(p: dict) -> np.ndarray:
     from sphgpu import Params
     names = [n for n, _ in Params._fields_]
@@ -81,7 +97,7 @@ def make_case(name: str) -> None:
         raise SystemExit("oracle/_ref is not built; run `make -C oracle ref`")
     r = Reference(p, f)
     r.initialize()
-    keep = KEEP_GSPH if p["sph_type"] == ics.GSPH else KEEP
+    keep = KEEP_GSPH if p["sph_type"] in (ics.GSPH, ics.GDISPH) and p["gsph_2nd_order"] else KEEP
     init = r.get_fields()[keep]
     e_init = r.energy()
     hvs_init = r.h_per_v_sig
@@ -140,8 +156,11 @@ def make_riemann() -> None:
 
 
 def main() -> None:
-    for name in CASES:
+    names = sys.argv[1:] or list(CASES)
+    for name in names:
         make_case(name)
+    if sys.argv[1:]:
+        return
     make_tree_and_lists()
     make_riemann()
 

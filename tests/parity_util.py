@@ -7,11 +7,24 @@ import numpy as np
 import ics
 
 # small instances of the five workloads (sizes the oracle finishes in seconds)
+def _gdisph(builder, second_order):
+    def make():
+        p, f = builder()
+        p.update(sph_type=ics.GDISPH, gsph_2nd_order=int(second_order))
+        return p, f
+    return make
+
+
 SMALL = {
     "sod_1d": lambda: ics.sod_1d(),
     "khi_2d": lambda: ics.khi_2d(nx=32),
     "sedov_3d": lambda: ics.sedov_3d(n1=16),
     "evrard_3d": lambda: ics.evrard_3d(n=3000),
+    # GDISPH: MUSCL-reconstructed HLL p* on the KHI, first order on the
+    # Sedov (the reference's driver reads use2ndOrderGSPH only for GSPH, so
+    # its GDISPH runs first order)
+    "khi_2d_gdisph": _gdisph(lambda: ics.khi_2d(nx=32), True),
+    "sedov_3d_gdisph": _gdisph(lambda: ics.sedov_3d(n1=16), False),
 }
 
 

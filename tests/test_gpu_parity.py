@@ -235,7 +235,7 @@ def _compare_fields(case, p, gf, of, prec):
     if t == ics.SSPH:
         e_gh = rel_err(gf[ics.F_GRADH], of[ics.F_GRADH])
         assert np.quantile(e_gh, 0.999) < tol_rho * 10
-    elif t == ics.DISPH:
+    elif t in (ics.DISPH, ics.GDISPH):
         # DISPH grad-h (d_pre_interaction.cpp) is a ratio whose denominator
         # 1 + h/(D n) sum dW/dh nearly cancels on a uniform lattice, so its
         # relative error is not meaningful; it enters the force only through

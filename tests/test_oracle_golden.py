@@ -17,7 +17,8 @@ from sphgpu import Params
 
 GOLDEN = Path(__file__).resolve().parent / "golden"
 CASES = ["sod_1d_ssph", "khi_2d_disph", "sedov_3d_gsph_iter", "evrard_3d_disph_grav",
-         "sedov_3d_gsph_hll", "sedov_3d_gsph_first_order", "sedov_3d_ssph_tdav_ac"]
+         "sedov_3d_gsph_hll", "sedov_3d_gsph_first_order", "sedov_3d_ssph_tdav_ac",
+         "khi_2d_gdisph", "sedov_3d_gdisph_first_order"]
 
 
 def params_from_vector(v: np.ndarray) -> dict:

@@ -126,3 +126,35 @@ def test_reference_sedov_low_background_pressure_breaks(solver, n1, pbg):
     r2.integrate(10)
     u2 = r2.get_fields()[ics.F_ENE]
     assert np.isfinite(u2).all() and (u2 > 0).all()
+
+
+def _gdisph_case(name):
+    from parity_util import SMALL
+    if name in SMALL:
+        return SMALL[name]()
+    if name == "sod_1d_gdisph_muscl":
+        p, f = ics.sod_1d()
+        p.update(sph_type=ics.GDISPH, gsph_2nd_order=1)
+    elif name == "sedov_3d_gdisph_muscl_tdav_ac":
+        p, f = ics.sedov_3d(n1=10)
+        p.update(sph_type=ics.GDISPH, gsph_2nd_order=1, use_time_dependent_av=1, use_ac=1)
+    else:  # khi_2d_gdisph_nobalsara_tdav
+        p, f = ics.khi_2d(nx=16)
+        p.update(sph_type=ics.GDISPH, gsph_2nd_order=1, use_balsara_switch=0, use_time_dependent_av=1)
+    return p, f
+
+
+@pytest.mark.parametrize("name", ["khi_2d_gdisph", "sedov_3d_gdisph", "sod_1d_gdisph_muscl",
+                                  "sedov_3d_gdisph_muscl_tdav_ac", "khi_2d_gdisph_nobalsara_tdav"])
+def test_gdisph_bit_exact(name):
+    """GDISPH (src/gdisph/gd_pre_interaction.cpp, gd_fluid_force.cpp): the
+    DISPH pre-interaction with GSPH gradients and the mass-weighted Balsara
+    switch, and the DISPH force with the HLL p* of the (MUSCL) pair states."""
+    p, f = _gdisph_case(name)
+    o, r = _both(p, f)
+    o.initialize()
+    r.initialize()
+    _same(o, r, "init")
+    o.integrate(3)
+    r.integrate(3)
+    _same(o, r, "3 steps")
