@@ -1,0 +1,127 @@
+"""The reference's parameter files and sample problems (CPU): parameter JSON
+mapping (src/solver.cpp:299-438) on the reference's own sample configs
+(copied to tests/golden/sample_configs), and the sample generators
+(src/sample/*.cpp) against the reference's Evrard sample output and against
+what each generator fixes by construction (counts, masses, states)."""
+from __future__ import annotations
+
+import json
+import math
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import ics
+import samples
+
+CFG = Path(__file__).resolve().parent / "golden" / "sample_configs"
+
+
+def test_params_from_reference_configs():
+    seen = 0
+    for path in sorted(CFG.glob("*.json")):
+        j = json.loads(path.read_text())
+        name = path.stem
+        dim = next((d for s, (d, _) in samples.SAMPLES.items() if name.startswith(s) and
+                    (s != "shock_tube" or not name.startswith("shock_tube_2d"))), None)
+        if name.startswith("shock_tube_2d"):
+            dim = 2
+        p = samples.params_from_json(path, dim)
+        st = {"ssph": ics.SSPH, "disph": ics.DISPH, "gsph": ics.GSPH, "gdisph": ics.GDISPH}[j.get("SPHType", "ssph")]
+        assert p["sph_type"] == st, name
+        nn = j.get("neighborNumber", 50)
+        # property_tree get<int> with a default: a non-integer text reads as the default
+        assert p["neighbor_number"] == (nn if isinstance(nn, int) else 50)
+        assert p["gamma"] == float(j.get("gamma", 1.6666666666666667))
+        assert p["kernel"] == {"cubic_spline": ics.CUBIC_SPLINE, "wendland": ics.WENDLAND}[j.get("kernel", "cubic_spline")]
+        assert p["periodic"] == int(j.get("periodic", False))
+        assert p["leaf_particle_num"] == j.get("leafParticleNumber", 1)
+        assert p["use_balsara_switch"] == int(j.get("useBalsaraSwitch", True))
+        if p["periodic"]:
+            assert p["range_max"][:dim] == j["rangeMax"] and p["range_min"][:dim] == j["rangeMin"]
+        if st == ics.GSPH:
+            assert p["gsph_2nd_order"] == int(j.get("use2ndOrderGSPH", True))
+            assert p["riemann_solver"] == (ics.RIEMANN_ITERATIVE if j.get("riemannSolver") == "iterative"
+                                           else ics.RIEMANN_HLL)
+        else:
+            assert p["gsph_2nd_order"] == 0 and p["riemann_solver"] == ics.RIEMANN_HLL
+        seen += 1
+    assert seen >= 12
+    with pytest.raises(ValueError):
+        samples.params_from_json({"SPHType": "srgsph"}, 1)
+    with pytest.raises(ValueError):
+        samples.params_from_json({"kernel": "quintic"}, 1)
+    with pytest.raises(ValueError):
+        samples.params_from_json({"periodic": True, "rangeMax": [1.0], "rangeMin": [0.0]}, 2)
+
+
+def test_evrard_lattice_matches_reference_sample_output():
+    d = json.loads((CFG.parent / "evrard_lattice_n30.json").read_text())
+    p, f = samples.load_sample("evrard", CFG / "evrard.json")
+    n = f.shape[1]
+    assert n == d["count"] == 14328
+    for off, rows in ((0, d["first"]), (n - 100, d["last"])):
+        for i, r in enumerate(rows):
+            assert tuple("%g" % f[ics.F_POS + k, off + i] for k in range(3)) == tuple(r), off + i
+    np.testing.assert_allclose(f[ics.F_MASS].sum(), 1.0, rtol=1e-12)
+    r = np.sqrt((f[ics.F_POS:ics.F_POS + 3] ** 2).sum(axis=0))
+    assert r.max() <= 1.0
+    np.testing.assert_allclose(f[ics.F_DENS], 1.0 / (2.0 * math.pi * r), rtol=1e-15)
+    assert (f[ics.F_ENE] == 0.05).all() and p["use_gravity"] == 1
+
+
+def test_mt19937_and_canonical():
+    # the C++ standard fixes the 10000th output of std::mt19937 (default seed 5489)
+    eng = samples._MT19937(5489)
+    for _ in range(9999):
+        eng()
+    assert eng() == 4123659995
+    eng = samples._MT19937(1)
+    xs = [samples._uniform(eng, -1.0, 1.0) for _ in range(1000)]
+    assert min(xs) >= -1.0 and max(xs) < 1.0 and abs(np.mean(xs)) < 0.1
+
+
+@pytest.mark.parametrize("name,kw,count", [
+    ("shock_tube", dict(N=40), 400), ("shock_tube_2d", dict(Nx=20, Ny=4), 80), ("vacuum", dict(N=80), 80),
+    ("strong_shock", dict(N=80), 80), ("gresho_chan_vortex", dict(N=16), 256),
+    ("pairing_instability", dict(N=16), 256), ("khi", dict(N=32), 768), ("sedov", dict(N=20), 400),
+    ("pressure_equilibrium", dict(N=20), 400), ("evrard_cold_collapse", dict(N=10), None),
+    ("hydrostatic", dict(N=8), None)])
+def test_generators(name, kw, count):
+    dim, _ = samples.SAMPLES[name]
+    p = ics.default_params(dim)
+    p["gamma"] = 1.4
+    f = samples.make_sample(name, p, **kw)
+    n = f.shape[1]
+    if count is not None:
+        assert n == count
+    np.testing.assert_array_equal(f[ics.F_ID], np.arange(n))
+    # u = P / ((gamma - 1) rho) for every particle (and P = (gamma-1) rho u)
+    np.testing.assert_allclose(f[ics.F_PRES], (p["gamma"] - 1.0) * f[ics.F_DENS] * f[ics.F_ENE], rtol=1e-14)
+    assert np.isfinite(f).all()
+    if name == "shock_tube":
+        left = f[ics.F_POS] < 0.5
+        assert left.sum() == 320 and (f[ics.F_DENS][left] == 1.0).all() and (f[ics.F_PRES][~left] == 0.1795).all()
+    if name == "khi":
+        band = (f[ics.F_POS + 1] > 0.25) & (f[ics.F_POS + 1] < 0.75)
+        assert (f[ics.F_DENS][band] == 2.0).all() and (f[ics.F_VEL][band] == 0.5).all()
+        assert band.sum() == 2 * (~band).sum()   # density 2 at equal mass: twice the particles
+    if name == "sedov":
+        hot = f[ics.F_ENE] > 1.0
+        np.testing.assert_allclose((f[ics.F_MASS] * f[ics.F_ENE])[hot].sum(), 1.0, rtol=1e-12)
+    if name == "hydrostatic":
+        inner = (np.abs(f[ics.F_POS]) < 0.25) & (np.abs(f[ics.F_POS + 1]) < 0.25)
+        assert (f[ics.F_DENS][inner] == 4.0).all() and (f[ics.F_DENS][~inner] == 1.0).all()
+        assert inner.sum() == 64  # (0.5 / dx1)^2 = N^2 with dx1 = 0.5 / N
+    if name == "gresho_chan_vortex":
+        r = np.hypot(f[ics.F_POS], f[ics.F_POS + 1])
+        vt = np.hypot(f[ics.F_VEL], f[ics.F_VEL + 1])
+        np.testing.assert_allclose(vt, np.where(r < 0.2, 5 * r, np.where(r < 0.4, 2 - 5 * r, 0.0)), atol=1e-14)
+
+
+def test_load_sample_by_key():
+    p, f = samples.load_sample(None, {"sample": "shock_tube", "N": 10, "SPHType": "gdisph", "gamma": 1.4})
+    assert f.shape[1] == 100 and p["sph_type"] == ics.GDISPH and p["gsph_2nd_order"] == 0
+    with pytest.raises(ValueError):
+        samples.load_sample(None, {"N": 10})
