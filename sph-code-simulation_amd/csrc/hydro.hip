@@ -255,6 +255,15 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
     } else {
         pres = gm1 * dens * ui;
     }
+    // members other than the particle itself, as bits over the first 256
+    // candidates
+    unsigned long long am[4];
+    int na_bits = 0;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+        am[b] = mb[b] & ~(((qself >> 6) == b) ? 1ull << (qself & 63) : 0ull);
+        na_bits += __popcll(am[b]);
+    }
     if (valid) {
         a.sml[i] = h64;
         a.dens[i] = (double)dens;
@@ -268,34 +277,7 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
         if (TYPE != SPHGPU_GSPH) a.rec_w[R_AUX][4 * (size_t)i + 0] = gradh;
         if (a.posf_w) a.posf_w[4 * (size_t)i + 3] = (float)h;
         if (fl) {
-            unsigned long long am[4];
-            int na = 0;
-#pragma unroll
-            for (int b = 0; b < 4; ++b) {
-                am[b] = mb[b] & ~(((qself >> 6) == b) ? 1ull << (qself & 63) : 0ull);
-                na += __popcll(am[b]);
-            }
-            if (fl_rows) {
-                na = nrow;
-            } else {
-                // row r of the force list = the r-th member: every lane of the
-                // wavefront stores into the same 256-byte row at once, and the
-                // candidate rows it reads were just loaded by the member pass
-                int *const col = a.fl_idx + list_slot(i, 0, a.fl_cap);
-                const int *const pcol = a.idx + list_slot(i, 0, a.cap);
-                const int nw = min(na, a.fl_cap);
-                int wd = 0;
-                unsigned long long m = am[0];
-                for (int r = 0; r < nw; ++r) {
-                    while (m == 0ull && wd < 3) {
-                        ++wd;
-                        m = (wd == 1) ? am[1] : (wd == 2) ? am[2] : am[3];
-                    }
-                    const int qc = (wd << 6) + __ffsll((long long)m) - 1;
-                    m &= m - 1ull;
-                    col[r * kWave] = pcol[qc * kWave];
-                }
-            }
+            const int na = fl_rows ? nrow : na_bits;
             a.fl_cnt_a[i] = min(na, a.fl_cap);
             a.fl_cnt[i] = min(na, a.fl_cap);
             // complete only when every j with r < h_i was a candidate
@@ -313,21 +295,49 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
     wave_add_atomic(a.stats, 2, valid ? newton_fail : 0);
     if (!valid) return;
 
+    // The member passes below visit the members in rank order: in step r
+    // every lane takes its r-th member (from the member bits; the candidate
+    // rows were just loaded by the density pass), so the loop runs as many
+    // steps as the largest member count rather than the candidate count, and
+    // the first such pass stores row r of the force list for all lanes at
+    // once.  The particle itself contributes nothing to these sums and is
+    // skipped.  Past 256 candidates the member test is repeated.
+    const int *const pcol = a.idx + list_slot(i, 0, a.cap);
+    int *const frow = (fl && !fl_rows) ? a.fl_idx + list_slot(i, 0, a.fl_cap) : nullptr;
+    bool rows_pending = frow != nullptr;
+    auto members = [&](auto &&body) {
+        const bool write = rows_pending;
+        rows_pending = false;
+        int wd = 0;
+        unsigned long long m = am[0];
+        for (int r = 0; r < na_bits; ++r) {
+            while (m == 0ull && wd < 3) {
+                ++wd;
+                m = (wd == 1) ? am[1] : (wd == 2) ? am[2] : am[3];
+            }
+            const int qc = (wd << 6) + __ffsll((long long)m) - 1;
+            m &= m - 1ull;
+            const int j = pcol[qc * kWave];
+            if (write && r < a.fl_cap) frow[r * kWave] = j;
+            T d[3];
+            const T r2 = rec_r2<D, T>(RP[j], xi, a.box, d);
+            body(j, d, r2);
+        }
+        for (int q = 256; q < nc; ++q) {
+            const int j = pcol[q * kWave];
+            if (j == i) continue;
+            T d[3];
+            const T r2 = rec_r2<D, T>(RP[j], xi, a.box, d);
+            if (!member(j, r2)) continue;
+            body(j, d, r2);
+        }
+    };
+
     if ((TYPE == SPHGPU_GSPH || TYPE == SPHGPU_GDISPH) && a.gsph2) {
         // MUSCL gradients, g_pre_interaction.cpp:112-136 (GDISPH:
         // gd_pre_interaction.cpp:125-148)
         T dd[3] = {0, 0, 0}, du[3] = {0, 0, 0}, dv[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-        for (int q = 0; q < nc; ++q) {
-            if (q < 256) {
-                unsigned long long m = 0;
-#pragma unroll
-                for (int b = 0; b < 4; ++b) m = ((q >> 6) == b) ? mb[b] : m;
-                if (!((m >> (q & 63)) & 1ull)) continue;
-            }
-            const int j = a.idx[list_slot(i, q, a.cap)];
-            T d[3];
-            const T r2 = rec_r2<D, T>(RP[j], xi, a.box, d);
-            if (q >= 256 && !member(j, r2)) continue;
+        members([&](int j, const T d[3], T r2) {
             const T r = tsqrt(r2);
             const T cw = Kern<D, K, T>::dwc(r, h);
             const V4 vj = RV[j];
@@ -345,7 +355,7 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
 #pragma unroll
                 for (int b2 = 0; b2 < D; ++b2) dv[b2][k] += dwk * mdv[b2];
             }
-        }
+        });
         const T rho_inv = tdiv(T(1.0), dens);
         T g[16];
 #pragma unroll
@@ -367,7 +377,10 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
 #pragma unroll
         for (int q = 0; q < 16; ++q) a.rec_w[R_G0 + (q >> 2)][4 * (size_t)i + (q & 3)] = g[q];
     }
-    if (TYPE == SPHGPU_GSPH) return;
+    if (TYPE == SPHGPU_GSPH) {
+        if (rows_pending) members([](int, const T *, T) {});
+        return;
+    }
 
     // ---- Balsara switch and time-dependent AV (pre_interaction.cpp:115-161;
     // GDISPH, gd_pre_interaction.cpp:150-197: Balsara sums weighted by mass
@@ -378,17 +391,7 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
     const bool disph_w = (TYPE == SPHGPU_DISPH) || (TYPE == SPHGPU_GDISPH && !do_bal);
     if (do_bal || a.td_av) {
         T div = T(0), rot[3] = {0, 0, 0};
-        for (int q = 0; q < nc; ++q) {
-            if (q < 256) {
-                unsigned long long m = 0;
-#pragma unroll
-                for (int b = 0; b < 4; ++b) m = ((q >> 6) == b) ? mb[b] : m;
-                if (!((m >> (q & 63)) & 1ull)) continue;
-            }
-            const int j = a.idx[list_slot(i, q, a.cap)];
-            T d[3];
-            const T r2 = rec_r2<D, T>(RP[j], xi, a.box, d);
-            if (q >= 256 && !member(j, r2)) continue;
+        members([&](int j, const T d[3], T r2) {
             const T r = tsqrt(r2);
             const T cw = Kern<D, K, T>::dwc(r, h);
             const V4 vj = RV[j];
@@ -411,7 +414,7 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
             } else if (D == 2) {
                 rot[0] += (vij[0] * dw[1 % D] - vij[1 % D] * dw[0]) * wt;
             }
-        }
+        });
         T rot_abs;
         if (disph_w) {
             const T p_inv = gm1 / pres;
@@ -443,6 +446,7 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
             a.rec_w[R_AUX][4 * (size_t)i + 2] = alpha;
         }
     }
+    if (rows_pending) members([](int, const T *, T) {});
 }
 
 // ---- force pair lists from the pre-interaction lists ----------------------------
