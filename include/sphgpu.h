@@ -338,6 +338,27 @@ int sphgpu_energy_file_write(const char *path, double time, double kinetic, doub
 /* The same row for the device state: the context's time and the energies of
  * sphgpu_energy_momentum. */
 int sphgpu_energy_file_append(sphgpu_ctx *ctx, const char *path, int32_t precision);
+/* VTKWriter::open + write_particles (src/writers/vtk_writer.cpp:17-331): a
+ * legacy-VTK UNSTRUCTURED_GRID with one vertex per particle, positions,
+ * density, pressure, mass, internal_energy, smoothing_length, sound_speed,
+ * particle_id, velocity and acceleration, as big-endian binary (binary != 0,
+ * the reference's default) or ASCII.  An empty set writes the header and
+ * returns SPHGPU_E_ARG, as the reference's write_particles fails. */
+int sphgpu_write_vtk_fields(const char *path, const sphgpu_output_meta *meta, int32_t dim, int32_t n,
+                            const double *fields, int32_t binary);
+int sphgpu_write_snapshot_vtk(sphgpu_ctx *ctx, const char *path, const sphgpu_output_meta *meta, int32_t binary);
+/* OutputMetadata::to_json (which = 0) or UnitSystem::to_json (which = 1)
+ * as nlohmann::json::dump(2) prints them (src/output_metadata.cpp:67-116,
+ * src/units.cpp:206-231): the text the HDF5 writer stores in its
+ * "metadata_json" and "unit_system" attributes.  *needed gets the size with
+ * the terminating NUL; buf_len 0 only queries it. */
+int sphgpu_output_meta_json(const sphgpu_output_meta *meta, int32_t which, char *buf, int32_t buf_len,
+                            int32_t *needed);
+/* OutputMetadata::from_json (src/output_metadata.cpp:118-168): absent keys
+ * take the reference's defaults; unit names follow units.type_enum.  Text
+ * that is not a JSON object, or a key of the wrong JSON type, returns
+ * SPHGPU_E_ARG. */
+int sphgpu_output_meta_from_json(const char *text, sphgpu_output_meta *meta);
 
 #ifdef __cplusplus
 }

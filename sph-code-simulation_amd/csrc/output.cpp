@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <cerrno>
 #include <climits>
+#include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -568,6 +569,152 @@ int sphgpu_energy_file_append(sphgpu_ctx *ctx, const char *path, int32_t precisi
     if ((rc = sphgpu_get_time(ctx, &t))) return rc;
     if ((rc = sphgpu_energy_momentum(ctx, e))) return rc;
     return sphgpu_energy_file_write(path, t, e[0], e[1], e[2], precision);
+}
+
+} // extern "C"
+
+// ---- legacy VTK --------------------------------------------------------------
+// VTKWriter (src/writers/vtk_writer.cpp:17-331): an UNSTRUCTURED_GRID of one
+// VTK_VERTEX cell per particle, positions and fields as float.  BINARY writes
+// big-endian 4-byte values with no line break after a block, so the next
+// keyword line follows the last value directly; ASCII prints each double with
+// the stream's default format ("%g", 6 significant digits), absent axes as
+// the literal "0.0", and a blank line after every block but CELLS and
+// CELL_TYPES.
+namespace {
+
+void put_be32(std::string &s, uint32_t v)
+{
+    const char b[4] = {(char)(v >> 24), (char)(v >> 16), (char)(v >> 8), (char)v};
+    s.append(b, 4);
+}
+
+void put_be_float(std::string &s, double v)
+{
+    const float x = (float)v;
+    uint32_t u;
+    std::memcpy(&u, &x, 4);
+    put_be32(s, u);
+}
+
+void put_g(std::string &s, double v)
+{
+    char buf[40];
+    const int k = std::snprintf(buf, sizeof(buf), "%g", v);
+    s.append(buf, (size_t)k);
+}
+
+// one block of per-particle values, formatted by host threads
+template <typename F> void vtk_block(std::string &out, long long n, F row)
+{
+    const int t = worker_count(n);
+    std::vector<std::string> part((size_t)std::max(t, 1));
+    parallel_ranges(n, [&](int k, long long a, long long b) {
+        std::string &s = part[(size_t)k];
+        for (long long i = a; i < b; ++i) row(s, i);
+    });
+    for (const auto &x : part) out += x;
+}
+
+} // namespace
+
+extern "C" {
+
+int sphgpu_write_vtk_fields(const char *path, const sphgpu_output_meta *meta, int32_t dim, int32_t n,
+                            const double *f, int32_t binary)
+{
+    if (!path || !meta || n < 0 || (n > 0 && !f) || dim < 1 || dim > 3) return SPHGPU_E_ARG;
+    const bool bin = binary != 0;
+    std::string s;
+    // VTKWriter::write_header (vtk_writer.cpp:40-52)
+    s += "# vtk DataFile Version 3.0\n";
+    s += std::string("SPH Simulation - ") + meta->simulation_name + "\n";
+    s += bin ? "BINARY\n" : "ASCII\n";
+    s += "DATASET UNSTRUCTURED_GRID\n";
+    const size_t N = (size_t)n;
+    auto fld = [&](int row, long long i) { return f[(size_t)row * N + (size_t)i]; };
+    auto vec3 = [&](int base) {
+        return [&, base](std::string &o, long long i) {
+            for (int a = 0; a < 3; ++a) {
+                if (bin) {
+                    put_be_float(o, a < dim ? fld(base + a, i) : 0.0);
+                } else if (a < dim) {
+                    if (a) o += ' ';
+                    put_g(o, fld(base + a, i));
+                } else {
+                    o += " 0.0";
+                }
+            }
+            if (!bin) o += '\n';
+        };
+    };
+    auto scalar = [&](int row) {
+        return [&, row](std::string &o, long long i) {
+            if (bin) put_be_float(o, fld(row, i));
+            else { put_g(o, fld(row, i)); o += '\n'; }
+        };
+    };
+    bool ok = n > 0;  // VTKWriter::write_particles refuses an empty set
+    if (ok) {
+        // write_points (vtk_writer.cpp:91-126)
+        s += "POINTS " + std::to_string(n) + " float\n";
+        vtk_block(s, n, vec3(SPHGPU_F_POS));
+        if (!bin) s += '\n';
+        // cells and cell types (vtk_writer.cpp:67-85)
+        s += "CELLS " + std::to_string(n) + " " + std::to_string(2ll * n) + "\n";
+        vtk_block(s, n, [&](std::string &o, long long i) {
+            if (bin) { put_be32(o, 1u); put_be32(o, (uint32_t)(int32_t)i); }
+            else { o += "1 "; put_int(o, i); o += '\n'; }
+        });
+        s += "CELL_TYPES " + std::to_string(n) + "\n";
+        vtk_block(s, n, [&](std::string &o, long long) {
+            if (bin) put_be32(o, 1u);
+            else o += "1\n";
+        });
+        // write_point_data (vtk_writer.cpp:128-223)
+        s += "POINT_DATA " + std::to_string(n) + "\n";
+        const std::pair<const char *, int> scalars[] = {
+            {"density", SPHGPU_F_DENS}, {"pressure", SPHGPU_F_PRES}, {"mass", SPHGPU_F_MASS},
+            {"internal_energy", SPHGPU_F_ENE}, {"smoothing_length", SPHGPU_F_SML}, {"sound_speed", SPHGPU_F_SOUND}};
+        for (const auto &sc : scalars) {
+            s += std::string("SCALARS ") + sc.first + " float 1\n";
+            s += "LOOKUP_TABLE default\n";
+            vtk_block(s, n, scalar(sc.second));
+            if (!bin) s += '\n';
+        }
+        s += "SCALARS particle_id int 1\n";
+        s += "LOOKUP_TABLE default\n";
+        vtk_block(s, n, [&](std::string &o, long long i) {
+            const int32_t id = (int32_t)fld(SPHGPU_F_ID, i);
+            if (bin) put_be32(o, (uint32_t)id);
+            else { put_int(o, id); o += '\n'; }
+        });
+        if (!bin) s += '\n';
+        // write_vectors (vtk_writer.cpp:225-293)
+        s += "VECTORS velocity float\n";
+        vtk_block(s, n, vec3(SPHGPU_F_VEL));
+        if (!bin) s += '\n';
+        s += "VECTORS acceleration float\n";
+        vtk_block(s, n, vec3(SPHGPU_F_ACC));
+        if (!bin) s += '\n';
+    }
+    FILE *fp = std::fopen(path, "wb");
+    if (!fp) return SPHGPU_E_ARG;
+    bool wr = std::fwrite(s.data(), 1, s.size(), fp) == s.size();
+    wr = (std::fclose(fp) == 0) && wr;
+    return (ok && wr) ? 0 : SPHGPU_E_ARG;
+}
+
+int sphgpu_write_snapshot_vtk(sphgpu_ctx *ctx, const char *path, const sphgpu_output_meta *meta, int32_t binary)
+{
+    if (!ctx || !path || !meta) return SPHGPU_E_ARG;
+    int32_t n = 0;
+    sphgpu_params p;
+    int rc;
+    if ((rc = sphgpu_get_params(ctx, &p, &n))) return rc;
+    std::vector<double> f((size_t)SPHGPU_NFIELDS * n);
+    if ((rc = sphgpu_download(ctx, f.data(), SPHGPU_MASK_ALL))) return rc;
+    return sphgpu_write_vtk_fields(path, meta, p.dim, n, f.data(), binary);
 }
 
 } // extern "C"

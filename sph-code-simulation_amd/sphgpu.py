@@ -141,6 +141,101 @@ def snapshot_filename(directory, prefix: str, count: int, ext: str) -> str:
     return buf.value.decode()
 
 
+def output_meta_json(meta: OutputMeta, which: int = 0) -> str:
+    """sphgpu_output_meta_json: OutputMetadata::to_json (which=0) or
+    UnitSystem::to_json (which=1) as nlohmann's dump(2) prints it."""
+    lib = load_library()
+    need = C.c_int32(0)
+    rc = lib.sphgpu_output_meta_json(C.byref(meta), int(which), None, 0, C.byref(need))
+    if rc:
+        raise SphGpuError(f"sphgpu_output_meta_json failed ({rc})")
+    buf = C.create_string_buffer(need.value)
+    rc = lib.sphgpu_output_meta_json(C.byref(meta), int(which), buf, need.value, None)
+    if rc:
+        raise SphGpuError(f"sphgpu_output_meta_json failed ({rc})")
+    return buf.value.decode()
+
+
+def output_meta_from_json(text: str) -> OutputMeta:
+    """sphgpu_output_meta_from_json: OutputMetadata::from_json."""
+    m = OutputMeta()
+    rc = load_library().sphgpu_output_meta_from_json(text.encode(), C.byref(m))
+    if rc:
+        raise SphGpuError(f"sphgpu_output_meta_from_json failed ({rc})")
+    return m
+
+
+def write_vtk_fields(path, meta: OutputMeta, dim: int, fields: np.ndarray, binary: bool = True):
+    """sphgpu_write_vtk_fields: a legacy-VTK snapshot from a host block."""
+    f = np.ascontiguousarray(fields, dtype=np.float64)
+    rc = load_library().sphgpu_write_vtk_fields(str(path).encode(), C.byref(meta), int(dim), f.shape[1], _dp(f),
+                                                int(bool(binary)))
+    if rc:
+        raise SphGpuError(f"sphgpu_write_vtk_fields failed ({rc})")
+
+
+HDF5_LIB_PATH = PKG / "libsphgpu_hdf5.so"
+_h5 = None
+
+
+def load_hdf5_library() -> C.CDLL:
+    """Load libsphgpu_hdf5.so (include/sphgpu_hdf5.h), the HDF5 snapshot
+    library; it is built only where the HDF5 C library is installed."""
+    global _h5
+    if _h5 is not None:
+        return _h5
+    load_library()  # its dependency, and the C++ runtime it shares
+    if not HDF5_LIB_PATH.exists():
+        raise SphGpuError(f"{HDF5_LIB_PATH} is not built (no HDF5 C library at build time)")
+    lib = C.CDLL(str(HDF5_LIB_PATH))
+    vp, ip, dp, i32p = C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int32)
+    mp = C.POINTER(OutputMeta)
+    sigs = {
+        "sphgpu_write_hdf5_fields": (ip, [C.c_char_p, mp, C.c_int32, C.c_int32, dp, C.c_int32]),
+        "sphgpu_write_snapshot_hdf5": (ip, [vp, C.c_char_p, mp, C.c_int32]),
+        "sphgpu_hdf5_row_count": (ip, [C.c_char_p, i32p]),
+        "sphgpu_read_hdf5_fields": (ip, [C.c_char_p, mp, C.c_int32, dp]),
+        "sphgpu_read_snapshot_hdf5": (ip, [vp, C.c_char_p, mp]),
+    }
+    for name, (res, args) in sigs.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _h5 = lib
+    return lib
+
+
+EXPORTED_HDF5 = ("sphgpu_write_hdf5_fields", "sphgpu_write_snapshot_hdf5", "sphgpu_hdf5_row_count",
+                 "sphgpu_read_hdf5_fields", "sphgpu_read_snapshot_hdf5")
+
+
+def write_hdf5_fields(path, meta: OutputMeta, dim: int, fields: np.ndarray, compression: int = 6):
+    """sphgpu_write_hdf5_fields: an HDF5 snapshot from a host block."""
+    f = np.ascontiguousarray(fields, dtype=np.float64)
+    rc = load_hdf5_library().sphgpu_write_hdf5_fields(str(path).encode(), C.byref(meta), int(dim), f.shape[1],
+                                                      _dp(f), int(compression))
+    if rc:
+        raise SphGpuError(f"sphgpu_write_hdf5_fields failed ({rc})")
+
+
+def hdf5_row_count(path) -> int:
+    r = C.c_int32(0)
+    rc = load_hdf5_library().sphgpu_hdf5_row_count(str(path).encode(), C.byref(r))
+    if rc:
+        raise SphGpuError(f"sphgpu_hdf5_row_count failed ({rc})")
+    return r.value
+
+
+def read_hdf5_fields(path, n: int):
+    """sphgpu_read_hdf5_fields -> (OutputMeta, [NFIELDS][n] block)."""
+    f = np.zeros((NFIELDS, n), dtype=np.float64)
+    m = OutputMeta()
+    rc = load_hdf5_library().sphgpu_read_hdf5_fields(str(path).encode(), C.byref(m), int(n), _dp(f))
+    if rc:
+        raise SphGpuError(f"sphgpu_read_hdf5_fields failed ({rc})")
+    return m, f
+
+
 def params_struct(d: dict) -> Params:
     p = Params()
     for name, _ in Params._fields_:
@@ -222,6 +317,10 @@ def load_library() -> C.CDLL:
         "sphgpu_energy_file_begin": (ip, [C.c_char_p, C.POINTER(OutputMeta)]),
         "sphgpu_energy_file_write": (ip, [C.c_char_p, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int32]),
         "sphgpu_energy_file_append": (ip, [vp, C.c_char_p, C.c_int32]),
+        "sphgpu_write_vtk_fields": (ip, [C.c_char_p, C.POINTER(OutputMeta), C.c_int32, C.c_int32, dp, C.c_int32]),
+        "sphgpu_write_snapshot_vtk": (ip, [vp, C.c_char_p, C.POINTER(OutputMeta), C.c_int32]),
+        "sphgpu_output_meta_json": (ip, [C.POINTER(OutputMeta), C.c_int32, C.c_char_p, C.c_int32, i32p]),
+        "sphgpu_output_meta_from_json": (ip, [C.c_char_p, C.POINTER(OutputMeta)]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name)
@@ -247,7 +346,8 @@ EXPORTED = (
     "sphgpu_enable_timing", "sphgpu_get_params", "sphgpu_output_meta_init", "sphgpu_snapshot_filename",
     "sphgpu_write_snapshot_csv", "sphgpu_write_csv_fields", "sphgpu_read_snapshot_csv",
     "sphgpu_read_csv_fields", "sphgpu_csv_row_count", "sphgpu_energy_file_begin",
-    "sphgpu_energy_file_write", "sphgpu_energy_file_append",
+    "sphgpu_energy_file_write", "sphgpu_energy_file_append", "sphgpu_write_vtk_fields",
+    "sphgpu_write_snapshot_vtk", "sphgpu_output_meta_json", "sphgpu_output_meta_from_json",
 )
 
 
@@ -469,6 +569,21 @@ class Context:
     def energy_file_append(self, path, precision: int = 16):
         self._check(self.lib.sphgpu_energy_file_append(self.h, str(path).encode(), int(precision)),
                     "energy_file_append")
+
+    def write_snapshot_vtk(self, path, meta: OutputMeta, binary: bool = True):
+        self._check(self.lib.sphgpu_write_snapshot_vtk(self.h, str(path).encode(), C.byref(meta),
+                                                       int(bool(binary))), "write_snapshot_vtk")
+
+    def write_snapshot_hdf5(self, path, meta: OutputMeta, compression: int = 6):
+        h5 = load_hdf5_library()
+        self._check(h5.sphgpu_write_snapshot_hdf5(self.h, str(path).encode(), C.byref(meta), int(compression)),
+                    "write_snapshot_hdf5")
+
+    def read_snapshot_hdf5(self, path) -> OutputMeta:
+        h5 = load_hdf5_library()
+        m = OutputMeta()
+        self._check(h5.sphgpu_read_snapshot_hdf5(self.h, str(path).encode(), C.byref(m)), "read_snapshot_hdf5")
+        return m
 
     def kernel_times(self) -> dict:
         buf = C.create_string_buffer(8192)

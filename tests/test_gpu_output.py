@@ -79,3 +79,32 @@ def test_snapshot_throughput_1m(tmp_path):
     print(f"snapshot 1M: {size / 1e6:.0f} MB; write {t1 - t0:.2f} s ({n / (t1 - t0) / 1e6:.2f} M rows/s, "
           f"download included), read {t2 - t1:.2f} s (upload included); iostream writer {t4 - t3:.2f} s "
           f"({n / (t4 - t3) / 1e6:.2f} M rows/s)")
+
+
+def test_vtk_and_hdf5_snapshots_through_context(tmp_path):
+    """The context entry points sphgpu_write_snapshot_vtk /
+    sphgpu_write_snapshot_hdf5 write what the host-block writers write for the
+    downloaded state, and sphgpu_read_snapshot_hdf5 resumes a second context
+    to the same state and time (OutputManager::load_for_resume for .h5)."""
+    if not sphgpu.HDF5_LIB_PATH.exists():
+        pytest.skip("libsphgpu_hdf5.so not built (no HDF5 C library at build time)")
+    p, f = ics.sedov_3d(n1=16)
+    a = sphgpu.Context(p, f.shape[1])
+    a.upload(f)
+    a.initialize()
+    a.integrate(2)
+    meta = a.output_meta(2)
+    g = a.download()
+    for binary in (True, False):
+        a.write_snapshot_vtk(tmp_path / "ctx.vtk", meta, binary=binary)
+        sphgpu.write_vtk_fields(tmp_path / "host.vtk", meta, 3, g, binary=binary)
+        assert (tmp_path / "ctx.vtk").read_bytes() == (tmp_path / "host.vtk").read_bytes()
+    a.write_snapshot_hdf5(tmp_path / "snap.h5", meta, compression=6)
+    assert sphgpu.hdf5_row_count(tmp_path / "snap.h5") == f.shape[1]
+    b = sphgpu.Context(p, f.shape[1])
+    m2 = b.read_snapshot_hdf5(tmp_path / "snap.h5")
+    assert m2.step == 2 and m2.time_code == a.time and b.time == a.time
+    assert sphgpu.output_meta_json(m2) == sphgpu.output_meta_json(meta)
+    gb = b.download()
+    for r in COLS:
+        np.testing.assert_array_equal(gb[r], g[r], err_msg=str(r))

@@ -1,5 +1,6 @@
-"""The C-ABI library loads and exports every entry point include/sphgpu.h
-declares (no compute calls: this runs without a GPU)."""
+"""The C-ABI libraries load and export every entry point include/sphgpu.h
+and include/sphgpu_hdf5.h declare (no compute calls: this runs without a
+GPU)."""
 from __future__ import annotations
 
 import ctypes
@@ -12,10 +13,11 @@ import ics
 import sphgpu
 
 HEADER = Path(__file__).resolve().parents[1] / "include" / "sphgpu.h"
+HDF5_HEADER = Path(__file__).resolve().parents[1] / "include" / "sphgpu_hdf5.h"
 
 
-def declared_entry_points() -> set[str]:
-    text = HEADER.read_text()
+def declared_entry_points(header: Path = HEADER) -> set[str]:
+    text = header.read_text()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     return set(re.findall(r"\b(sphgpu_[a-z0-9_]+)\s*\(", text))
 
@@ -27,6 +29,15 @@ def test_library_exports_every_declared_symbol():
     missing = sorted(n for n in names if not hasattr(lib, n))
     assert not missing, missing
     assert set(sphgpu.EXPORTED) == names
+
+
+@pytest.mark.skipif(not sphgpu.HDF5_LIB_PATH.exists(), reason="libsphgpu_hdf5.so not built (no HDF5 C library)")
+def test_hdf5_library_exports_every_declared_symbol():
+    lib = sphgpu.load_hdf5_library()
+    names = declared_entry_points(HDF5_HEADER)
+    missing = sorted(n for n in names if not hasattr(lib, n))
+    assert not missing, missing
+    assert set(sphgpu.EXPORTED_HDF5) == names
 
 
 def test_params_struct_layout_matches_header():
