@@ -59,6 +59,20 @@ template <typename T> struct PreArgs {
     int *fl_flag;                 // set when the derived list cannot be used
 };
 
+// neighbour loops handle candidates in groups of this many
+constexpr int kGroup = 4;
+
+// single components of a particle record: the mass (velocity record, w) and
+// the specific internal energy (thermo record, w)
+template <typename V4> SPH_DEV auto rec_mass(const V4 *rv, int j) -> decltype(rv->w)
+{
+    return reinterpret_cast<const decltype(rv->w) *>(rv)[4 * (size_t)j + 3];
+}
+template <typename V4> SPH_DEV auto rec_u(const V4 *rt, int j) -> decltype(rt->w)
+{
+    return reinterpret_cast<const decltype(rt->w) *>(rt)[4 * (size_t)j + 3];
+}
+
 template <int D, typename T>
 SPH_DEV T rec_r2(const typename V4T<T>::type &pj, const T xi[3], const Box &b, T d[3])
 {
@@ -109,6 +123,8 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
     const V4 *__restrict__ RV = a.rv.r[R_VELM];
     const V4 *__restrict__ RT = a.rv.r[R_THERMO];
     const int nc = valid ? a.cnt[i] : 0;
+    // this particle's candidate column: entry q at pcol[q * kWave]
+    const int *const pcol = a.idx + list_slot(i, 0, a.cap);
     const V4 pi_ = RP[i], vi_ = RV[i], ti_ = RT[i];
     const T xi[3] = {pi_.x, pi_.y, pi_.z};
     const T vi[3] = {vi_.x, vi_.y, vi_.z};
@@ -130,14 +146,28 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
         for (; it < 10;) {
             const T hb = h;
             Acc<T> dn, ddn;
-            for (int q = 0; q < nc; ++q) {
-                const int j = a.idx[list_slot(i, q, a.cap)];
-                T d[3];
-                const T r = tsqrt(rec_r2<D, T>(RP[j], xi, a.box, d));
-                if (r >= h) continue;
-                const T wt = (TYPE == SPHGPU_DISPH || TYPE == SPHGPU_GDISPH) ? T(1) : RV[j].w;
-                dn.add(wt * Kern<D, K, T>::w(r, h));
-                ddn.add(wt * Kern<D, K, T>::dhw(r, h));
+            // candidates in groups of kGroup: the group's index, position
+            // and mass loads are issued together, so a group waits for
+            // memory once; the sums run in candidate order
+            for (int q0 = 0; q0 < nc; q0 += kGroup) {
+                int jg[kGroup];
+                V4 pg[kGroup];
+                T wg[kGroup];
+#pragma unroll
+                for (int u = 0; u < kGroup; ++u) jg[u] = pcol[min(q0 + u, nc - 1) * kWave];
+#pragma unroll
+                for (int u = 0; u < kGroup; ++u) pg[u] = RP[jg[u]];
+#pragma unroll
+                for (int u = 0; u < kGroup; ++u)
+                    wg[u] = (TYPE == SPHGPU_DISPH || TYPE == SPHGPU_GDISPH) ? T(1) : rec_mass(RV, jg[u]);
+#pragma unroll
+                for (int u = 0; u < kGroup; ++u) {
+                    T d[3];
+                    const T r = tsqrt(rec_r2<D, T>(pg[u], xi, a.box, d));
+                    if (q0 + u >= nc || r >= h) continue;
+                    dn.add(wg[u] * Kern<D, K, T>::w(r, h));
+                    ddn.add(wg[u] * Kern<D, K, T>::dhw(r, h));
+                }
             }
             const T dens = dn.get(), ddens = ddn.get();
             const T f = dens * powh<D>(h) - b;
@@ -191,11 +221,37 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
         const float padX = (float)((Xb + a.root[3]) * 3.814697265625e-06);
         z2 = padX * padX;
     }
-    for (int q = 0; q < nc; ++q) {
-        const int j = a.idx[list_slot(i, q, a.cap)];
+    for (int q0 = 0; q0 < nc; q0 += kGroup) {
+    // a group of candidates: indices and positions first, then the records
+    // of the members, each set of loads issued together
+    int jg[kGroup];
+    V4 pg[kGroup], vg[kGroup];
+    T cg[kGroup], ug[kGroup];
+    bool memb[kGroup];
+#pragma unroll
+    for (int u = 0; u < kGroup; ++u) jg[u] = pcol[min(q0 + u, nc - 1) * kWave];
+#pragma unroll
+    for (int u = 0; u < kGroup; ++u) pg[u] = RP[jg[u]];
+#pragma unroll
+    for (int u = 0; u < kGroup; ++u) {
         T d[3];
-        const T r2 = rec_r2<D, T>(RP[j], xi, a.box, d);
-        if (!member(j, r2)) continue;
+        memb[u] = q0 + u < nc && member(jg[u], rec_r2<D, T>(pg[u], xi, a.box, d));
+    }
+#pragma unroll
+    for (int u = 0; u < kGroup; ++u) {
+        if (!memb[u]) continue;
+        vg[u] = RV[jg[u]];
+        const V4 t = RT[jg[u]];
+        cg[u] = t.z;
+        ug[u] = t.w;
+    }
+#pragma unroll
+    for (int u = 0; u < kGroup; ++u) {
+        const int q = q0 + u;
+        if (!memb[u]) continue;
+        const int j = jg[u];
+        T d[3];
+        const T r2 = rec_r2<D, T>(pg[u], xi, a.box, d);
         const unsigned long long bit = 1ull << (q & 63);
 #pragma unroll
         for (int b = 0; b < 4; ++b) mb[b] |= ((q >> 6) == b) ? bit : 0ull;
@@ -217,11 +273,11 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
         }
         const T r = tsqrt(r2);
         const T wk = Kern<D, K, T>::w(r, h);
-        const V4 vj = RV[j];
+        const V4 vj = vg[u];
         const T mj = vj.w;
         if (TYPE == SPHGPU_DISPH || TYPE == SPHGPU_GDISPH) {
             const T dh = Kern<D, K, T>::dhw(r, h);
-            const T uj = RT[j].w;
+            const T uj = ug[u];
             dens_a.add(mj * wk);
             n_a.add(wk);
             pres_a.add(mj * uj * wk);
@@ -238,9 +294,10 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
             T dot = T(0);
 #pragma unroll
             for (int k = 0; k < D; ++k) dot = (k == 0) ? d[k] * (vi[k] - vjv[k]) : dot + d[k] * (vi[k] - vjv[k]);
-            const T vs = ci + RT[j].z - tdiv(T(3.0) * dot, r);
+            const T vs = ci + cg[u] - tdiv(T(3.0) * dot, r);
             if (vs > vsig) vsig = vs;
         }
+    }
     }
     const T dens = dens_a.get();
     const T gm1 = T(a.gamma - 1.0);
@@ -302,26 +359,45 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
     // the first such pass stores row r of the force list for all lanes at
     // once.  The particle itself contributes nothing to these sums and is
     // skipped.  Past 256 candidates the member test is repeated.
-    const int *const pcol = a.idx + list_slot(i, 0, a.cap);
     int *const frow = (fl && !fl_rows) ? a.fl_idx + list_slot(i, 0, a.fl_cap) : nullptr;
     bool rows_pending = frow != nullptr;
+    // body(j, d, r2, v_j (velocity, mass), u_j); members in groups of
+    // kGroup with their loads issued together
     auto members = [&](auto &&body) {
         const bool write = rows_pending;
         rows_pending = false;
         int wd = 0;
         unsigned long long m = am[0];
-        for (int r = 0; r < na_bits; ++r) {
-            while (m == 0ull && wd < 3) {
-                ++wd;
-                m = (wd == 1) ? am[1] : (wd == 2) ? am[2] : am[3];
+        for (int r0 = 0; r0 < na_bits; r0 += kGroup) {
+            int jg[kGroup];
+#pragma unroll
+            for (int u = 0; u < kGroup; ++u) {
+                while (m == 0ull && wd < 3) {
+                    ++wd;
+                    m = (wd == 1) ? am[1] : (wd == 2) ? am[2] : am[3];
+                }
+                // past the last member: repeat the previous one (not used)
+                const int qc = m ? (wd << 6) + __ffsll((long long)m) - 1 : -1;
+                m &= m - 1ull;
+                jg[u] = qc >= 0 ? pcol[qc * kWave] : (u ? jg[u - 1] : i);
             }
-            const int qc = (wd << 6) + __ffsll((long long)m) - 1;
-            m &= m - 1ull;
-            const int j = pcol[qc * kWave];
-            if (write && r < a.fl_cap) frow[r * kWave] = j;
-            T d[3];
-            const T r2 = rec_r2<D, T>(RP[j], xi, a.box, d);
-            body(j, d, r2);
+            V4 pg[kGroup], vg[kGroup];
+            T ug[kGroup];
+#pragma unroll
+            for (int u = 0; u < kGroup; ++u) {
+                pg[u] = RP[jg[u]];
+                vg[u] = RV[jg[u]];
+                ug[u] = rec_u(RT, jg[u]);
+            }
+#pragma unroll
+            for (int u = 0; u < kGroup; ++u) {
+                const int r = r0 + u;
+                if (r >= na_bits) break;
+                if (write && r < a.fl_cap) frow[r * kWave] = jg[u];
+                T d[3];
+                const T r2 = rec_r2<D, T>(pg[u], xi, a.box, d);
+                body(jg[u], d, r2, vg[u], ug[u]);
+            }
         }
         for (int q = 256; q < nc; ++q) {
             const int j = pcol[q * kWave];
@@ -329,7 +405,7 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
             T d[3];
             const T r2 = rec_r2<D, T>(RP[j], xi, a.box, d);
             if (!member(j, r2)) continue;
-            body(j, d, r2);
+            body(j, d, r2, RV[j], rec_u(RT, j));
         }
     };
 
@@ -337,12 +413,11 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
         // MUSCL gradients, g_pre_interaction.cpp:112-136 (GDISPH:
         // gd_pre_interaction.cpp:125-148)
         T dd[3] = {0, 0, 0}, du[3] = {0, 0, 0}, dv[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-        members([&](int j, const T d[3], T r2) {
+        members([&](int, const T d[3], T r2, const V4 &vj, T uj) {
             const T r = tsqrt(r2);
             const T cw = Kern<D, K, T>::dwc(r, h);
-            const V4 vj = RV[j];
             const T mj = vj.w;
-            const T mdu = mj * (RT[j].w - ui);
+            const T mdu = mj * (uj - ui);
             const T vjv[3] = {vj.x, vj.y, vj.z};
             T mdv[3];
 #pragma unroll
@@ -378,7 +453,7 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
         for (int q = 0; q < 16; ++q) a.rec_w[R_G0 + (q >> 2)][4 * (size_t)i + (q & 3)] = g[q];
     }
     if (TYPE == SPHGPU_GSPH) {
-        if (rows_pending) members([](int, const T *, T) {});
+        if (rows_pending) members([](int, const T *, T, const V4 &, T) {});
         return;
     }
 
@@ -391,10 +466,9 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
     const bool disph_w = (TYPE == SPHGPU_DISPH) || (TYPE == SPHGPU_GDISPH && !do_bal);
     if (do_bal || a.td_av) {
         T div = T(0), rot[3] = {0, 0, 0};
-        members([&](int j, const T d[3], T r2) {
+        members([&](int, const T d[3], T r2, const V4 &vj, T uj) {
             const T r = tsqrt(r2);
             const T cw = Kern<D, K, T>::dwc(r, h);
-            const V4 vj = RV[j];
             const T vjv[3] = {vj.x, vj.y, vj.z};
             T dw[3], vij[3];
 #pragma unroll
@@ -402,7 +476,7 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
                 dw[k] = d[k] * cw;
                 vij[k] = vi[k] - vjv[k];
             }
-            const T wt = disph_w ? vj.w * RT[j].w : vj.w;
+            const T wt = disph_w ? vj.w * uj : vj.w;
             T dot = T(0);
 #pragma unroll
             for (int k = 0; k < D; ++k) dot = (k == 0) ? vij[k] * dw[k] : dot + vij[k] * dw[k];
@@ -446,7 +520,7 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
             a.rec_w[R_AUX][4 * (size_t)i + 2] = alpha;
         }
     }
-    if (rows_pending) members([](int, const T *, T) {});
+    if (rows_pending) members([](int, const T *, T, const V4 &, T) {});
 }
 
 // ---- force pair lists from the pre-interaction lists ----------------------------
