@@ -41,7 +41,10 @@ SPH_DEV int xcd_block(int b, int nb)
 }
 
 template <typename T> SPH_DEV T tabs(T x) { return x < T(0) ? -x : x; }
-SPH_DEV float tsqrt(float x) { return __builtin_sqrtf(x); }
+// FP32 square root: the hardware v_sqrt_f32 (1 ulp) without the compiler's
+// rescaling of denormal inputs, which pair quantities (r^2 of distinct
+// particles, pressures and densities) never are; sqrt(0) = 0 stays exact
+SPH_DEV float tsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 SPH_DEV double tsqrt(double x) { return __builtin_sqrt(x); }
 SPH_DEV float texp(float x) { return __expf(x); }
 SPH_DEV double texp(double x) { return exp(x); }
