@@ -98,7 +98,7 @@ SPH_DEV float abs_min_image_f(float d, int periodic, float L)
 // candidates were moved to their image nearest the group centre.
 template <int D, int IS_IJ, bool SHIFTED>
 SPH_DEV void test_range(const float4 *srec, int t0, int t1, const float xf[3], const float Lf[3], float Rf,
-                        float R2lo, float R2hi, float blo, float bhi, float z2, unsigned &km, unsigned &ku)
+                        float R2lo, float R2hi, float delta, float z2, unsigned &km, unsigned &ku)
 {
     unsigned kk = 0, uu = 0;
 #pragma unroll 4
@@ -122,8 +122,8 @@ SPH_DEV void test_range(const float4 *srec, int t0, int t1, const float xf[3], c
         float lo2 = R2lo, hi2 = R2hi;
         if (IS_IJ) {
             const float Hf = fmaxf(Rf, c.w);
-            lo2 = Hf * Hf * blo;
-            hi2 = Hf * Hf * bhi;
+            lo2 = Hf * Hf * (1.0f - delta);
+            hi2 = Hf * Hf * (1.0f + delta);
         }
         const unsigned bit = 1u << (t - t0);
         bool keep = r2f < lo2;
@@ -140,6 +140,37 @@ SPH_DEV void test_range(const float4 *srec, int t0, int t1, const float xf[3], c
     }
     km = kk;
     ku = uu;
+}
+
+// The same test for the radius search (is_ij = 0) on candidates stored
+// relative to the group centre g: record (y, |y|^2) with y = x_j - g (its
+// image nearest g), so that r^2 = |y_i|^2 + |y_j|^2 - 2 y_i . y_j is one add
+// and D fused multiply-adds (m2y = -2 y_i, yyi = |y_i|^2).  The candidates
+// run from last to first and each decision is shifted in at bit 0 with a
+// carry-in add (v_addc_co_u32 d, d + d + vcc), one instruction per mask; km
+// gets r^2 < R2lo, kle gets !(r^2 > R2hi), the band is kle & ~km.
+template <int D>
+SPH_DEV void test_range_expand(const float4 *srec, int t0, int t1, const float m2y[3], float yyi, float R2lo,
+                               float R2hi, unsigned &km, unsigned &ku)
+{
+    unsigned kk = 0, kl = 0;
+#pragma unroll 4
+    for (int t = t1 - 1; t >= t0; --t) {
+        const float4 c = srec[t];
+        float r2f = yyi + c.w;
+        if (D > 2) r2f = __builtin_fmaf(m2y[2], c.z, r2f);
+        if (D > 1) r2f = __builtin_fmaf(m2y[1], c.y, r2f);
+        r2f = __builtin_fmaf(m2y[0], c.x, r2f);
+        asm("v_cmp_gt_f32 vcc, %[lo], %[r2]\n\t"
+            "v_addc_co_u32 %[kk], vcc, %[kk], %[kk], vcc\n\t"
+            "v_cmp_nlt_f32 vcc, %[hi], %[r2]\n\t"
+            "v_addc_co_u32 %[kl], vcc, %[kl], %[kl], vcc"
+            : [kk] "+v"(kk), [kl] "+v"(kl)
+            : [r2] "v"(r2f), [lo] "v"(R2lo), [hi] "v"(R2hi)
+            : "vcc");
+    }
+    km = kk;
+    ku = kl & ~kk;
 }
 
 template <int D, int IS_IJ>
@@ -163,8 +194,6 @@ __global__ __launch_bounds__(256, 4) void k_search(SearchArgs a)
     // is at most this wide relative to H^2)
     const float delta = mk / Rf + kMarginC;
     const bool fast = (Rf * Rf > 1e-30f) && (delta < 0.5f);
-    const float blo = 1.0f - delta, bhi = 1.0f + delta;
-    const float R2lo = Rf * Rf * blo, R2hi = Rf * Rf * bhi;
     float Lf[3] = {1.f, 1.f, 1.f};
 #pragma unroll
     for (int k = 0; k < D; ++k) Lf[k] = a.box.rangef[k];
@@ -198,6 +227,13 @@ __global__ __launch_bounds__(256, 4) void k_search(SearchArgs a)
     // be pre-shifted to the minimum image seen from the group centre
     float g[3] = {0, 0, 0};
     int shift = 1;
+    // radius search with candidates relative to g (EXPAND): this lane's
+    // -2 (x_i - g), |x_i - g|^2 and its band widened for that form
+    const bool expand_ok = !IS_IJ;
+    bool expand = false;
+    float m2y[3] = {0, 0, 0}, yyi = 0.f;
+    // this piece's band bounds on r^2 (direct or relative form)
+    float R2lo = 0.f, R2hi = 0.f;
 
     // the FP64 predicate (the reference's) for candidate j; the particle's
     // own FP64 position and radius are reloaded here (a rare path) rather
@@ -228,7 +264,17 @@ __global__ __launch_bounds__(256, 4) void k_search(SearchArgs a)
         // `shift` the position is moved to its image nearest the group centre,
         // which for every pair closer than the search radius is the minimum
         // image seen from the group's particles
-        if (lane < nb) {
+        if (lane < nb && expand) {
+            const float4 pj = a.posf[cand];
+            float q[3] = {pj.x, pj.y, pj.z}, y[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+            for (int k = 0; k < D; ++k) {
+                const float d = q[k] - g[k];
+                y[k] = !a.box.periodic ? d : (fabsf(d) <= 0.5f * Lf[k] ? d : (d > 0.f ? d - Lf[k] : d + Lf[k]));
+            }
+            srec[lane] = make_float4(y[0], y[1], y[2], __builtin_fmaf(y[0], y[0], __builtin_fmaf(y[1], y[1], y[2] * y[2])));
+            sidx[lane] = cand;
+        } else if (lane < nb) {
             float4 pj = a.posf[cand];
             if (a.box.periodic && shift) {
                 float q[3] = {pj.x, pj.y, pj.z};
@@ -245,12 +291,15 @@ __global__ __launch_bounds__(256, 4) void k_search(SearchArgs a)
         wave_lds_fence();
         unsigned km0 = 0, km1 = 0, ku0 = 0, ku1 = 0;
         const int n0 = min(nb, 32);
-        if (!a.box.periodic || shift) {
-            test_range<D, IS_IJ, true>(srec, 0, n0, xf, Lf, Rf, R2lo, R2hi, blo, bhi, z2, km0, ku0);
-            if (nb > 32) test_range<D, IS_IJ, true>(srec, 32, nb, xf, Lf, Rf, R2lo, R2hi, blo, bhi, z2, km1, ku1);
+        if (expand) {
+            test_range_expand<D>(srec, 0, n0, m2y, yyi, R2lo, R2hi, km0, ku0);
+            if (nb > 32) test_range_expand<D>(srec, 32, nb, m2y, yyi, R2lo, R2hi, km1, ku1);
+        } else if (!a.box.periodic || shift) {
+            test_range<D, IS_IJ, true>(srec, 0, n0, xf, Lf, Rf, R2lo, R2hi, delta, z2, km0, ku0);
+            if (nb > 32) test_range<D, IS_IJ, true>(srec, 32, nb, xf, Lf, Rf, R2lo, R2hi, delta, z2, km1, ku1);
         } else {
-            test_range<D, IS_IJ, false>(srec, 0, n0, xf, Lf, Rf, R2lo, R2hi, blo, bhi, z2, km0, ku0);
-            if (nb > 32) test_range<D, IS_IJ, false>(srec, 32, nb, xf, Lf, Rf, R2lo, R2hi, blo, bhi, z2, km1, ku1);
+            test_range<D, IS_IJ, false>(srec, 0, n0, xf, Lf, Rf, R2lo, R2hi, delta, z2, km0, ku0);
+            if (nb > 32) test_range<D, IS_IJ, false>(srec, 32, nb, xf, Lf, Rf, R2lo, R2hi, delta, z2, km1, ku1);
         }
         unsigned long long km = ((unsigned long long)km1 << 32) | km0;
         unsigned long long ku = ((unsigned long long)ku1 << 32) | ku0;
@@ -340,6 +389,41 @@ __global__ __launch_bounds__(256, 4) void k_search(SearchArgs a)
 #pragma unroll
             for (int k = 0; k < D; ++k) ok = ok && (rmax + hb[k] + 2.f * padX < 0.49f * Lf[k]);
             shift = __builtin_amdgcn_readfirstlane(ok ? 1 : 0);  // wave-uniform by construction
+        }
+        // relative form: y_i = x_i - g.  Its FP32 sum |y_i|^2 + |y_j|^2 -
+        // 2 y_i . y_j rounds by at most 6 u (|y_i| + |y_j|)^2 (first order),
+        // and a candidate whose decision can change has r < 2R, so |y_j| <
+        // |y_i| + 2R: the band grows by 8 u (2 |y_i| + 2R)^2 / R^2.  The
+        // components y carry at most 7 u X of rounding (the positions, the
+        // differences and the box range), inside the 15 u X the search's
+        // band allows.
+        expand = expand_ok && (!a.box.periodic || shift);
+        if (expand) {
+            float yy = 0.f;
+#pragma unroll
+            for (int k = 0; k < D; ++k) {
+                const float y = xf[k] - g[k];
+                m2y[k] = -2.0f * y;
+                yy = __builtin_fmaf(y, y, yy);
+            }
+            yyi = yy;
+            // the r > 2R part of that bound needs |y_i| well below 2^-24 / (6u)
+            // times R; pieces far wider than their radii keep the direct form
+            expand = __ballot(mine && !(yy <= 4096.0f * Rf * Rf)) == 0ull;
+        }
+        if (expand) {
+            const float yy = yyi;
+            const float ext = 2.0f * sqrtf(yy) + 2.0f * Rf;
+            const float de = delta + 8.0f * 5.9604644775390625e-08f * (ext * ext) / (Rf * Rf);
+            R2lo = Rf * Rf * (1.0f - de);
+            R2hi = Rf * Rf * (1.0f + de);
+            if (!(fast && de < 0.5f)) {
+                R2lo = -1.0f;  // no FP32 decision: every candidate is in the band
+                R2hi = __builtin_inff();
+            }
+        } else {
+            R2lo = Rf * Rf * (1.0f - delta);
+            R2hi = Rf * Rf * (1.0f + delta);
         }
 
         // reach test of node k against this piece's box (lane-parallel)
