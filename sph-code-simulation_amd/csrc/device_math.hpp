@@ -273,6 +273,36 @@ template <typename T> struct TwoShock {
         T pn = p + tdiv((ustar_r - ustar_l) * (z_l * z_r), z_r - z_l);
         return (T(1e-25) < pn) ? pn : T(1e-25);  // std::max(smallp, p)
     }
+    // step() that also returns its Newton factor F = z_l z_r / (z_r - z_l)
+    // (the FP32 form, which refines the last iterate in FP64 with F; the
+    // update is written p + (u*_R - u*_L) F)
+    SPH_DEV T step_f(T p, T &F) const
+    {
+        T w_l = T(1.0) + tdiv(gamma1 * (p - p_l), p_l);
+        w_l = cl * tsqrt(w_l);
+        T w_r = T(1.0) + tdiv(gamma1 * (p - p_r), p_r);
+        w_r = cr * tsqrt(w_r);
+        T z_l = T(4.0) * V_l * w_l * w_l;
+        z_l = tdiv(-z_l * w_l, z_l - gamma2 * (p - p_l));
+        T z_r = T(4.0) * V_r * w_r * w_r;
+        z_r = tdiv(z_r * w_r, z_r - gamma2 * (p - p_r));
+        const T ustar_l = u_l - tdiv(p - p_l, w_l);
+        const T ustar_r = u_r + tdiv(p - p_r, w_r);
+        F = tdiv(z_l * z_r, z_r - z_l);
+        const T pn = p + (ustar_r - ustar_l) * F;
+        return (T(1e-25) < pn) ? pn : T(1e-25);
+    }
+    SPH_DEV int iterate_f(T &p, T &F) const
+    {
+        int it = 0;
+        for (int iter = 0; iter < 20; ++iter) {
+            const T p_old = p;
+            p = step_f(p, F);
+            it = iter + 1;
+            if (tdiv(tabs(p - p_old), p) < T(1e-6)) break;
+        }
+        return it;
+    }
     // the Newton factor z_l z_r / (z_r - z_l) of step() at p
     SPH_DEV T newton_factor(T p) const
     {
@@ -310,6 +340,23 @@ template <typename T> struct TwoShock {
         return T(0.5) * (ustar_l + ustar_r);
     }
 };
+
+// refine_step64 with v* = (u*_L + u*_R) / 2 of the same FP64 evaluation at
+// p32 (g_fluid_force.cpp:288-301 evaluates v* at the final iterate)
+SPH_DEV double refine_step64_v(const float l[4], const float r[4], double gamma, double gamma1, float p32, float F,
+                               double &vstar)
+{
+    const double p = p32;
+    const double u_l = l[0], rho_l = l[1], p_l = l[2];
+    const double u_r = r[0], rho_r = r[1], p_r = r[2];
+    const double w_l = __builtin_sqrt(gamma * rho_l * (p_l + gamma1 * (p - p_l)));
+    const double w_r = __builtin_sqrt(gamma * rho_r * (p_r + gamma1 * (p - p_r)));
+    const double ustar_l = u_l - (p - p_l) / w_l;
+    const double ustar_r = u_r + (p - p_r) / w_r;
+    vstar = 0.5 * (ustar_l + ustar_r);
+    const double pn = p + (ustar_r - ustar_l) * (double)F;
+    return (1e-25 < pn) ? pn : 1e-25;  // std::max(smallp, p)
+}
 
 // One FP64 Newton step of the two-shock iteration from an FP32 iterate p32:
 // the residual u*_R(p) - u*_L(p), which cancels near the root, is evaluated

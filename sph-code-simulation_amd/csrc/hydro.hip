@@ -906,6 +906,44 @@ SPH_DEV int solve_pair(const T l[4], const T r[4], T gamma, int refine64, const 
     T p;
     int it;
     fallback = 0;
+    if (sizeof(T) == 4 && refine64) {
+        // FP32 iterations that keep their last Newton factor F; the final
+        // step is taken in FP64 with that F (the factor at the last iterate's
+        // start, within the iteration tolerance of the one at p: it only
+        // scales the small correction), and v* comes from the same FP64
+        // evaluation
+        T F = T(0);
+        if (guess) {
+            const T g = *guess;
+            bool conv = false;
+            it = 0;
+            if (g > T(1e-25) && g < T(1e30)) {
+                p = g;
+                for (int k = 0; k < 4; ++k) {
+                    const T p_old = p;
+                    p = ts.step_f(p, F);
+                    ++it;
+                    if (tdiv(tabs(p - p_old), p) < T(1e-6)) { conv = true; break; }
+                }
+            }
+            if (!conv) {
+                fallback = 1;
+                p = ts.plain_guess();
+                it += ts.iterate_f(p, F);
+            }
+        } else {
+            p = ts.plain_guess();
+            it = ts.iterate_f(p, F);
+        }
+        const float lf[4] = {(float)l[0], (float)l[1], (float)l[2], (float)l[3]};
+        const float rf[4] = {(float)r[0], (float)r[1], (float)r[2], (float)r[3]};
+        const double g64 = (double)gamma;
+        double v64 = 0.0;
+        const double pd = refine_step64_v(lf, rf, g64, 0.5 * (1.0 + g64) / g64, (float)p, (float)F, v64);
+        pstar = (T)pd;
+        vstar = (T)v64;
+        return it;
+    }
     if (guess) {
         // learned starting point: Newton from it for at most 4 steps; if that
         // does not meet the solver's tolerance, restart from the plain guess
