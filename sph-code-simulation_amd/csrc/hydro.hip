@@ -560,13 +560,27 @@ __global__ __launch_bounds__(256) void k_flist_scatter(FlistArgs a)
     for (int k = 0; k < D; ++k) xi64[k] = a.x64[k][i];
     const float xf[3] = {pi.x, pi.y, pi.z};
     const float mk = fp32_margin_k(a.root, D);
-    for (int q = 0; q < na; ++q) {
-        const int j = a.idx[list_slot(i, q, a.cap)];
-        const float4 pj = a.posf[j];
+    const int *const pcol = a.idx + list_slot(i, 0, a.cap);
+    // members in groups of kGroup, each group's loads issued together
+    for (int q0 = 0; q0 < na; q0 += kGroup) {
+    int jg[kGroup];
+    float4 pg[kGroup];
+    double hg[kGroup];
+#pragma unroll
+    for (int u = 0; u < kGroup; ++u) jg[u] = pcol[min(q0 + u, na - 1) * kWave];
+#pragma unroll
+    for (int u = 0; u < kGroup; ++u) pg[u] = a.posf[jg[u]];
+#pragma unroll
+    for (int u = 0; u < kGroup; ++u) hg[u] = pg[u].w > pi.w ? 0.0 : a.sml[jg[u]];
+#pragma unroll
+    for (int u = 0; u < kGroup; ++u) {
+        if (q0 + u >= na) break;
+        const int j = jg[u];
+        const float4 pj = pg[u];
         // FP32 rounding is monotone: h_j > h_i when the rounded values say so,
         // and then r < h_i <= h_j puts i in the list of j already
         if (pj.w > pi.w) continue;
-        const double hj = a.sml[j];
+        const double hj = hg[u];
         if (!(hj < hi)) continue;
         bool in_j;  // r_ij < h_j
         if (FP64) {
@@ -584,6 +598,7 @@ __global__ __launch_bounds__(256) void k_flist_scatter(FlistArgs a)
             atomicOr(a.flag, 1);
             atomicAdd(stat_ptr(a.stats, 19), 1ull);
         }
+    }
     }
 }
 
@@ -997,15 +1012,21 @@ SPH_DEV void force_gsph_slots(const ForceArgs<T> &a, int i, int nc, int nmax, co
     const V4 *__restrict__ RV = a.rv.r[R_VELM];
     const V4 *__restrict__ RT = a.rv.r[R_THERMO];
     long long iters = 0, wave_max_sum = 0, calls = 0, fallbacks = 0, learned = 0;
+    // the next slot's pair index is loaded one slot ahead, so its record
+    // gathers do not wait for it
+    const int *const pcol = a.idx + list_slot(i, 0, a.cap);
+    int j_next = nc > 0 ? pcol[0] : 0;
     for (int q = 0; q < nmax; ++q) {
         const bool act = q < nc;
         int it = 0;
         int j = 0;
+        const int j_cur = j_next;
+        if (q + 1 < nc) j_next = pcol[(q + 1) * kWave];
         T d[3] = {0, 0, 0}, e[3] = {0, 0, 0}, r = T(1), r_inv = T(1);
         T left[4] = {0, 1, 1, 1}, right[4] = {0, 1, 1, 1};
         T rhoj = T(1), hj = T(1), mj = T(0);
         if (act) {
-            j = a.idx[list_slot(i, q, a.cap)];
+            j = j_cur;
             const V4 pj = RP[j], vjr = RV[j], tj = RT[j];
             r = tsqrt(rec_r2<D, T>(pj, xi, a.box, d));
             r_inv = tdiv(T(1.0), r);
