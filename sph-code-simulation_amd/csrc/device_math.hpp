@@ -104,6 +104,24 @@ template <typename T> SPH_DEV T pow3(T x) { return x * x * x; }
 template <typename T> SPH_DEV T pow5(T x) { return x * x * x * x * x; }
 template <typename T> SPH_DEV T pow6(T x) { return x * x * x * x * x * x; }
 
+// (a)_+ as the kernels write it, 0.5 (a + |a|): for a > 0 the sum 2a and the
+// halving are exact and for a < 0 the sum is +0, so it equals max(a, 0) bit
+// for bit (one instruction; a = c - q is never -0)
+SPH_DEV float pos_part(float a) { return __builtin_fmaxf(a, 0.f); }
+SPH_DEV double pos_part(double a) { return __builtin_fmax(a, 0.0); }
+// ((|2 - q| + 2) - q) / 2, the cubic spline's dhw form: the sum rounds in
+// another order, so FP64 keeps the reference's expression; FP32 uses the max
+template <typename T> SPH_DEV T pos_part_dhw(T c, T q) { return (tabs(c - q) + c - q) * T(0.5); }
+template <> SPH_DEV float pos_part_dhw<float>(float c, float q) { return fmaxf(c - q, 0.f); }
+
+// |minimum-image d| in FP32 for r^2 only: min(|d|, L - |d|) equals |d - L
+// rint(d / L)| for |d| < L (the box keeps positions there), two instructions
+SPH_DEV float abs_rij_axis(float xi, float xj, const Box &b, int k)
+{
+    const float d = fabsf(xi - xj);
+    return b.periodic ? fminf(d, fabsf(b.rangef[k] - d)) : d;
+}
+
 // include/kernel/kernel_function.hpp:8-16 and pre_interaction.cpp:217-225
 template <int D, typename T> SPH_DEV T powh(T h) { return D == 1 ? h : D == 2 ? h * h : h * h * h; }
 template <int D, typename T> SPH_DEV T powh_m1(T h) { return D == 1 ? T(1) : D == 2 ? h : h * h; }
@@ -135,8 +153,7 @@ template <int D, typename T> struct Kern<D, SPHGPU_CUBIC_SPLINE, T> {
         const T sig = T(Sigma<D>::cubic);
         const T h_ = h * T(0.5);
         const T q = tdiv(r, h_);
-        return tdiv(sig, powh<D>(h_)) * (T(0.25) * pow3(T(0.5) * (T(2.0) - q + tabs(T(2.0) - q))) -
-                                    pow3(T(0.5) * (T(1.0) - q + tabs(T(1.0) - q))));
+        return tdiv(sig, powh<D>(h_)) * (T(0.25) * pow3(pos_part(T(2.0) - q)) - pow3(pos_part(T(1.0) - q)));
     }
     static SPH_DEV T dwc(T r, T h)
     {
@@ -145,7 +162,7 @@ template <int D, typename T> struct Kern<D, SPHGPU_CUBIC_SPLINE, T> {
         const T h_ = h * T(0.5);
         const T q = tdiv(r, h_);
         return tdiv(-sig, powh<D>(h_) * h_ * r) *
-               (T(0.75) * sqr(T(0.5) * (T(2.0) - q + tabs(T(2.0) - q))) - T(3.0) * sqr(T(0.5) * (T(1.0) - q + tabs(T(1.0) - q))));
+               (T(0.75) * sqr(pos_part(T(2.0) - q)) - T(3.0) * sqr(pos_part(T(1.0) - q)));
     }
     static SPH_DEV T dhw(T r, T h)
     {
@@ -153,8 +170,8 @@ template <int D, typename T> struct Kern<D, SPHGPU_CUBIC_SPLINE, T> {
         const T h_ = h * T(0.5);
         const T q = tdiv(r, h_);
         return tdiv(T(0.5) * sig, powh<D>(h_) * h_) *
-               (sqr((tabs(T(2.0) - q) + T(2.0) - q) * T(0.5)) * (T((3. + D) * 0.25) * q - T(0.5 * D)) +
-                sqr((tabs(T(1.0) - q) + T(1.0) - q) * T(0.5)) * (T(-3.0 - D) * q + T(D)));
+               (sqr(pos_part_dhw(T(2.0), q)) * (T((3. + D) * 0.25) * q - T(0.5 * D)) +
+                sqr(pos_part_dhw(T(1.0), q)) * (T(-3.0 - D) * q + T(D)));
     }
 };
 
@@ -163,19 +180,19 @@ template <int D, typename T> struct Kern<D, SPHGPU_WENDLAND, T> {
     {
         const T sig = T(Sigma<D>::c4);
         const T q = tdiv(r, h);
-        return tdiv(sig, powh<D>(h)) * pow6(T(0.5) * (T(1.0) - q + tabs(T(1.0) - q))) * (T(1.0) + T(6.0) * q + T(35.0 / 3.0) * q * q);
+        return tdiv(sig, powh<D>(h)) * pow6(pos_part(T(1.0) - q)) * (T(1.0) + T(6.0) * q + T(35.0 / 3.0) * q * q);
     }
     static SPH_DEV T dwc(T r, T h)
     {
         const T sig = T(Sigma<D>::c4);
         const T q = tdiv(r, h);
-        return tdiv(T(-56.0 / 3.0) * sig, powh<D>(h) * sqr(h)) * pow5(T(0.5) * (T(1.0) - q + tabs(T(1.0) - q))) * (T(1.0) + T(5.0) * q);
+        return tdiv(T(-56.0 / 3.0) * sig, powh<D>(h) * sqr(h)) * pow5(pos_part(T(1.0) - q)) * (T(1.0) + T(5.0) * q);
     }
     static SPH_DEV T dhw(T r, T h)
     {
         const T sig = T(Sigma<D>::c4);
         const T q = tdiv(r, h);
-        return tdiv(-sig, powh<D>(h) * h * T(3.0)) * pow5(T(0.5) * (T(1.0) - q + tabs(T(1.0) - q))) *
+        return tdiv(-sig, powh<D>(h) * h * T(3.0)) * pow5(pos_part(T(1.0) - q)) *
                (T(3.0 * D) + T(15.0 * D) * q + T(-56.0 + 17.0 * D) * q * q - T(35.0 * (8.0 + D)) * pow3(q));
     }
 };

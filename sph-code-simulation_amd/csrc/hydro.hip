@@ -162,8 +162,20 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
                     wg[u] = (TYPE == SPHGPU_DISPH || TYPE == SPHGPU_GDISPH) ? T(1) : rec_mass(RV, jg[u]);
 #pragma unroll
                 for (int u = 0; u < kGroup; ++u) {
-                    T d[3];
-                    const T r = tsqrt(rec_r2<D, T>(pg[u], xi, a.box, d));
+                    T r2;
+                    if constexpr (sizeof(T) == 4) {
+                        // only |r| is needed here
+                        r2 = T(0);
+#pragma unroll
+                        for (int k = 0; k < D; ++k) {
+                            const T dk = abs_rij_axis(xi[k], k == 0 ? pg[u].x : k == 1 ? pg[u].y : pg[u].z, a.box, k);
+                            r2 = (k == 0) ? dk * dk : r2 + dk * dk;
+                        }
+                    } else {
+                        T d[3];
+                        r2 = rec_r2<D, T>(pg[u], xi, a.box, d);
+                    }
+                    const T r = tsqrt(r2);
                     if (q0 + u >= nc || r >= h) continue;
                     dn.add(wg[u] * Kern<D, K, T>::w(r, h));
                     ddn.add(wg[u] * Kern<D, K, T>::dhw(r, h));
@@ -245,6 +257,7 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
         cg[u] = t.z;
         ug[u] = t.w;
     }
+
 #pragma unroll
     for (int u = 0; u < kGroup; ++u) {
         const int q = q0 + u;
@@ -810,8 +823,12 @@ __global__ __launch_bounds__(256) void k_force_sph(ForceArgs<T> a)
     const T gamma = T(a.gamma);
     Acc<T> ac[3], de;
     const int nc = a.cnt[i];
+    // the next pair index is loaded one pair ahead
+    const int *const pcol = a.idx + list_slot(i, 0, a.cap);
+    int j_next = nc > 0 ? pcol[0] : 0;
     for (int q = 0; q < nc; ++q) {
-        const int j = a.idx[list_slot(i, q, a.cap)];
+        const int j = j_next;
+        if (q + 1 < nc) j_next = pcol[(q + 1) * kWave];
         const V4 pj = RP[j], vj = RV[j], tj = RT[j], aj = RA[j];
         T d[3];
         const T r = tsqrt(rec_r2<D, T>(pj, xi, a.box, d));
