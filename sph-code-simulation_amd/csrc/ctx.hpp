@@ -87,6 +87,10 @@ struct Tree {
     // radix sort scratch
     unsigned int *hist = nullptr;
     int hist_blocks = 0;
+    // node count and the partial-sort check, read together by the host
+    int *sync_words = nullptr;         // [0] node count, [1] prefix run > leaf
+    bool full_sort = false;            // a partial sort failed its check once
+    long long partial_sorts = 0, full_sorts = 0;
     unsigned int *scan_tmp = nullptr;  // tile sums of the multi-block scans
 };
 

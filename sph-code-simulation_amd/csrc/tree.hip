@@ -130,7 +130,26 @@ __global__ void k_keys(const double *__restrict__ x, const double *__restrict__ 
     perm[i] = i;
 }
 
+__global__ void k_zero_words(int *__restrict__ w, int n)
+{
+    if ((int)threadIdx.x < n) w[threadIdx.x] = 0;
+}
+
 // ---- 2. radix sort -------------------------------------------------------------
+// Partial sort check: the keys were sorted on their top `cut`-free bits only
+// (the low radix passes skipped), i.e. on the first D digits; that order is
+// the full order as far as the tree goes when no cell of depth D holds more
+// than `leaf` particles (no node deeper than level D + 1 exists, and every
+// node boundary the build uses lies at depth <= D).  flag[1] gets 1 if some
+// cell of depth D holds more.
+__global__ void k_prefix_runs(const unsigned long long *__restrict__ key, int n, int leaf, int cut,
+                              int *__restrict__ flag)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool bad = i + leaf < n && (key[i] >> cut) == (key[i + leaf] >> cut);
+    if (__ballot(bad) && (threadIdx.x & (kWave - 1)) == 0) atomicOr(flag + 1, 1);
+}
+
 __global__ __launch_bounds__(kSortThreads) void k_radix_hist(const unsigned long long *__restrict__ key, int n,
                                                             int shift, unsigned int *__restrict__ hist, int nblocks)
 {
@@ -685,6 +704,7 @@ int tree_alloc(sphgpu_ctx *c)
     t.hist_blocks = (n + kSortTile - 1) / kSortTile;
     bool ok = al((void **)&t.key, 8ull * n) && al((void **)&t.key_alt, 8ull * n) && al((void **)&t.perm, 4ull * n) &&
               al((void **)&t.perm_alt, 4ull * n) && al((void **)&t.kdig, n) && al((void **)&t.lam, n) &&
+              al((void **)&t.sync_words, 4ull * 2) &&
               al((void **)&t.leaf_end, 4ull * n) && al((void **)&t.ncount, 4ull * (n + 1)) &&
               al((void **)&t.nbase, 4ull * (n + 1)) && al((void **)&t.start, 4ull * t.node_cap) &&
               al((void **)&t.end, 4ull * t.node_cap) && al((void **)&t.skip, 4ull * t.node_cap) &&
@@ -717,7 +737,7 @@ int tree_alloc(sphgpu_ctx *c)
 void tree_free(sphgpu_ctx *c)
 {
     Tree &t = c->tree;
-    void *ps[] = {t.key, t.key_alt, t.perm, t.perm_alt, t.kdig, t.lam, t.leaf_end, t.ncount, t.nbase, t.start,
+    void *ps[] = {t.key, t.key_alt, t.perm, t.perm_alt, t.kdig, t.lam, t.sync_words, t.leaf_end, t.ncount, t.nbase, t.start,
                   t.end, t.skip, t.level, t.center, t.edge, t.ks, t.mass, t.com, t.bmax1, t.bmax2, t.bsum1,
                   t.bsum2, t.root, t.hist, t.nbox, t.nlink, t.scan_tmp, t.nhalf, t.nchild, t.child, t.ccount};
     for (void *p : ps)
@@ -781,12 +801,26 @@ int tree_build(sphgpu_ctx *c)
                        t.key, t.perm);
     time_end(c);
 
-    // stable LSD radix sort over the dim*maxTreeLevel key bits
+    // stable LSD radix sort over the dim*maxTreeLevel key bits.  Large
+    // 2D/3D sets sort on the top 12 digits only (the low passes are skipped)
+    // and check that no depth-12 cell holds more than leafParticleNumber
+    // particles; otherwise the tree is built again with the full sort (and
+    // every later build sorts fully).
     time_begin(c, "tree_sort");
     const int key_bits = sh.bits * sh.ml;
     const int passes = (key_bits + 7) / 8;
     const int nb = t.hist_blocks;
-    for (int p = 0; p < passes; ++p) {
+    constexpr int kSortedDigits = 12;
+    int skip = 0, cut = 0;
+    if (!t.full_sort && n >= 65536 && c->dim >= 2) {
+        const int keep = (c->dim * kSortedDigits + 7) / 8;
+        if (keep < passes) {
+            skip = passes - keep;
+            cut = key_bits - c->dim * ((key_bits - 8 * skip) / c->dim);
+        }
+    }
+    hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, c->stream, t.sync_words, 2);
+    for (int p = skip; p < passes; ++p) {
         const int shift = 8 * p;
         hipLaunchKernelGGL(k_radix_hist, dim3(nb), dim3(kSortThreads), 0, c->stream, (const unsigned long long *)t.key, n,
                            shift, t.hist, nb);
@@ -797,6 +831,9 @@ int tree_build(sphgpu_ctx *c)
         std::swap(t.key, t.key_alt);
         std::swap(t.perm, t.perm_alt);
     }
+    if (skip)
+        hipLaunchKernelGGL(k_prefix_runs, dim3(grid_for(n, 256)), dim3(256), 0, c->stream,
+                           (const unsigned long long *)t.key, n, sh.leaf, cut, t.sync_words);
     time_end(c);
     HIPCHK(c, hipGetLastError());
 
@@ -812,9 +849,24 @@ int tree_build(sphgpu_ctx *c)
                        t.lam, t.leaf_end, t.ncount);
     scan_counts(c->stream, (const unsigned *)t.ncount, (unsigned *)t.nbase, n, t.scan_tmp, true);
     HIPCHK(c, hipGetLastError());
-    int total = 0;
-    HIPCHK(c, hipMemcpyAsync(&total, t.nbase + n, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    int words[2] = {0, 0};
+    HIPCHK(c, hipMemcpyAsync(t.sync_words, t.nbase + n, sizeof(int), hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(words, t.sync_words, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    const int total = words[0];
+    if (skip) {
+        if (words[1]) {
+            // a depth-12 cell holds more than a leaf: sort fully from here on
+            // (the state is already in the partial order; the full build
+            // starts from it)
+            time_end(c);
+            t.full_sort = true;
+            return tree_build(c);
+        }
+        ++t.partial_sorts;
+    } else {
+        ++t.full_sorts;
+    }
     if (total > t.node_cap) {
         t.valid = false;
         return fail(c, SPHGPU_E_NO_FREE_NODE, "There is no free node.");

@@ -75,6 +75,40 @@ def test_tree_topology(case):
         np.testing.assert_allclose(d["com"], rec[:, 8:11], rtol=0, atol=1e-12)
 
 
+@pytest.mark.parametrize("cluster", [False, True])
+def test_tree_large_partial_sort(cluster):
+    """Above 65536 particles in 2D/3D the key sort skips its low radix passes
+    and checks that no depth-12 cell holds more than a leaf; the tree must be
+    the reference's either way.  The clustered case packs 40 particles into a
+    cell of edge 1e-7 (more than a leaf below depth 12), so the check fails
+    and the build sorts fully."""
+    p, f = ics.sedov_3d(n1=48)
+    f = f.copy()
+    if cluster:
+        rng = np.random.default_rng(3)
+        f[ics.F_POS:ics.F_POS + 3, :40] = 0.123 + 1e-7 * rng.random((3, 40))
+    f[ics.F_SML] = guess_h(p, f)
+    o = Oracle(p, f)
+    o.make_tree()
+    rec, mem = o.tree_dump()
+    g = gpu_ctx(p, f, pair_precision=ics.PAIR_FP64)
+    for _ in range(2):  # a second build starts from the first one's order
+        g.build_tree()
+        d = g.tree_dump()
+        assert len(rec) == len(d["level"])
+        np.testing.assert_array_equal(rec[:, 0].astype(int), d["level"])
+        np.testing.assert_array_equal(rec[1:, 1].astype(int), d["count"][1:])
+        np.testing.assert_array_equal(rec[:, 2].astype(int), d["is_leaf"])
+        np.testing.assert_array_equal(rec[:, 3:6], d["center"])
+        np.testing.assert_array_equal(rec[:, 6], d["edge"])
+        for k in np.nonzero(rec[:, 2] == 1)[0]:
+            cnt, off = int(rec[k, 1]), int(rec[k, 12])
+            np.testing.assert_array_equal(np.sort(mem[off:off + cnt]),
+                                          np.sort(d["members"][d["first"][k]:d["first"][k] + cnt]))
+    if cluster:
+        assert d["level"].max() > 13
+
+
 @pytest.mark.parametrize("case", list(SMALL))
 def test_leaf_members(case):
     p, f = SMALL[case]()
