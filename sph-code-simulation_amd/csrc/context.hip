@@ -410,8 +410,8 @@ static int integrate_impl(sphgpu_ctx *c, int nsteps)
         if ((rc = timestep(c))) return rc;
         if ((rc = predict(c))) return rc;
         if ((rc = tree_build(c))) return rc;
-        if ((rc = pre_interaction(c))) return rc;
-        if ((rc = fluid_force(c))) return rc;
+        if ((rc = pre_interaction_deferred(c))) return rc;
+        if ((rc = fluid_force(c))) return rc;  // settles the deferred search first
         if ((rc = gravity(c))) return rc;
         if ((rc = correct(c))) return rc;
         if ((rc = advance_time(c))) return rc;

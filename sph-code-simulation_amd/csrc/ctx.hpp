@@ -164,6 +164,9 @@ struct sphgpu_ctx {
     // search): 0 none, 1 built but not yet checked, 2 valid
     int flist = 0;
     long long flist_searches = 0;  // FluidForce calls that searched instead
+    // the pre-interaction search's overflow flag has not been read yet (the
+    // time step reads it together with the force-list flag, one copy)
+    bool pre_search_pending = false;
 
     // timing
     bool timing = false;
@@ -212,12 +215,20 @@ int tree_dump(sphgpu_ctx *c, int64_t *n_nodes, int32_t *level, int32_t *count, i
 int lists_alloc(sphgpu_ctx *c, Lists &l, int cap);
 void lists_free(Lists &l);
 // is_ij = 0: radius hs[i] (r2 < hs^2);  is_ij = 1: sqrt(r2) < max(h_i, h_j), r2 != 0
-int search(sphgpu_ctx *c, Lists &l, int is_ij, const double *radius);
+// defer: launch only; the overflow flag is read later (pre_resolve)
+int search(sphgpu_ctx *c, Lists &l, int is_ij, const double *radius, bool defer = false);
 // settles a pending force list (flist == 1) from the device flag: 0 = use it
 int flist_check(sphgpu_ctx *c);
 
 // hydro.hip
 int pre_interaction(sphgpu_ctx *c);
+// the same with its search's overflow check left to pre_resolve (not with
+// the time-dependent viscosity, whose alpha update cannot be repeated)
+int pre_interaction_deferred(sphgpu_ctx *c);
+// reads a deferred search's overflow flag (and the force-list flag in the
+// same copy); after an overflow the lists grow and the pre-interaction runs
+// again from the same search radii
+int pre_resolve(sphgpu_ctx *c);
 int fluid_force(sphgpu_ctx *c);
 int riemann_batch(sphgpu_ctx *c, int solver, int precision, int n, const double *left,
                   const double *right, double *pstar, double *vstar, int32_t *iters,

@@ -675,9 +675,11 @@ __global__ void k_hguess(int n, const double *__restrict__ mass, const double *_
     hs[i] = pow(nn * mass[i] / (dens[i] * A), 1.0 / D) * kappa;
 }
 
-__global__ void k_minbits_init(unsigned long long *m)
+// resets the h / v_sig minimum copies and, when given, the force-list flag
+__global__ void k_minbits_init(unsigned long long *m, int *flag)
 {
     if (threadIdx.x < kStatCopies) m[threadIdx.x] = 0x7fefffffffffffffull;  // DBL_MAX
+    if (flag && threadIdx.x == 0) *flag = 0;
 }
 
 __global__ void k_store_hvs(const unsigned long long *m, double *dscal)
@@ -1495,7 +1497,34 @@ template <typename T> void dispatch_force(sphgpu_ctx *c)
 
 } // namespace
 
-int pre_interaction(sphgpu_ctx *c)
+static int pre_core(sphgpu_ctx *c, bool defer, bool guess_h);
+
+int pre_interaction(sphgpu_ctx *c) { return pre_core(c, false, true); }
+
+int pre_interaction_deferred(sphgpu_ctx *c) { return pre_core(c, !c->prm.use_time_dependent_av, true); }
+
+int pre_resolve(sphgpu_ctx *c)
+{
+    if (!c->pre_search_pending) return 0;
+    c->pre_search_pending = false;
+    int w[3] = {1, 0, 1};
+    HIPCHK(c, hipMemcpyAsync(w, c->dflag, sizeof(w), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (w[0]) {
+        // the search overflowed: grow its lists (the reference's capacity is
+        // nn * 1000) and run the pre-interaction again from the same radii
+        const long long want = 2ll * c->lpre.cap;
+        if (want > 1000ll * c->prm.neighbor_number)
+            return fail(c, SPHGPU_E_NEIGHBOR_OVERFLOW, "Neighbor list overflow: increase neighbor_list_size in defines.hpp");
+        int rc = lists_alloc(c, c->lpre, (int)want);
+        if (rc) return fail(c, rc, "neighbour list reallocation failed");
+        return pre_core(c, false, false);
+    }
+    if (c->flist == 1) c->flist = w[2] ? 0 : 2;
+    return 0;
+}
+
+static int pre_core(sphgpu_ctx *c, bool defer, bool guess_h)
 {
     if (!c->tree.valid) return fail(c, SPHGPU_E_STATE, "pre_interaction needs a built tree");
     int rc;
@@ -1520,13 +1549,14 @@ int pre_interaction(sphgpu_ctx *c)
         HIPCHK(c, hipStreamSynchronize(c->stream));
         c->pre_first = false;
     }
-    time_begin(c, "pre_hguess");
-    hguess(c, kappa, c->hs);
-    time_end(c);
-    if ((rc = search(c, c->lpre, 0, c->hs))) return rc;
-    hipLaunchKernelGGL(k_minbits_init, dim3(1), dim3(kStatCopies), 0, c->stream, c->dmin);
+    if (guess_h) {
+        time_begin(c, "pre_hguess");
+        hguess(c, kappa, c->hs);
+        time_end(c);
+    }
+    if ((rc = search(c, c->lpre, 0, c->hs, defer))) return rc;
     const bool fl = c->prm.iterative_sml != 0;
-    if (fl) HIPCHK(c, hipMemsetAsync(c->dflag + 2, 0, sizeof(int), c->stream));
+    hipLaunchKernelGGL(k_minbits_init, dim3(1), dim3(kStatCopies), 0, c->stream, c->dmin, fl ? c->dflag + 2 : nullptr);
     time_begin(c, "pre_interaction");
     if (c->prm.pair_precision == SPHGPU_PAIR_FP64) dispatch_pre<double>(c);
     else dispatch_pre<float>(c);

@@ -633,6 +633,10 @@ void lists_free(Lists &l)
 
 int flist_check(sphgpu_ctx *c)
 {
+    if (c->pre_search_pending) {
+        const int rc = pre_resolve(c);
+        if (rc) return rc;
+    }
     if (c->flist == 1) {
         int bad = 1;
         HIPCHK(c, hipMemcpyAsync(&bad, c->dflag + 2, sizeof(int), hipMemcpyDeviceToHost, c->stream));
@@ -642,12 +646,14 @@ int flist_check(sphgpu_ctx *c)
     return c->flist == 2 ? 0 : 1;
 }
 
-int search(sphgpu_ctx *c, Lists &l, int is_ij, const double *radius)
+int search(sphgpu_ctx *c, Lists &l, int is_ij, const double *radius, bool defer)
 {
     Tree &t = c->tree;
     if (!t.valid) return fail(c, SPHGPU_E_STATE, "neighbour search without a tree");
     if (&l == &c->lforce) c->flist = 0;  // the derived force list is overwritten
-    for (int attempt = 0; attempt < 6; ++attempt) {
+    // the list grows by doubling until the candidates fit; the reference's
+    // own capacity, nn * 1000 entries, bounds it (the loop ends there)
+    for (;;) {
         SearchArgs a;
         a.n = c->n;
         a.n_nodes = t.n_nodes;
@@ -672,6 +678,10 @@ int search(sphgpu_ctx *c, Lists &l, int is_ij, const double *radius)
 #undef LAUNCH
         time_end(c);
         HIPCHK(c, hipGetLastError());
+        if (defer) {
+            c->pre_search_pending = true;
+            return 0;
+        }
         int over = 0;
         HIPCHK(c, hipMemcpyAsync(&over, c->dflag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -683,7 +693,6 @@ int search(sphgpu_ctx *c, Lists &l, int is_ij, const double *radius)
         int rc = lists_alloc(c, l, (int)want);
         if (rc) return fail(c, rc, "neighbour list reallocation failed");
     }
-    return fail(c, SPHGPU_E_NEIGHBOR_OVERFLOW, "Neighbor list overflow");
 }
 
 } // namespace sphgpu

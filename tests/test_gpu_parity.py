@@ -388,3 +388,41 @@ def test_sedov_energy_drift_gate(prec):
     # sum_i m_i |v_i| <= sqrt(2 K M) (total mass M = 1): momentum relative to it
     pscale = np.sqrt(2.0 * eg[0] * f[ics.F_MASS].sum())
     assert np.all(np.abs(eg[4:7]) <= (1e-12 if prec == ics.PAIR_FP64 else 1e-5) * pscale)
+
+
+def test_deferred_search_overflow_reruns_pre():
+    """In a time step the pre-interaction search's overflow flag is read
+    together with the force-list flag, after the pre-interaction kernels ran
+    on the (capacity-clipped) lists.  An overflow then grows the lists and
+    runs the pre-interaction again from the same search radii, so the step
+    equals the one the separate entry points take (each checks its search at
+    once).  Densities cut 100x on 20 particles make their next h guess 4.6x
+    larger: about 100x the usual candidate count, far above the lists'
+    capacity (and below the reference's limit of nn * 1000)."""
+    p, f = ics.sedov_3d(n1=24)
+    ctxs = []
+    for _ in range(2):
+        c = gpu_ctx(p, f)
+        c.initialize()
+        g = c.download()
+        g[ics.F_DENS, :20] *= 0.01
+        c.upload(g, mask=1 << ics.F_DENS)
+        c.reset_stats()
+        ctxs.append(c)
+    a, b = ctxs
+    a.integrate(1)
+    b.timestep()
+    b.predict()
+    b.build_tree()
+    b.pre_interaction()
+    b.fluid_force()
+    b.gravity()
+    b.correct()
+    ga, gb = a.download(), b.download()
+    for r in (ics.F_POS, ics.F_VEL, ics.F_ENE, ics.F_DENS, ics.F_PRES, ics.F_SML, ics.F_ACC, ics.F_NEIGHBOR):
+        np.testing.assert_array_equal(ga[r], gb[r], err_msg=str(r))
+    sa, sb = a.stats(), b.stats()
+    # both searched with the same growing capacities; the step's
+    # pre-interaction kernel ran twice (clipped lists, then full ones)
+    assert sa["pre_candidates"] == sb["pre_candidates"] > 20 * 5000
+    assert sa["pre_newton_iters"] > sb["pre_newton_iters"]
