@@ -350,11 +350,18 @@ constexpr int kMaxLevels = 64;
 __global__ void k_lambda(const unsigned char *__restrict__ kdig, TreeShape sh, unsigned char *__restrict__ lam,
                          int *__restrict__ leaf_end, int *__restrict__ ncount)
 {
+    // per thread and level t: the nearest boundary with kdig <= t to the
+    // left (offset <= 0) and to the right (offset >= 1) of i, in LDS (a
+    // level-indexed table in registers would go to scratch memory)
+    extern __shared__ short s_lr[];
+    const int B = blockDim.x;
+    short *const nl = s_lr + threadIdx.x;
+    short *const nr = s_lr + (size_t)(sh.ml + 1) * B + threadIdx.x;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= sh.n) return;
     const int ML = sh.ml;
-    int nl[kMaxLevels + 2], nr[kMaxLevels + 2];
-    for (int t = 0; t <= ML; ++t) { nl[t] = -1; nr[t] = 0x7fffffff; }
+    constexpr short kNone = -32768, kNoneR = 32767;
+    for (int t = 0; t <= ML; ++t) { nl[t * B] = kNone; nr[t * B] = kNoneR; }
     // nearest boundary to the left with kdig <= t (position of the group start)
     {
         int m = ML + 2;
@@ -362,7 +369,7 @@ __global__ void k_lambda(const unsigned char *__restrict__ kdig, TreeShape sh, u
         for (int j = i; j >= lo && m > 0; --j) {
             const int kj = kdig[j];
             if (kj < m) {
-                for (int t = kj; t < m && t <= ML; ++t) nl[t] = j;
+                for (int t = kj; t < m && t <= ML; ++t) nl[t * B] = (short)(j - i);
                 m = kj;
             }
         }
@@ -374,7 +381,7 @@ __global__ void k_lambda(const unsigned char *__restrict__ kdig, TreeShape sh, u
         for (int j = i + 1; j <= hi && m > 0; ++j) {
             const int kj = (j == sh.n) ? 0 : (int)kdig[j];
             if (kj < m) {
-                for (int t = kj; t < m && t <= ML; ++t) nr[t] = j;
+                for (int t = kj; t < m && t <= ML; ++t) nr[t * B] = (short)(j - i);
                 m = kj;
             }
         }
@@ -383,9 +390,10 @@ __global__ void k_lambda(const unsigned char *__restrict__ kdig, TreeShape sh, u
     int lend = -1;
     for (int l = 2; l <= ML; ++l) {
         const int t = l - 1;
-        if (nl[t] >= 0 && nr[t] != 0x7fffffff && nr[t] - nl[t] <= sh.leaf) {
+        const int a = nl[t * B], b = nr[t * B];
+        if (a != kNone && b != kNoneR && b - a <= sh.leaf) {
             L = l;
-            lend = nr[t];
+            lend = i + b;
             break;
         }
     }
@@ -845,7 +853,8 @@ int tree_build(sphgpu_ctx *c)
     time_begin(c, "tree_nodes");
     hipLaunchKernelGGL(k_kdig, dim3(grid_for(n, 256)), dim3(256), 0, c->stream, (const unsigned long long *)t.key, sh,
                        t.kdig);
-    hipLaunchKernelGGL(k_lambda, dim3(grid_for(n, 128)), dim3(128), 0, c->stream, (const unsigned char *)t.kdig, sh,
+    hipLaunchKernelGGL(k_lambda, dim3(grid_for(n, 128)), dim3(128), sizeof(short) * 2 * (size_t)(sh.ml + 1) * 128,
+                       c->stream, (const unsigned char *)t.kdig, sh,
                        t.lam, t.leaf_end, t.ncount);
     scan_counts(c->stream, (const unsigned *)t.ncount, (unsigned *)t.nbase, n, t.scan_tmp, true);
     HIPCHK(c, hipGetLastError());
