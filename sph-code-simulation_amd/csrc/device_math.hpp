@@ -358,18 +358,31 @@ template <typename T> struct TwoShock {
     }
 };
 
+// 1 / sqrt(x) in FP64: the hardware estimate (v_rsq_f64, about 2^-22
+// relative) and two Newton steps y (3 - x y^2) / 2, each squaring the
+// relative error; a non-positive x gives NaN or inf as sqrt and division do
+SPH_DEV double rsqrt64(double x)
+{
+    double y = __builtin_amdgcn_rsq(x);
+    y = y * (1.5 - 0.5 * x * y * y);
+    y = y * (1.5 - 0.5 * x * y * y);
+    return y;
+}
+
 // refine_step64 with v* = (u*_L + u*_R) / 2 of the same FP64 evaluation at
-// p32 (g_fluid_force.cpp:288-301 evaluates v* at the final iterate)
+// p32 (g_fluid_force.cpp:288-301 evaluates v* at the final iterate); the
+// quotients (p - p0) / w are products with 1 / w = rsqrt(gamma rho (p0 +
+// gamma1 (p - p0))) (a few ulp, far below the FP32 p* this refines)
 SPH_DEV double refine_step64_v(const float l[4], const float r[4], double gamma, double gamma1, float p32, float F,
                                double &vstar)
 {
     const double p = p32;
     const double u_l = l[0], rho_l = l[1], p_l = l[2];
     const double u_r = r[0], rho_r = r[1], p_r = r[2];
-    const double w_l = __builtin_sqrt(gamma * rho_l * (p_l + gamma1 * (p - p_l)));
-    const double w_r = __builtin_sqrt(gamma * rho_r * (p_r + gamma1 * (p - p_r)));
-    const double ustar_l = u_l - (p - p_l) / w_l;
-    const double ustar_r = u_r + (p - p_r) / w_r;
+    const double iw_l = rsqrt64(gamma * rho_l * (p_l + gamma1 * (p - p_l)));
+    const double iw_r = rsqrt64(gamma * rho_r * (p_r + gamma1 * (p - p_r)));
+    const double ustar_l = u_l - (p - p_l) * iw_l;
+    const double ustar_r = u_r + (p - p_r) * iw_r;
     vstar = 0.5 * (ustar_l + ustar_r);
     const double pn = p + (ustar_r - ustar_l) * (double)F;
     return (1e-25 < pn) ? pn : 1e-25;  // std::max(smallp, p)
