@@ -456,9 +456,12 @@ template <typename T> struct Acc {
     SPH_DEV Acc() : s(T(0)), c(T(0)) {}
     SPH_DEV void add(T x)
     {
+        // Knuth's TwoSum: e = (s - (t - z)) + (x - z) is the exact rounding
+        // error of t = s + x whatever the magnitudes (the same value as
+        // Neumaier's ordered form, without its comparison)
         const T t = s + x;
-        if (tabs(s) >= tabs(x)) c += (s - t) + x;
-        else c += (x - t) + s;
+        const T z = t - s;
+        c += (s - (t - z)) + (x - z);
         s = t;
     }
     SPH_DEV T get() const { return s + c; }

@@ -1176,10 +1176,13 @@ SPH_DEV void force_gsph_slots(const ForceArgs<T> &a, int i, int nc, int nmax, co
         }
         if (SOLVER == SPHGPU_RIEMANN_ITERATIVE) {
             iters += it;
-            int m = it;
-            for (int off = kWave / 2; off > 0; off >>= 1) {
-                const int o = __shfl_xor(m, off, kWave);
-                m = o > m ? o : m;
+            // the wavefront's largest count (at most 24: 5 bits), bit by bit
+            // from the top with ballots
+            int m = 0;
+#pragma unroll
+            for (int b = 4; b >= 0; --b) {
+                const int cand = m | (1 << b);
+                if (__ballot(it >= cand)) m = cand;
             }
             wave_max_sum += m;
         }
