@@ -106,9 +106,13 @@ __global__ void k_bbox_final(const double *__restrict__ part, int nblocks, int d
 // ---- 1. keys ----------------------------------------------------------------
 __global__ void k_keys(const double *__restrict__ x, const double *__restrict__ y, const double *__restrict__ z,
                        const double *__restrict__ root, TreeShape sh, unsigned long long *__restrict__ key,
-                       int *__restrict__ perm)
+                       int *__restrict__ perm, int *__restrict__ sync_words)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {  // node count, partial-sort check
+        sync_words[0] = 0;
+        sync_words[1] = 0;
+    }
     if (i >= sh.n) return;
     double pos[3] = {x[i], sh.dim > 1 ? y[i] : 0.0, sh.dim > 2 ? z[i] : 0.0};
     double c[3] = {root[0], root[1], root[2]};
@@ -130,25 +134,14 @@ __global__ void k_keys(const double *__restrict__ x, const double *__restrict__ 
     perm[i] = i;
 }
 
-__global__ void k_zero_words(int *__restrict__ w, int n)
-{
-    if ((int)threadIdx.x < n) w[threadIdx.x] = 0;
-}
 
 // ---- 2. radix sort -------------------------------------------------------------
-// Partial sort check: the keys were sorted on their top `cut`-free bits only
-// (the low radix passes skipped), i.e. on the first D digits; that order is
-// the full order as far as the tree goes when no cell of depth D holds more
-// than `leaf` particles (no node deeper than level D + 1 exists, and every
-// node boundary the build uses lies at depth <= D).  flag[1] gets 1 if some
-// cell of depth D holds more.
-__global__ void k_prefix_runs(const unsigned long long *__restrict__ key, int n, int leaf, int cut,
-                              int *__restrict__ flag)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool bad = i + leaf < n && (key[i] >> cut) == (key[i + leaf] >> cut);
-    if (__ballot(bad) && (threadIdx.x & (kWave - 1)) == 0) atomicOr(flag + 1, 1);
-}
+// Partial sort check (in k_kdig): the keys were sorted on their bits above
+// `cut` only (the low radix passes skipped), i.e. on the first D digits;
+// that order is the full order as far as the tree goes when no cell of depth
+// D holds more than `leaf` particles (no node deeper than level D + 1 exists,
+// and every node boundary the build uses lies at depth <= D).  flag[1] gets 1
+// if some cell of depth D holds more.
 
 __global__ __launch_bounds__(kSortThreads) void k_radix_hist(const unsigned long long *__restrict__ key, int n,
                                                             int shift, unsigned int *__restrict__ hist, int nblocks)
@@ -333,9 +326,16 @@ __global__ void k_gather_int2(const int *__restrict__ a, int *__restrict__ ad, c
 }
 
 // ---- 4. first differing level ---------------------------------------------------
-__global__ void k_kdig(const unsigned long long *__restrict__ key, TreeShape sh, unsigned char *__restrict__ kdig)
+// also the partial-sort check (cut >= 0): flag[1] = 1 when particles i and
+// i + leaf share the sorted top digits (key >> cut), see k_prefix_runs
+__global__ void k_kdig(const unsigned long long *__restrict__ key, TreeShape sh, unsigned char *__restrict__ kdig,
+                       int cut, int *__restrict__ flag)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (cut >= 0) {
+        const bool bad = i + sh.leaf < sh.n && (key[i] >> cut) == (key[i + sh.leaf] >> cut);
+        if (__ballot(bad) && (threadIdx.x & (kWave - 1)) == 0) atomicOr(flag + 1, 1);
+    }
     if (i >= sh.n) return;
     if (i == 0) { kdig[0] = 0; return; }
     const unsigned long long x = key[i - 1] ^ key[i];
@@ -806,7 +806,7 @@ int tree_build(sphgpu_ctx *c)
         hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(64), 0, c->stream, (const double *)c->red, nbb, c->dim, t.root);
     }
     hipLaunchKernelGGL(k_keys, dim3(grid_for(n, 256)), dim3(256), 0, c->stream, px, py, pz, (const double *)t.root, sh,
-                       t.key, t.perm);
+                       t.key, t.perm, t.sync_words);
     time_end(c);
 
     // stable LSD radix sort over the dim*maxTreeLevel key bits.  Large
@@ -827,7 +827,6 @@ int tree_build(sphgpu_ctx *c)
             cut = key_bits - c->dim * ((key_bits - 8 * skip) / c->dim);
         }
     }
-    hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, c->stream, t.sync_words, 2);
     for (int p = skip; p < passes; ++p) {
         const int shift = 8 * p;
         hipLaunchKernelGGL(k_radix_hist, dim3(nb), dim3(kSortThreads), 0, c->stream, (const unsigned long long *)t.key, n,
@@ -839,9 +838,6 @@ int tree_build(sphgpu_ctx *c)
         std::swap(t.key, t.key_alt);
         std::swap(t.perm, t.perm_alt);
     }
-    if (skip)
-        hipLaunchKernelGGL(k_prefix_runs, dim3(grid_for(n, 256)), dim3(256), 0, c->stream,
-                           (const unsigned long long *)t.key, n, sh.leaf, cut, t.sync_words);
     time_end(c);
     HIPCHK(c, hipGetLastError());
 
@@ -852,7 +848,7 @@ int tree_build(sphgpu_ctx *c)
 
     time_begin(c, "tree_nodes");
     hipLaunchKernelGGL(k_kdig, dim3(grid_for(n, 256)), dim3(256), 0, c->stream, (const unsigned long long *)t.key, sh,
-                       t.kdig);
+                       t.kdig, skip ? cut : -1, t.sync_words);
     hipLaunchKernelGGL(k_lambda, dim3(grid_for(n, 128)), dim3(128), sizeof(short) * 2 * (size_t)(sh.ml + 1) * 128,
                        c->stream, (const unsigned char *)t.kdig, sh,
                        t.lam, t.leaf_end, t.ncount);
