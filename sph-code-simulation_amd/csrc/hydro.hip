@@ -233,6 +233,12 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
         const float padX = (float)((Xb + a.root[3]) * 3.814697265625e-06);
         z2 = padX * padX;
     }
+    // second-order GSPH / GDISPH: the MUSCL gradient sums run in this member
+    // pass too (members other than the particle itself, in candidate order,
+    // the order of a separate member pass)
+    constexpr bool kGrad = TYPE == SPHGPU_GSPH || TYPE == SPHGPU_GDISPH;
+    const bool grad_on = kGrad && a.gsph2;
+    T gdd[3] = {0, 0, 0}, gdu[3] = {0, 0, 0}, gdv[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
     for (int q0 = 0; q0 < nc; q0 += kGroup) {
     // a group of candidates: indices and positions first, then the records
     // of the members, each set of loads issued together
@@ -309,6 +315,22 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
             for (int k = 0; k < D; ++k) dot = (k == 0) ? d[k] * (vi[k] - vjv[k]) : dot + d[k] * (vi[k] - vjv[k]);
             const T vs = ci + cg[u] - tdiv(T(3.0) * dot, r);
             if (vs > vsig) vsig = vs;
+            if (kGrad && grad_on) {
+                // g_pre_interaction.cpp:112-136 (GDISPH: gd_pre_interaction.cpp:125-148)
+                const T cw = Kern<D, K, T>::dwc(r, h);
+                const T mdu = mj * (ug[u] - ui);
+                T mdv[3];
+#pragma unroll
+                for (int b2 = 0; b2 < D; ++b2) mdv[b2] = mj * (vjv[b2] - vi[b2]);
+#pragma unroll
+                for (int k = 0; k < D; ++k) {
+                    const T dwk = d[k] * cw;
+                    gdd[k] += dwk * mj;
+                    gdu[k] += dwk * mdu;
+#pragma unroll
+                    for (int b2 = 0; b2 < D; ++b2) gdv[b2][k] += dwk * mdv[b2];
+                }
+            }
         }
     }
     }
@@ -422,28 +444,16 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
         }
     };
 
-    if ((TYPE == SPHGPU_GSPH || TYPE == SPHGPU_GDISPH) && a.gsph2) {
-        // MUSCL gradients, g_pre_interaction.cpp:112-136 (GDISPH:
-        // gd_pre_interaction.cpp:125-148)
-        T dd[3] = {0, 0, 0}, du[3] = {0, 0, 0}, dv[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-        members([&](int, const T d[3], T r2, const V4 &vj, T uj) {
-            const T r = tsqrt(r2);
-            const T cw = Kern<D, K, T>::dwc(r, h);
-            const T mj = vj.w;
-            const T mdu = mj * (uj - ui);
-            const T vjv[3] = {vj.x, vj.y, vj.z};
-            T mdv[3];
+    if (kGrad && grad_on) {
+        // MUSCL gradients (summed in the density pass)
+        T dd[3], du[3], dv[3][3];
 #pragma unroll
-            for (int b2 = 0; b2 < D; ++b2) mdv[b2] = mj * (vjv[b2] - vi[b2]);
+        for (int k = 0; k < 3; ++k) {
+            dd[k] = gdd[k];
+            du[k] = gdu[k];
 #pragma unroll
-            for (int k = 0; k < D; ++k) {
-                const T dwk = d[k] * cw;
-                dd[k] += dwk * mj;
-                du[k] += dwk * mdu;
-#pragma unroll
-                for (int b2 = 0; b2 < D; ++b2) dv[b2][k] += dwk * mdv[b2];
-            }
-        });
+            for (int b2 = 0; b2 < 3; ++b2) dv[b2][k] = gdv[b2][k];
+        }
         const T rho_inv = tdiv(T(1.0), dens);
         T g[16];
 #pragma unroll
@@ -466,7 +476,21 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
         for (int q = 0; q < 16; ++q) a.rec_w[R_G0 + (q >> 2)][4 * (size_t)i + (q & 3)] = g[q];
     }
     if (TYPE == SPHGPU_GSPH) {
-        if (rows_pending) members([](int, const T *, T, const V4 &, T) {});
+        if (rows_pending) {
+            // the force-list rows alone: row r of every lane = its r-th member
+            int wd = 0;
+            unsigned long long m = am[0];
+            const int nw = min(na_bits, a.fl_cap);
+            for (int r = 0; r < nw; ++r) {
+                while (m == 0ull && wd < 3) {
+                    ++wd;
+                    m = (wd == 1) ? am[1] : (wd == 2) ? am[2] : am[3];
+                }
+                const int qc = (wd << 6) + __ffsll((long long)m) - 1;
+                m &= m - 1ull;
+                frow[r * kWave] = pcol[qc * kWave];
+            }
+        }
         return;
     }
 
