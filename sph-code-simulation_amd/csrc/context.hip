@@ -409,7 +409,10 @@ static int integrate_impl(sphgpu_ctx *c, int nsteps)
     for (int s = 0; s < nsteps; ++s) {
         if ((rc = timestep(c))) return rc;
         if ((rc = predict(c))) return rc;
-        if ((rc = tree_build(c))) return rc;
+        c->in_step = true;
+        rc = tree_build(c);
+        c->in_step = false;
+        if (rc) return rc;
         if ((rc = pre_interaction_deferred(c))) return rc;
         if ((rc = fluid_force(c))) return rc;  // settles the deferred search first
         if ((rc = gravity(c))) return rc;

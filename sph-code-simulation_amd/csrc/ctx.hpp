@@ -167,6 +167,9 @@ struct sphgpu_ctx {
     // the pre-interaction search's overflow flag has not been read yet (the
     // time step reads it together with the force-list flag, one copy)
     bool pre_search_pending = false;
+    // set by the time-step loop around its tree build: fields that the step
+    // recomputes before reading them are not permuted
+    bool in_step = false;
 
     // timing
     bool timing = false;

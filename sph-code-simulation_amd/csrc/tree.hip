@@ -758,14 +758,20 @@ static int permute_state(sphgpu_ctx *c)
     const int n = c->n;
     const int *perm = c->tree.perm;
     std::vector<double **> fields;
+    // inside a time step the fields that the step recomputes before anyone
+    // reads them stay unpermuted: acc and du/dt (FluidForce assigns them)
+    // and the MUSCL gradients (the second-order pre-interaction does)
+    const bool light = c->in_step;
+    const bool gsph = c->prm.sph_type == SPHGPU_GSPH || c->prm.sph_type == SPHGPU_GDISPH;
     for (int k = 0; k < c->dim; ++k) {
         fields.push_back(&c->pos[k]);
         fields.push_back(&c->vel[k]);
         fields.push_back(&c->vel_p[k]);
-        fields.push_back(&c->acc[k]);
+        if (!light) fields.push_back(&c->acc[k]);
     }
-    for (int s = 0; s < S_NSCALAR; ++s) fields.push_back(&c->sc[s]);
-    if (c->prm.sph_type == SPHGPU_GSPH || c->prm.sph_type == SPHGPU_GDISPH) {
+    for (int s = 0; s < S_NSCALAR; ++s)
+        if (!(light && s == S_DENE)) fields.push_back(&c->sc[s]);
+    if (gsph && !(light && c->prm.gsph_2nd_order)) {
         for (int k = 0; k < c->dim; ++k) {
             fields.push_back(&c->grad[k]);
             fields.push_back(&c->grad[3 + k]);
