@@ -289,7 +289,9 @@ int refresh_mirrors(sphgpu_ctx *c)
     RecArgs a;
     a.n = c->n;
     const bool gsph2 = (c->prm.sph_type == SPHGPU_GSPH || c->prm.sph_type == SPHGPU_GDISPH) && c->prm.gsph_2nd_order;
-    a.nrec = gsph2 ? R_NREC : R_AUX + 1;
+    // the gradient records are rebuilt from the gradients unless the time
+    // step's pre-interaction is about to recompute both
+    a.nrec = (gsph2 && !c->in_step) ? R_NREC : R_AUX + 1;
     for (int f = 0; f < 4 * R_NREC; ++f) a.src[f] = nullptr;
     for (int k = 0; k < c->dim; ++k) {
         a.src[4 * R_POSH + k] = c->pos[k];
