@@ -239,6 +239,14 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
     constexpr bool kGrad = TYPE == SPHGPU_GSPH || TYPE == SPHGPU_GDISPH;
     const bool grad_on = kGrad && a.gsph2;
     T gdd[3] = {0, 0, 0}, gdu[3] = {0, 0, 0}, gdv[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    // and the Balsara / time-dependent AV sums (pre_interaction.cpp:115-161;
+    // GDISPH, gd_pre_interaction.cpp:150-197: weighted by mass, or by mass x
+    // u as in DISPH when the AV runs without Balsara)
+    constexpr bool kBal = TYPE != SPHGPU_GSPH;
+    const bool do_bal = a.balsara_on && D != 1;
+    const bool bal_on = kBal && (do_bal || a.td_av);
+    const bool disph_w = (TYPE == SPHGPU_DISPH) || (TYPE == SPHGPU_GDISPH && !do_bal);
+    T bdiv = T(0), brot[3] = {0, 0, 0};
     for (int q0 = 0; q0 < nc; q0 += kGroup) {
     // a group of candidates: indices and positions first, then the records
     // of the members, each set of loads issued together
@@ -331,6 +339,27 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
                     for (int b2 = 0; b2 < D; ++b2) gdv[b2][k] += dwk * mdv[b2];
                 }
             }
+            if (kBal && bal_on) {
+                const T cw = Kern<D, K, T>::dwc(r, h);
+                T dw[3], vij[3];
+#pragma unroll
+                for (int k = 0; k < D; ++k) {
+                    dw[k] = d[k] * cw;
+                    vij[k] = vi[k] - vjv[k];
+                }
+                const T wt = disph_w ? mj * ug[u] : mj;
+                T bdot = T(0);
+#pragma unroll
+                for (int k = 0; k < D; ++k) bdot = (k == 0) ? vij[k] * dw[k] : bdot + vij[k] * dw[k];
+                bdiv -= wt * bdot;
+                if (D == 3) {
+                    brot[0] += (vij[1] * dw[2 % D] - vij[2 % D] * dw[1 % D]) * wt;
+                    brot[1] += (vij[2 % D] * dw[0] - vij[0] * dw[2 % D]) * wt;
+                    brot[2] += (vij[0] * dw[1 % D] - vij[1 % D] * dw[0]) * wt;
+                } else if (D == 2) {
+                    brot[0] += (vij[0] * dw[1 % D] - vij[1 % D] * dw[0]) * wt;
+                }
+            }
         }
     }
     }
@@ -387,62 +416,12 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
     wave_add_atomic(a.stats, 2, valid ? newton_fail : 0);
     if (!valid) return;
 
-    // The member passes below visit the members in rank order: in step r
-    // every lane takes its r-th member (from the member bits; the candidate
-    // rows were just loaded by the density pass), so the loop runs as many
-    // steps as the largest member count rather than the candidate count, and
-    // the first such pass stores row r of the force list for all lanes at
-    // once.  The particle itself contributes nothing to these sums and is
-    // skipped.  Past 256 candidates the member test is repeated.
+    // The member sums (density, v_sig, MUSCL gradients, Balsara / AV) were
+    // taken in the density pass; what is left is writing the force-list rows
+    // (row r of every lane = its r-th member, from the member bits, so a
+    // store covers one 256-byte row) and finishing the sums.
     int *const frow = (fl && !fl_rows) ? a.fl_idx + list_slot(i, 0, a.fl_cap) : nullptr;
     bool rows_pending = frow != nullptr;
-    // body(j, d, r2, v_j (velocity, mass), u_j); members in groups of
-    // kGroup with their loads issued together
-    auto members = [&](auto &&body) {
-        const bool write = rows_pending;
-        rows_pending = false;
-        int wd = 0;
-        unsigned long long m = am[0];
-        for (int r0 = 0; r0 < na_bits; r0 += kGroup) {
-            int jg[kGroup];
-#pragma unroll
-            for (int u = 0; u < kGroup; ++u) {
-                while (m == 0ull && wd < 3) {
-                    ++wd;
-                    m = (wd == 1) ? am[1] : (wd == 2) ? am[2] : am[3];
-                }
-                // past the last member: repeat the previous one (not used)
-                const int qc = m ? (wd << 6) + __ffsll((long long)m) - 1 : -1;
-                m &= m - 1ull;
-                jg[u] = qc >= 0 ? pcol[qc * kWave] : (u ? jg[u - 1] : i);
-            }
-            V4 pg[kGroup], vg[kGroup];
-            T ug[kGroup];
-#pragma unroll
-            for (int u = 0; u < kGroup; ++u) {
-                pg[u] = RP[jg[u]];
-                vg[u] = RV[jg[u]];
-                ug[u] = rec_u(RT, jg[u]);
-            }
-#pragma unroll
-            for (int u = 0; u < kGroup; ++u) {
-                const int r = r0 + u;
-                if (r >= na_bits) break;
-                if (write && r < a.fl_cap) frow[r * kWave] = jg[u];
-                T d[3];
-                const T r2 = rec_r2<D, T>(pg[u], xi, a.box, d);
-                body(jg[u], d, r2, vg[u], ug[u]);
-            }
-        }
-        for (int q = 256; q < nc; ++q) {
-            const int j = pcol[q * kWave];
-            if (j == i) continue;
-            T d[3];
-            const T r2 = rec_r2<D, T>(RP[j], xi, a.box, d);
-            if (!member(j, r2)) continue;
-            body(j, d, r2, RV[j], rec_u(RT, j));
-        }
-    };
 
     if (kGrad && grad_on) {
         // MUSCL gradients (summed in the density pass)
@@ -475,57 +454,32 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
 #pragma unroll
         for (int q = 0; q < 16; ++q) a.rec_w[R_G0 + (q >> 2)][4 * (size_t)i + (q & 3)] = g[q];
     }
-    if (TYPE == SPHGPU_GSPH) {
-        if (rows_pending) {
-            // the force-list rows alone: row r of every lane = its r-th member
-            int wd = 0;
-            unsigned long long m = am[0];
-            const int nw = min(na_bits, a.fl_cap);
-            for (int r = 0; r < nw; ++r) {
-                while (m == 0ull && wd < 3) {
-                    ++wd;
-                    m = (wd == 1) ? am[1] : (wd == 2) ? am[2] : am[3];
-                }
-                const int qc = (wd << 6) + __ffsll((long long)m) - 1;
-                m &= m - 1ull;
-                frow[r * kWave] = pcol[qc * kWave];
+    if (rows_pending) {
+        // the force-list rows: row r of every lane = its r-th member
+        int wd = 0;
+        unsigned long long m = am[0];
+        const int nw = min(na_bits, a.fl_cap);
+        for (int r = 0; r < nw; ++r) {
+            while (m == 0ull && wd < 3) {
+                ++wd;
+                m = (wd == 1) ? am[1] : (wd == 2) ? am[2] : am[3];
             }
+            const int qc = (wd << 6) + __ffsll((long long)m) - 1;
+            m &= m - 1ull;
+            frow[r * kWave] = pcol[qc * kWave];
         }
-        return;
+        rows_pending = false;
     }
+    if (TYPE == SPHGPU_GSPH) return;
 
     // ---- Balsara switch and time-dependent AV (pre_interaction.cpp:115-161;
     // GDISPH, gd_pre_interaction.cpp:150-197: Balsara sums weighted by mass
     // and divided by density, the AV without Balsara as in DISPH) ----
     const T dt = T(a.dscal[0]);
     T alpha = a.rv.r[R_AUX][i].z;
-    const bool do_bal = a.balsara_on && D != 1;
-    const bool disph_w = (TYPE == SPHGPU_DISPH) || (TYPE == SPHGPU_GDISPH && !do_bal);
     if (do_bal || a.td_av) {
-        T div = T(0), rot[3] = {0, 0, 0};
-        members([&](int, const T d[3], T r2, const V4 &vj, T uj) {
-            const T r = tsqrt(r2);
-            const T cw = Kern<D, K, T>::dwc(r, h);
-            const T vjv[3] = {vj.x, vj.y, vj.z};
-            T dw[3], vij[3];
-#pragma unroll
-            for (int k = 0; k < D; ++k) {
-                dw[k] = d[k] * cw;
-                vij[k] = vi[k] - vjv[k];
-            }
-            const T wt = disph_w ? vj.w * uj : vj.w;
-            T dot = T(0);
-#pragma unroll
-            for (int k = 0; k < D; ++k) dot = (k == 0) ? vij[k] * dw[k] : dot + vij[k] * dw[k];
-            div -= wt * dot;
-            if (D == 3) {
-                rot[0] += (vij[1] * dw[2 % D] - vij[2 % D] * dw[1 % D]) * wt;
-                rot[1] += (vij[2 % D] * dw[0] - vij[0] * dw[2 % D]) * wt;
-                rot[2] += (vij[0] * dw[1 % D] - vij[1 % D] * dw[0]) * wt;
-            } else if (D == 2) {
-                rot[0] += (vij[0] * dw[1 % D] - vij[1 % D] * dw[0]) * wt;
-            }
-        });
+        // sums from the density pass
+        T div = bdiv, rot[3] = {brot[0], brot[1], brot[2]};
         T rot_abs;
         if (disph_w) {
             const T p_inv = gm1 / pres;
@@ -557,7 +511,6 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
             a.rec_w[R_AUX][4 * (size_t)i + 2] = alpha;
         }
     }
-    if (rows_pending) members([](int, const T *, T, const V4 &, T) {});
 }
 
 // ---- force pair lists from the pre-interaction lists ----------------------------
