@@ -207,15 +207,15 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
         const int band = fp32_band((float)r2, (float)h, mk);
         return band > 0 || (band == 0 && r_less(r2_64(a.x64, xi64, j, a.box, D), h64));
     };
-    // membership bits of the first 256 candidates, for the later member
-    // passes (beyond 256 the test is repeated)
+    // membership bits of the first 256 candidates, for the force-list rows
+    // written after this pass
     unsigned long long mb[4] = {0, 0, 0, 0};
     Acc<T> dens_a, dhd_a, n_a, pres_a, dhp_a, dhn_a;
     T vsig = ci * T(2.0);
     int w = 0;
     // members other than coincident ones start the force pair list: with
     // h_i below the search radius they are exactly {j : r_ij < h_i, r_ij != 0}.
-    // They are written to the force list after the member pass, one row per
+    // They are written to the force list after this pass, one row per
     // member rank, from the membership bits (past 256 candidates, during the
     // pass); a coincident pair j != i (r_ij = 0) makes the derived list
     // unusable for this step (the force searches instead).
@@ -233,9 +233,9 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
         const float padX = (float)((Xb + a.root[3]) * 3.814697265625e-06);
         z2 = padX * padX;
     }
-    // second-order GSPH / GDISPH: the MUSCL gradient sums run in this member
-    // pass too (members other than the particle itself, in candidate order,
-    // the order of a separate member pass)
+    // second-order GSPH / GDISPH: the MUSCL gradient sums run in this pass
+    // too (members other than the particle itself, in candidate order; the
+    // particle itself would add exactly zero)
     constexpr bool kGrad = TYPE == SPHGPU_GSPH || TYPE == SPHGPU_GDISPH;
     const bool grad_on = kGrad && a.gsph2;
     T gdd[3] = {0, 0, 0}, gdu[3] = {0, 0, 0}, gdv[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
