@@ -355,11 +355,47 @@ __global__ void k_lambda(const unsigned char *__restrict__ kdig, TreeShape sh, u
     // level-indexed table in registers would go to scratch memory)
     extern __shared__ short s_lr[];
     const int B = blockDim.x;
-    short *const nl = s_lr + threadIdx.x;
-    short *const nr = s_lr + (size_t)(sh.ml + 1) * B + threadIdx.x;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= sh.n) return;
     const int ML = sh.ml;
+    if (sh.leaf <= 64) {
+        // With pmL(d) = min kdig over [i - d, i] and pmR(e) = min kdig over
+        // (i, i + e] (kdig[n] = 0), the node of level t + 1 around i starts
+        // within d and ends within e positions iff pmL(d) <= t and pmR(e) <=
+        // t, so it holds <= leaf particles iff max(pmL(d), pmR(leaf - d)) <= t
+        // for some d < leaf: the leaf level is 1 + min over d of that max.
+        unsigned char *const pm = reinterpret_cast<unsigned char *>(s_lr) + threadIdx.x;
+        const int leaf = sh.leaf;
+        int m = 255;
+        for (int e = 1; e <= leaf; ++e) {
+            const int j = i + e;
+            m = min(m, j >= sh.n ? 0 : (int)kdig[j]);
+            pm[(e - 1) * B] = (unsigned char)m;
+        }
+        int ml = 255, best = 255;
+        for (int d = 0; d < leaf; ++d) {
+            const int j = i - d;
+            if (j >= 0) ml = min(ml, (int)kdig[j]);  // kdig[0] = 0 bounds it below 0
+            best = min(best, max(ml, (int)pm[(leaf - d - 1) * B]));
+        }
+        int L = ML + 1, lend = -1;
+        if (best + 1 <= ML) {
+            L = max(2, best + 1);
+            const int t = L - 1;
+            for (int e = 1; e <= leaf; ++e)
+                if (pm[(e - 1) * B] <= t) {
+                    lend = i + e;
+                    break;
+                }
+        }
+        lam[i] = (unsigned char)L;
+        leaf_end[i] = lend;
+        const int kd = kdig[i];
+        ncount[i] = (i == 0 ? 1 : 0) + max(0, L - max(2, kd + 1) + 1);
+        return;
+    }
+    short *const nl = s_lr + threadIdx.x;
+    short *const nr = s_lr + (size_t)(sh.ml + 1) * B + threadIdx.x;
     constexpr short kNone = -32768, kNoneR = 32767;
     for (int t = 0; t <= ML; ++t) { nl[t * B] = kNone; nr[t * B] = kNoneR; }
     // nearest boundary to the left with kdig <= t (position of the group start)
@@ -855,7 +891,8 @@ int tree_build(sphgpu_ctx *c)
     time_begin(c, "tree_nodes");
     hipLaunchKernelGGL(k_kdig, dim3(grid_for(n, 256)), dim3(256), 0, c->stream, (const unsigned long long *)t.key, sh,
                        t.kdig, skip ? cut : -1, t.sync_words);
-    hipLaunchKernelGGL(k_lambda, dim3(grid_for(n, 128)), dim3(128), sizeof(short) * 2 * (size_t)(sh.ml + 1) * 128,
+    const size_t lam_lds = sh.leaf <= 64 ? (size_t)sh.leaf * 128 : sizeof(short) * 2 * (size_t)(sh.ml + 1) * 128;
+    hipLaunchKernelGGL(k_lambda, dim3(grid_for(n, 128)), dim3(128), lam_lds,
                        c->stream, (const unsigned char *)t.kdig, sh,
                        t.lam, t.leaf_end, t.ncount);
     scan_counts(c->stream, (const unsigned *)t.ncount, (unsigned *)t.nbase, n, t.scan_tmp, true);
