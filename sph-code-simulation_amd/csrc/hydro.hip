@@ -560,26 +560,29 @@ __global__ __launch_bounds__(256) void k_flist_scatter(FlistArgs a)
     for (int u = 0; u < kGroup; ++u) jg[u] = pcol[min(q0 + u, na - 1) * kWave];
 #pragma unroll
     for (int u = 0; u < kGroup; ++u) pg[u] = a.posf[jg[u]];
+    // FP32 rounding is monotone: rounded values that differ order h_i and
+    // h_j as the FP64 values do, so the FP32 form reads the FP64 h_j only
+    // for equal rounded values or an undecided FP32 distance test
 #pragma unroll
-    for (int u = 0; u < kGroup; ++u) hg[u] = pg[u].w > pi.w ? 0.0 : a.sml[jg[u]];
+    for (int u = 0; u < kGroup; ++u)
+        hg[u] = (FP64 ? pg[u].w <= pi.w : pg[u].w == pi.w) ? a.sml[jg[u]] : 0.0;
 #pragma unroll
     for (int u = 0; u < kGroup; ++u) {
         if (q0 + u >= na) break;
         const int j = jg[u];
         const float4 pj = pg[u];
-        // FP32 rounding is monotone: h_j > h_i when the rounded values say so,
-        // and then r < h_i <= h_j puts i in the list of j already
+        // h_j > h_i: r < h_i <= h_j puts i in the list of j already
         if (pj.w > pi.w) continue;
-        const double hj = hg[u];
-        if (!(hj < hi)) continue;
+        const bool tie = FP64 || pj.w == pi.w;
+        if (tie && !(hg[u] < hi)) continue;
         bool in_j;  // r_ij < h_j
         if (FP64) {
-            in_j = r_less(r2_64(a.x64, xi64, j, a.box, D), hj);
+            in_j = r_less(r2_64(a.x64, xi64, j, a.box, D), hg[u]);
         } else {
             float d[3];
             const float r2f = rec_r2<D, float>(pj, xf, a.box, d);
             const int band = fp32_band(r2f, pj.w, mk);
-            in_j = band > 0 || (band == 0 && r_less(r2_64(a.x64, xi64, j, a.box, D), hj));
+            in_j = band > 0 || (band == 0 && r_less(r2_64(a.x64, xi64, j, a.box, D), tie ? hg[u] : a.sml[j]));
         }
         if (in_j) continue;
         const int s = atomicAdd(&a.cnt[j], 1);
