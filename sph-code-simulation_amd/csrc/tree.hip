@@ -119,13 +119,17 @@ __global__ void k_keys(const double *__restrict__ x, const double *__restrict__ 
     double e = root[3];
     unsigned long long kk = 0;
     for (int lev = 0; lev < sh.ml; ++lev) {
+        // child centre, src/bhtree.cpp:233-239: c + (+-1 * edge) * 0.25; the
+        // products are exact (sign and power-of-two scalings), so c + q or
+        // c - q with q = edge * 0.25 rounds identically
+        const double q = e * 0.25;
         unsigned dig = 0;
-        for (int k = 0; k < sh.dim; ++k) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            if (k >= sh.dim) break;
             const bool up = pos[k] > c[k];  // src/bhtree.cpp:211-218
             dig |= (up ? 1u : 0u) << k;
-            // child centre, src/bhtree.cpp:233-239: c + (+-1 * edge) * 0.25
-            const double s = up ? 1.0 : -1.0;
-            c[k] = c[k] + s * e * 0.25;
+            c[k] = c[k] + (up ? q : -q);
         }
         e = e * 0.5;
         kk = (kk << sh.bits) | dig;
@@ -592,15 +596,27 @@ struct RangeTables {
     const double *h, *b1, *b2;
 };
 
-__device__ double range_max(RangeTables t, int s, int e)
+// Lanes per node in the node passes: a node's range splits into partial
+// 64-blocks at both ends, partial 4096-blocks of whole 64-blocks, and whole
+// 4096-blocks; each of the group's lanes takes every kNodeLanes-th entry of
+// each piece, so the root costs tens of loads per lane instead of hundreds
+// in one thread.
+constexpr int kNodeLanes = 8;
+
+// max of h over [s, e), all kNodeLanes lanes of the group taking part (max is
+// exact, so the order does not matter)
+__device__ double range_max_group(RangeTables t, int s, int e, int sub)
 {
     double m = 0.0;
-    while (s < e && (s & 63)) { m = fmax(m, t.h[s]); ++s; }
-    while (e > s && (e & 63)) { --e; m = fmax(m, t.h[e]); }
-    int b0 = s >> 6, b1 = e >> 6;
-    while (b0 < b1 && (b0 & 63)) { m = fmax(m, t.b1[b0]); ++b0; }
-    while (b1 > b0 && (b1 & 63)) { --b1; m = fmax(m, t.b1[b1]); }
-    for (int B = b0 >> 6; B < (b1 >> 6); ++B) m = fmax(m, t.b2[B]);
+    const int s1 = min(e, (s + 63) & ~63), e1 = max(s1, e & ~63);  // whole 64-blocks [s1, e1)
+    for (int j = s + sub; j < s1; j += kNodeLanes) m = fmax(m, t.h[j]);
+    for (int j = e1 + sub; j < e; j += kNodeLanes) m = fmax(m, t.h[j]);
+    const int b0 = s1 >> 6, b1 = e1 >> 6;
+    const int c0 = min(b1, (b0 + 63) & ~63), c1 = max(c0, b1 & ~63);  // whole 4096-blocks
+    for (int j = b0 + sub; j < c0; j += kNodeLanes) m = fmax(m, t.b1[j]);
+    for (int j = c1 + sub; j < b1; j += kNodeLanes) m = fmax(m, t.b1[j]);
+    for (int B = (c0 >> 6) + sub; B < (c1 >> 6); B += kNodeLanes) m = fmax(m, t.b2[B]);
+    for (int off = kNodeLanes / 2; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, kNodeLanes));
     return m;
 }
 
@@ -615,9 +631,11 @@ __device__ float float_up(double x)
 __global__ void k_node_ks(const int *__restrict__ nstart, const int *__restrict__ nend, int total, RangeTables t,
                           double *__restrict__ ks, int4 *__restrict__ nlink)
 {
-    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= total) return;
-    const double k = range_max(t, nstart[idx], nend[idx]);
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    const int idx = g / kNodeLanes, sub = g % kNodeLanes;
+    const bool ok = idx < total;
+    const double k = range_max_group(t, ok ? nstart[idx] : 0, ok ? nend[idx] : 0, sub);
+    if (!ok || sub) return;
     ks[idx] = k;
     int4 l = nlink[idx];
     l.w = (l.w & (int)0x80000000) | __float_as_int(float_up(k));
@@ -636,8 +654,10 @@ __global__ void k_node_records(int total, int dim, const int *__restrict__ nstar
                                const double *__restrict__ pz, float4 *__restrict__ nbox, float2 *__restrict__ nhalf,
                                int4 *__restrict__ nlink)
 {
-    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= total) return;
+    // kNodeLanes lanes per node share a leaf's member loop
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    const int idx = g / kNodeLanes, sub = g % kNodeLanes;
+    if (idx >= total) return;  // whole groups leave together
     const int s = nstart[idx], e = nend[idx];
     const bool leaf = nlevel[idx] < 0;
     double c[3] = {ncenter[idx], ncenter[(size_t)cap + idx], ncenter[2 * (size_t)cap + idx]};
@@ -646,16 +666,22 @@ __global__ void k_node_records(int total, int dim, const int *__restrict__ nstar
         const double *p[3] = {px, py, pz};
         for (int k = 0; k < dim; ++k) {
             double lo = p[k][s], hi = lo;
-            for (int j = s + 1; j < e; ++j) {
+            for (int j = s + 1 + sub; j < e; j += kNodeLanes) {
                 const double v = p[k][j];
                 lo = v < lo ? v : lo;
                 hi = v > hi ? v : hi;
+            }
+            for (int off = kNodeLanes / 2; off > 0; off >>= 1) {
+                const double ol = __shfl_xor(lo, off, kNodeLanes), oh = __shfl_xor(hi, off, kNodeLanes);
+                lo = ol < lo ? ol : lo;
+                hi = oh > hi ? oh : hi;
             }
             c[k] = 0.5 * (lo + hi);
             // half extent, and rounding of the centre in FP64
             h[k] = 0.5 * (hi - lo) + 4.0 * 1.1102230246251565e-16 * (fabs(lo) + fabs(hi));
         }
     }
+    if (sub) return;
     nbox[idx] = make_float4((float)c[0], (float)c[1], (float)c[2], float_up(h[0]));
     nhalf[idx] = make_float2(float_up(h[1]), float_up(h[2]));
     nlink[idx] = make_int4(s, e, nskip[idx], leaf ? (int)0x80000000 : 0);
@@ -924,7 +950,7 @@ int tree_build(sphgpu_ctx *c)
                        (const unsigned char *)t.kdig, (const unsigned char *)t.lam, (const int *)t.leaf_end,
                        (const int *)t.nbase, (const double *)t.root, sh, total, t.start, t.end, t.skip, t.level,
                        t.center, t.edge, t.node_cap, c->dstats);
-    hipLaunchKernelGGL(k_node_records, dim3(grid_for(total, 256)), dim3(256), 0, c->stream, total, c->dim,
+    hipLaunchKernelGGL(k_node_records, dim3(grid_for((long long)total * kNodeLanes, 256)), dim3(256), 0, c->stream, total, c->dim,
                        (const int *)t.start, (const int *)t.end, (const int *)t.skip, (const int *)t.level,
                        (const double *)t.center, (const double *)t.edge, t.node_cap, (const double *)c->pos[0],
                        (const double *)(c->dim > 1 ? c->pos[1] : c->pos[0]),
@@ -973,7 +999,7 @@ int tree_set_kernel(sphgpu_ctx *c)
     hipLaunchKernelGGL(k_block_max, dim3(grid_for((long long)nb2 * 64, 256)), dim3(256), 0, c->stream,
                        (const double *)t.bmax1, nb1, t.bmax2);
     RangeTables rt{c->sc[S_SML], t.bmax1, t.bmax2};
-    hipLaunchKernelGGL(k_node_ks, dim3(grid_for(t.n_nodes, 128)), dim3(128), 0, c->stream, (const int *)t.start,
+    hipLaunchKernelGGL(k_node_ks, dim3(grid_for((long long)t.n_nodes * kNodeLanes, 128)), dim3(128), 0, c->stream, (const int *)t.start,
                        (const int *)t.end, t.n_nodes, rt, t.ks, t.nlink);
     time_end(c);
     HIPCHK(c, hipGetLastError());
