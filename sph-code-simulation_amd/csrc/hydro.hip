@@ -15,6 +15,7 @@
 //   FluidForce (GSPH)       src/gsph/g_fluid_force.cpp:44-304
 #include <hip/hip_runtime.h>
 
+#include <climits>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -595,6 +596,8 @@ __global__ __launch_bounds__(256) void k_flist_scatter(FlistArgs a)
     }
 }
 
+constexpr int kRunSort = 16;
+
 __global__ __launch_bounds__(256) void k_flist_order(FlistArgs a)
 {
     const int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
@@ -602,8 +605,29 @@ __global__ __launch_bounds__(256) void k_flist_order(FlistArgs a)
     if (i < a.n) {
         const int na = a.cnt_a[i];
         nt = min(a.cnt[i], a.cap);
-        // insertion sort of the appended run (a few entries)
-        for (int q = na + 1; q < nt; ++q) {
+        const int m = nt - na;  // length of the appended run
+        if (m > 1 && m <= kRunSort) {
+            // short runs (the usual case): all loads issued together, an
+            // odd-even transposition network in registers (kRunSort rounds
+            // sort kRunSort entries; the padding sorts to the end)
+            int v[kRunSort];
+#pragma unroll
+            for (int u = 0; u < kRunSort; ++u) v[u] = u < m ? a.idx[list_slot(i, na + u, a.cap)] : INT_MAX;
+#pragma unroll
+            for (int r = 0; r < kRunSort; ++r) {
+#pragma unroll
+                for (int u = r & 1; u + 1 < kRunSort; u += 2) {
+                    const int lo = min(v[u], v[u + 1]), hi = max(v[u], v[u + 1]);
+                    v[u] = lo;
+                    v[u + 1] = hi;
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < kRunSort; ++u)
+                if (u < m) a.idx[list_slot(i, na + u, a.cap)] = v[u];
+        }
+        // insertion sort of a longer appended run
+        for (int q = na + 1; m > kRunSort && q < nt; ++q) {
             const int v = a.idx[list_slot(i, q, a.cap)];
             int p = q - 1;
             int u = a.idx[list_slot(i, p, a.cap)];
