@@ -351,6 +351,8 @@ __global__ void k_kdig(const unsigned long long *__restrict__ key, TreeShape sh,
 // ---- 5. leaf level and node counts ----------------------------------------------
 constexpr int kMaxLevels = 64;
 
+constexpr int kLambdaThreads = 128;
+
 __global__ void k_lambda(const unsigned char *__restrict__ kdig, TreeShape sh, unsigned char *__restrict__ lam,
                          int *__restrict__ leaf_end, int *__restrict__ ncount)
 {
@@ -358,11 +360,20 @@ __global__ void k_lambda(const unsigned char *__restrict__ kdig, TreeShape sh, u
     // left (offset <= 0) and to the right (offset >= 1) of i, in LDS (a
     // level-indexed table in registers would go to scratch memory)
     extern __shared__ short s_lr[];
+    // kdig over [block start - leaf, block end + leaf) (0 outside [0, n))
+    __shared__ unsigned char s_kd[kLambdaThreads + 2 * 64];
     const int B = blockDim.x;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= sh.n) return;
     const int ML = sh.ml;
     if (sh.leaf <= 64) {
+        const int wb = blockIdx.x * B - sh.leaf;
+        for (int t = threadIdx.x; t < B + 2 * sh.leaf; t += B) {
+            const int j = wb + t;
+            s_kd[t] = (j < 0 || j >= sh.n) ? (unsigned char)0 : kdig[j];
+        }
+        __syncthreads();
+        if (i >= sh.n) return;
+        const unsigned char *const kw = s_kd + (i - wb);  // kw[x] = kdig[i + x]
         // With pmL(d) = min kdig over [i - d, i] and pmR(e) = min kdig over
         // (i, i + e] (kdig[n] = 0), the node of level t + 1 around i starts
         // within d and ends within e positions iff pmL(d) <= t and pmR(e) <=
@@ -372,14 +383,13 @@ __global__ void k_lambda(const unsigned char *__restrict__ kdig, TreeShape sh, u
         const int leaf = sh.leaf;
         int m = 255;
         for (int e = 1; e <= leaf; ++e) {
-            const int j = i + e;
-            m = min(m, j >= sh.n ? 0 : (int)kdig[j]);
+            m = min(m, (int)kw[e]);  // kdig[n] = 0
             pm[(e - 1) * B] = (unsigned char)m;
         }
         int ml = 255, best = 255;
         for (int d = 0; d < leaf; ++d) {
-            const int j = i - d;
-            if (j >= 0) ml = min(ml, (int)kdig[j]);  // kdig[0] = 0 bounds it below 0
+            // the window's zeros left of 0 change nothing: kdig[0] = 0 came first
+            ml = min(ml, (int)kw[-d]);
             best = min(best, max(ml, (int)pm[(leaf - d - 1) * B]));
         }
         int L = ML + 1, lend = -1;
@@ -394,10 +404,11 @@ __global__ void k_lambda(const unsigned char *__restrict__ kdig, TreeShape sh, u
         }
         lam[i] = (unsigned char)L;
         leaf_end[i] = lend;
-        const int kd = kdig[i];
+        const int kd = kw[0];
         ncount[i] = (i == 0 ? 1 : 0) + max(0, L - max(2, kd + 1) + 1);
         return;
     }
+    if (i >= sh.n) return;
     short *const nl = s_lr + threadIdx.x;
     short *const nr = s_lr + (size_t)(sh.ml + 1) * B + threadIdx.x;
     constexpr short kNone = -32768, kNoneR = 32767;
@@ -917,8 +928,8 @@ int tree_build(sphgpu_ctx *c)
     time_begin(c, "tree_nodes");
     hipLaunchKernelGGL(k_kdig, dim3(grid_for(n, 256)), dim3(256), 0, c->stream, (const unsigned long long *)t.key, sh,
                        t.kdig, skip ? cut : -1, t.sync_words);
-    const size_t lam_lds = sh.leaf <= 64 ? (size_t)sh.leaf * 128 : sizeof(short) * 2 * (size_t)(sh.ml + 1) * 128;
-    hipLaunchKernelGGL(k_lambda, dim3(grid_for(n, 128)), dim3(128), lam_lds,
+    const size_t lam_lds = sh.leaf <= 64 ? (size_t)sh.leaf * kLambdaThreads : sizeof(short) * 2 * (size_t)(sh.ml + 1) * kLambdaThreads;
+    hipLaunchKernelGGL(k_lambda, dim3(grid_for(n, kLambdaThreads)), dim3(kLambdaThreads), lam_lds,
                        c->stream, (const unsigned char *)t.kdig, sh,
                        t.lam, t.leaf_end, t.ncount);
     scan_counts(c->stream, (const unsigned *)t.ncount, (unsigned *)t.nbase, n, t.scan_tmp, true);
