@@ -89,6 +89,8 @@ struct Tree {
     int hist_blocks = 0;
     // node count and the partial-sort check, read together by the host
     int *sync_words = nullptr;         // [0] node count, [1] prefix run > leaf
+    int *host_words = nullptr;         // pinned host copy of sync_words
+    hipEvent_t words_ready = nullptr;  // recorded after that copy
     bool full_sort = false;            // a partial sort failed its check once
     long long partial_sorts = 0, full_sorts = 0;
     unsigned int *scan_tmp = nullptr;  // tile sums of the multi-block scans

@@ -481,11 +481,17 @@ __device__ int gallop_end(const unsigned long long *__restrict__ key, int n, int
 
 __global__ void k_fill(const unsigned long long *__restrict__ key, const unsigned char *__restrict__ kdig,
                        const unsigned char *__restrict__ lam, const int *__restrict__ leaf_end,
-                       const int *__restrict__ nbase, const double *__restrict__ root, TreeShape sh, int total,
+                       const int *__restrict__ nbase, const double *__restrict__ root, TreeShape sh,
+                       const int *__restrict__ sync_words,
                        int *__restrict__ nstart, int *__restrict__ nend, int *__restrict__ nskip,
                        int *__restrict__ nlevel, double *__restrict__ ncenter, double *__restrict__ nedge, int cap,
                        long long *__restrict__ stats)
 {
+    // the node count comes from the device (the host reads it while this
+    // runs); nothing is written when it exceeds the capacity or when the
+    // partial-sort check failed (the build is then repeated)
+    const int total = sync_words[0];
+    if (total > cap || sync_words[1] != 0) return;
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     int leaves = 0, depth = 0;
     const int idx0 = (s < sh.n) ? nbase[s] : 0;
@@ -800,6 +806,8 @@ int tree_alloc(sphgpu_ctx *c)
               al((void **)&t.child, 4ull * (t.node_cap + 8)) && al((void **)&t.ccount, 4ull * t.node_cap);
     const long long scan_max = std::max<long long>(std::max<long long>(n + 1, 256ll * t.hist_blocks), t.node_cap);
     ok = ok && al((void **)&t.scan_tmp, 4ull * ((scan_max + kScanTile - 1) / kScanTile + 1));
+    ok = ok && hipHostMalloc((void **)&t.host_words, 4ull * 2, hipHostMallocDefault) == hipSuccess &&
+         hipEventCreateWithFlags(&t.words_ready, hipEventDisableTiming) == hipSuccess;
     if (!ok) return SPHGPU_E_NOMEM;
     if (c->prm.periodic) {
         double r[8] = {0};
@@ -823,6 +831,8 @@ void tree_free(sphgpu_ctx *c)
                   t.bsum2, t.root, t.hist, t.nbox, t.nlink, t.scan_tmp, t.nhalf, t.nchild, t.child, t.ccount};
     for (void *p : ps)
         if (p) hipFree(p);
+    if (t.host_words) hipHostFree(t.host_words);
+    if (t.words_ready) hipEventDestroy(t.words_ready);
     t = Tree();
 }
 
@@ -934,17 +944,27 @@ int tree_build(sphgpu_ctx *c)
                        t.lam, t.leaf_end, t.ncount);
     scan_counts(c->stream, (const unsigned *)t.ncount, (unsigned *)t.nbase, n, t.scan_tmp, true);
     HIPCHK(c, hipGetLastError());
-    int words[2] = {0, 0};
     HIPCHK(c, hipMemcpyAsync(t.sync_words, t.nbase + n, sizeof(int), hipMemcpyDeviceToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(words, t.sync_words, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    const int total = words[0];
+    HIPCHK(c, hipMemcpyAsync(t.host_words, t.sync_words, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipEventRecord(t.words_ready, c->stream));
+    hipLaunchKernelGGL(k_fill, dim3(grid_for(n, 128)), dim3(128), 0, c->stream, (const unsigned long long *)t.key,
+                       (const unsigned char *)t.kdig, (const unsigned char *)t.lam, (const int *)t.leaf_end,
+                       (const int *)t.nbase, (const double *)t.root, sh, (const int *)t.sync_words, t.start, t.end,
+                       t.skip, t.level, t.center, t.edge, t.node_cap, c->dstats);
+    HIPCHK(c, hipGetLastError());
+    time_end(c);
+    // the pair records depend on the permuted particle state only: they are
+    // refreshed while the host waits for the node count
+    c->flist = 0;  // particles were reordered
+    rc = refresh_mirrors(c);
+    if (rc) return rc;
+    HIPCHK(c, hipEventSynchronize(t.words_ready));
+    const int total = t.host_words[0];
     if (skip) {
-        if (words[1]) {
+        if (t.host_words[1]) {
             // a depth-12 cell holds more than a leaf: sort fully from here on
             // (the state is already in the partial order; the full build
             // starts from it)
-            time_end(c);
             t.full_sort = true;
             return tree_build(c);
         }
@@ -957,10 +977,7 @@ int tree_build(sphgpu_ctx *c)
         return fail(c, SPHGPU_E_NO_FREE_NODE, "There is no free node.");
     }
     t.n_nodes = total;
-    hipLaunchKernelGGL(k_fill, dim3(grid_for(n, 128)), dim3(128), 0, c->stream, (const unsigned long long *)t.key,
-                       (const unsigned char *)t.kdig, (const unsigned char *)t.lam, (const int *)t.leaf_end,
-                       (const int *)t.nbase, (const double *)t.root, sh, total, t.start, t.end, t.skip, t.level,
-                       t.center, t.edge, t.node_cap, c->dstats);
+    time_begin(c, "tree_node_records");
     hipLaunchKernelGGL(k_node_records, dim3(grid_for((long long)total * kNodeLanes, 256)), dim3(256), 0, c->stream, total, c->dim,
                        (const int *)t.start, (const int *)t.end, (const int *)t.skip, (const int *)t.level,
                        (const double *)t.center, (const double *)t.edge, t.node_cap, (const double *)c->pos[0],
@@ -992,9 +1009,7 @@ int tree_build(sphgpu_ctx *c)
     HIPCHK(c, hipGetLastError());
     t.valid = true;
     c->sorted_valid = true;
-    c->flist = 0;  // particles were reordered
-    // the inputs of the pair loops moved: refresh their FP32 copies
-    return refresh_mirrors(c);
+    return 0;
 }
 
 // BHTree::set_kernel: kernel_size = max smoothing length in each subtree
