@@ -249,6 +249,8 @@ int sphgpu_create(const sphgpu_params *p, int n, sphgpu_ctx **out)
     ok = ok && alloc((void **)&c->dscal, sizeof(double) * 16) && alloc((void **)&c->dmin, sizeof(unsigned long long) * kStatCopies) &&
          alloc((void **)&c->dstats, sizeof(long long) * kStatSlots * kStatCopies) && alloc((void **)&c->dflag, sizeof(int) * 8);
     ok = ok && hipHostMalloc((void **)&c->hscal, sizeof(double) * 16, hipHostMallocDefault) == hipSuccess;
+    ok = ok && hipHostMalloc((void **)&c->hflag, sizeof(int) * 8, hipHostMallocDefault) == hipSuccess &&
+         hipEventCreateWithFlags(&c->flag_ready, hipEventDisableTiming) == hipSuccess;
     ok = ok && tree_alloc(c) == 0;
     // neighbour lists: about 2.5x the expected candidate count nn * 1.2^D
     const double kap = p->iterative_sml ? std::pow(1.2, p->dim) : 1.0;
@@ -285,6 +287,8 @@ int sphgpu_destroy(sphgpu_ctx *c)
     hipFree(c->hs); hipFree(c->stage); hipFree(c->red);
     hipFree(c->dscal); hipFree(c->dmin); hipFree(c->dstats); hipFree(c->dflag);
     if (c->hscal) hipHostFree(c->hscal);
+    if (c->hflag) hipHostFree(c->hflag);
+    if (c->flag_ready) hipEventDestroy(c->flag_ready);
     tree_free(c);
     lists_free(c->lpre);
     lists_free(c->lforce);

@@ -155,6 +155,8 @@ struct sphgpu_ctx {
     // counters, see sphgpu_stats layout
     long long *dstats = nullptr;
     int *dflag = nullptr;        // [0] list overflow, [1] node overflow
+    int *hflag = nullptr;        // pinned host copy of dflag
+    hipEvent_t flag_ready = nullptr;  // recorded after that copy
 
     sphgpu::Tree tree;
     sphgpu::Lists lpre, lforce;

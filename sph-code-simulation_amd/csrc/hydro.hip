@@ -695,6 +695,9 @@ __global__ void k_store_hvs(const unsigned long long *m, double *dscal)
 
 // ---- FluidForce --------------------------------------------------------------------
 template <typename T> struct ForceArgs {
+    // when set: the launch does nothing if skip_if[0] or skip_if[2] is
+    // nonzero (the overflow flags of the lists it would read)
+    const int *skip_if = nullptr;
     int n;
     const int *idx, *cnt;
     int cap;
@@ -797,6 +800,7 @@ SPH_DEV void pair_states(bool second, T gamma, T dt, const T d[3], T r, const T 
 template <int D, int K, int TYPE, typename T>
 __global__ __launch_bounds__(256) void k_force_sph(ForceArgs<T> a)
 {
+    if (a.skip_if && (a.skip_if[0] | a.skip_if[2])) return;  // whole grid
     typedef typename V4T<T>::type V4;
     const int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (i >= a.n) return;
@@ -1204,6 +1208,7 @@ SPH_DEV void force_gsph_slots(const ForceArgs<T> &a, int i, int nc, int nmax, co
 template <int D, int K, typename T, int SOLVER, int LEARNED>
 __global__ __launch_bounds__(256) void k_force_gsph(ForceArgs<T> a)
 {
+    if (a.skip_if && (a.skip_if[0] | a.skip_if[2])) return;  // whole grid
     const int i0 = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const bool valid = i0 < a.n;
     const int i = valid ? i0 : a.n - 1;
@@ -1467,9 +1472,10 @@ template <typename T> ForceArgs<T> force_args(sphgpu_ctx *c)
     return a;
 }
 
-template <int D, int K, typename T> void launch_force(sphgpu_ctx *c)
+template <int D, int K, typename T> void launch_force(sphgpu_ctx *c, const int *skip_if)
 {
     ForceArgs<T> a = force_args<T>(c);
+    a.skip_if = skip_if;
     const dim3 g(grid_for(c->n, 256)), b(256);
     if (c->prm.sph_type == SPHGPU_SSPH) {
         hipLaunchKernelGGL((k_force_sph<D, K, SPHGPU_SSPH, T>), g, b, 0, c->stream, a);
@@ -1486,20 +1492,20 @@ template <int D, int K, typename T> void launch_force(sphgpu_ctx *c)
     }
 }
 
-template <int D, typename T> void dispatch_force_kernel(sphgpu_ctx *c)
+template <int D, typename T> void dispatch_force_kernel(sphgpu_ctx *c, const int *skip_if)
 {
     switch (c->prm.kernel) {
-    case SPHGPU_CUBIC_SPLINE: launch_force<D, SPHGPU_CUBIC_SPLINE, T>(c); break;
-    case SPHGPU_WENDLAND: if (D > 1) launch_force<D, SPHGPU_WENDLAND, T>(c); break;
-    default: launch_force<D, SPHGPU_GAUSSIAN, T>(c); break;
+    case SPHGPU_CUBIC_SPLINE: launch_force<D, SPHGPU_CUBIC_SPLINE, T>(c, skip_if); break;
+    case SPHGPU_WENDLAND: if (D > 1) launch_force<D, SPHGPU_WENDLAND, T>(c, skip_if); break;
+    default: launch_force<D, SPHGPU_GAUSSIAN, T>(c, skip_if); break;
     }
 }
 
-template <typename T> void dispatch_force(sphgpu_ctx *c)
+template <typename T> void dispatch_force(sphgpu_ctx *c, const int *skip_if)
 {
-    if (c->dim == 1) dispatch_force_kernel<1, T>(c);
-    else if (c->dim == 2) dispatch_force_kernel<2, T>(c);
-    else dispatch_force_kernel<3, T>(c);
+    if (c->dim == 1) dispatch_force_kernel<1, T>(c, skip_if);
+    else if (c->dim == 2) dispatch_force_kernel<2, T>(c, skip_if);
+    else dispatch_force_kernel<3, T>(c, skip_if);
 }
 
 } // namespace
@@ -1586,6 +1592,26 @@ int fluid_force(sphgpu_ctx *c)
     if (!c->tree.valid) return fail(c, SPHGPU_E_STATE, "fluid_force needs a built tree");
     int rc;
     if (c->mirror_dirty && (rc = refresh_mirrors(c))) return rc;
+    if (c->flist == 1) {
+        // Derived lists not yet checked: the force runs at once over them and
+        // skips itself on the device when a list flag is set (the search's
+        // overflow, or a force list that needs the is_ij search); the host
+        // reads the same flags meanwhile.  After a skip, nothing was written
+        // and the checked path below runs.
+        HIPCHK(c, hipMemcpyAsync(c->hflag, c->dflag, 3 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipEventRecord(c->flag_ready, c->stream));
+        time_begin(c, "fluid_force");
+        if (c->prm.pair_precision == SPHGPU_PAIR_FP64) dispatch_force<double>(c, c->dflag);
+        else dispatch_force<float>(c, c->dflag);
+        time_end(c);
+        HIPCHK(c, hipGetLastError());
+        HIPCHK(c, hipEventSynchronize(c->flag_ready));
+        if (c->hflag[0] == 0 && c->hflag[2] == 0) {
+            c->pre_search_pending = false;
+            c->flist = 2;
+            return 0;
+        }
+    }
     // the pair lists derived by the pre-interaction when they are complete,
     // the is_ij tree search otherwise
     const int fc = flist_check(c);
@@ -1595,8 +1621,8 @@ int fluid_force(sphgpu_ctx *c)
         if ((rc = search(c, c->lforce, 1, c->sc[S_SML]))) return rc;
     }
     time_begin(c, "fluid_force");
-    if (c->prm.pair_precision == SPHGPU_PAIR_FP64) dispatch_force<double>(c);
-    else dispatch_force<float>(c);
+    if (c->prm.pair_precision == SPHGPU_PAIR_FP64) dispatch_force<double>(c, nullptr);
+    else dispatch_force<float>(c, nullptr);
     time_end(c);
     HIPCHK(c, hipGetLastError());
     return 0;
