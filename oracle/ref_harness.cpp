@@ -159,7 +159,7 @@ void *ref_create(const sphgpu_params *prm, int n)
     /* module wiring, src/solver.cpp:686-724 */
 #ifdef SPHGPU_DROPIN
     /* the drop-in: the GPU Module adapters in place of the reference modules,
-     * the change INTEGRATION.md describes for Solver::initialize */
+     * the change a maintainer makes to Solver::initialize's module wiring */
     r->ts = std::make_shared<sph::gpu::TimeStep>();
     r->pre = std::make_shared<sph::gpu::PreInteraction>();
     r->force = std::make_shared<sph::gpu::FluidForce>();
