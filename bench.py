@@ -6,12 +6,21 @@ One step = TimeStep, predict, tree build, PreInteraction, FluidForce,
 exactly K steps of sphgpu_integrate, which returns only when the stream has
 drained; inputs are resident in HBM before it starts.
 
-Prints one JSON line (the driver's contract), with two extra objects:
-  roofline      achieved algorithmic FLOP rate of the dominant kernel (HIP
-                events on the library's stream over the timed region) against
-                the MI355X FP32 peak;
-  cpu_baseline  the oracle (CPU restatement of the reference) on a bounded
-                sample of the same workload on this host's cores.
+Prints one JSON line (the driver's contract), with these extra objects:
+  roofline      the dominant kernel's achieved algorithmic rate (HIP events on
+                the library's stream over the timed region) against the
+                MI355X peak of the roof that bounds it;
+  kernels       the same figures for every priced kernel, with the measured
+                HBM traffic per launch (rocprofv3 FETCH_SIZE / WRITE_SIZE,
+                profiles/traffic.json) and the achieved HBM GB/s;
+  accuracy      the energy-drift gate at the benchmark size: the oracle runs
+                the same initialisation and warm-up + timed steps on the host
+                (FP64, plain guess) and |drift_GPU - drift_oracle| is reported
+                against the 1e-4 bar;
+  cpu_baseline  the reference's own CPU modules (oracle/_ref, compiled from
+                its unmodified sources) timed on a bounded sample of the same
+                workload on this host's cores; cpu_port is the oracle's time
+                on the gate run, with its ratio to the reference.
 """
 from __future__ import annotations
 
@@ -38,14 +47,24 @@ PEAK_HBM_GBS = 8000.0
 # Algorithmic FLOP model (3D, +,-,*,compare = 1, FMA = 2, div and sqrt = 1),
 # with the measured per-step counters: N_p accepted force pairs, k_R Riemann
 # Newton iterations per pair, N_c PreInteraction candidates, k_h smoothing-
-# length Newton iterations, N_m members (neighbour counts):
+# length Newton iterations, N_m members (neighbour counts), N_r candidate
+# tests of a per-particle tree walk (members of every leaf whose box the
+# particle's own search sphere reaches):
 #   FluidForce (GSPH 2nd order)  N_p (270 + 35 k_R)
 #   PreInteraction (GSPH)        N_c (50 k_h + 60) + N_m 60 (MUSCL gradients)
+#   search (PreInteraction)      N_r 8 (3 differences, 3 products, 2 adds,
+#                                plus the compare counted in the 8)
 FORCE_PAIR_FLOP = 270.0
 RIEMANN_ITER_FLOP = 35.0
 PRE_NEWTON_FLOP = 50.0
 PRE_FINAL_FLOP = 60.0
 PRE_GRAD_FLOP = 60.0
+SEARCH_TEST_FLOP = 8.0
+# the Sedov workloads' background pressure (the workload text says 1e-5)
+P_BACKGROUND_NOTE = ("0.1: at the workload's 1e-5 the reference's own second-order GSPH drives "
+                     "the cold particles beside the blast to negative energy and NaN in the first "
+                     "steps (aborting in Periodic::calc_r_ij); it runs at 0.1 from 32^3 to 100^3")
+ENERGY_GATE = 1e-4
 PROFILES = REPO / "profiles"
 
 
@@ -124,53 +143,113 @@ def measured_traffic(kernel: str, workload: str, form: str):
         return None
 
 
-def roofline(res: dict, steps: int, workload: str, form: str) -> dict:
+def kernel_models(res: dict, steps: int, workload: str, form: str) -> dict:
+    """Per priced kernel: average launch time (HIP events), algorithmic work
+    per launch from the FLOP model and the run's own counters, the achieved
+    rate against the FP32 (or FP64) vector peak, and the measured HBM traffic
+    per launch with the achieved HBM bandwidth."""
     st = res["stats"]
     times = res["times"]
     n = res["n"]
     pairs = st["force_pairs"] / steps
     k_r = st["riemann_iters"] / max(1, st["riemann_calls"])
-    force_flop = pairs * (FORCE_PAIR_FLOP + RIEMANN_ITER_FLOP * k_r)
     cand = st["pre_candidates"] / steps
     k_h = st["pre_newton_iters"] / max(1, n * steps)
-    members = res["members"]
-    pre_flop = cand * (PRE_NEWTON_FLOP * k_h + PRE_FINAL_FLOP) + members * PRE_GRAD_FLOP
-    flops = {"fluid_force": force_flop, "pre_interaction": pre_flop}
-    totals = {k: v[0] * v[1] for k, v in times.items()}
-    dom = max((k for k in flops if k in times), key=lambda k: totals[k])
-    avg_ms = times[dom][0]
-    achieved = flops[dom] / (avg_ms * 1e-3) / 1e12
+    work = {
+        "fluid_force": pairs * (FORCE_PAIR_FLOP + RIEMANN_ITER_FLOP * k_r),
+        "pre_interaction": cand * (PRE_NEWTON_FLOP * k_h + PRE_FINAL_FLOP) + res["members"] * PRE_GRAD_FLOP,
+    }
+    if st.get("search_reach_tests"):
+        searches = max(1, times.get("search_pre", (0.0, 0))[1])
+        work["search_pre"] = st["search_reach_tests"] / searches * SEARCH_TEST_FLOP
     peak = PEAK_FP32_TFLOPS if form != "fp64" else PEAK_FP64_TFLOPS
+    out = {}
+    for k, flop in work.items():
+        if k not in times:
+            continue
+        avg_ms = times[k][0]
+        achieved = flop / (avg_ms * 1e-3) / 1e12
+        e = {"avg_ms": round(avg_ms, 4), "launches_per_step": round(times[k][1] / steps, 3),
+             "bound": "valu", "flop_per_launch": flop, "achieved_tflops": round(achieved, 3),
+             "peak_tflops": peak, "frac": round(achieved / peak, 4)}
+        tr = measured_traffic(k, workload, form)
+        if tr is not None:
+            e["traffic_bytes"] = tr
+            e["hbm_gbps"] = round(tr / (avg_ms * 1e-3) / 1e9, 1)
+            e["hbm_frac"] = round(tr / (avg_ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)
+        out[k] = e
+    return out
+
+
+def roofline(kernels: dict) -> dict:
+    """The dominant priced kernel (largest time per step)."""
+    dom = max(kernels, key=lambda k: kernels[k]["avg_ms"] * kernels[k]["launches_per_step"])
+    e = kernels[dom]
     return {
-        "kernel": dom, "bound": "mfma", "achieved": round(achieved, 3), "peak": peak,
-        "unit": "TFLOP/s", "frac": round(achieved / peak, 4),
-        "traffic": measured_traffic(dom, workload, form),
-        "flop_per_launch": flops[dom], "avg_ms": avg_ms,
-        "note": ("VALU-bound pair loop (FP32 vector, non-packed) priced against the MI355X FP32 peak; "
-                 "algorithmic FLOP from the measured pair and iteration counters"),
+        "kernel": dom, "bound": "valu", "achieved": e["achieved_tflops"], "peak": e["peak_tflops"],
+        "unit": "TFLOP/s", "frac": e["frac"], "traffic": e.get("traffic_bytes"),
+        "flop_per_launch": e["flop_per_launch"], "avg_ms": e["avg_ms"],
+        "note": ("VALU-issue-bound pair/search loop, algorithmic FLOP from the measured counters, priced "
+                 "against the MI355X vector peak (packed FP32 instructions are not used, so the attainable "
+                 "FP32 rate is half of it)"),
     }
 
 
-def cpu_baseline(seconds_hint: float = 15.0) -> dict:
-    """The oracle on a bounded sample: Sedov 3D GSPH 64^3, 4 steps."""
+def _checkers():
     sys.path.insert(0, str(REPO / "tests"))
-    from checkers import Oracle, _oracle_lib  # noqa: F401
-
+    import checkers  # noqa: E402
     import ctypes
     lib = ctypes.CDLL(str(REPO / "oracle" / "liboracle.so"))
     lib.orc_num_threads.restype = ctypes.c_int
-    cores = lib.orc_num_threads()
-    p, f = ics.sedov_3d(n1=64)
-    o = Oracle(p, f)
-    o.initialize()
-    steps = 4
+    return checkers, lib.orc_num_threads()
+
+
+def accuracy_gate(workload: str, steps: int, drift_gpu: float) -> dict:
+    """The oracle (FP64, plain guess, reference arithmetic) runs the same
+    initialisation and `steps` steps of the workload on the host; the GPU's
+    energy drift over the same steps must be within ENERGY_GATE of it."""
+    checkers, cores = _checkers()
+    p, f = make_workload(workload, "fp64")
+    o = checkers.Oracle(p, f)
     t0 = time.perf_counter()
-    o.integrate(steps)
+    o.initialize()
+    e0 = o.energy()
     t1 = time.perf_counter()
+    o.integrate(steps)
+    t2 = time.perf_counter()
+    e1 = o.energy()
+    o.close()
+    drift_o = float((e1[3] - e0[3]) / abs(e0[3]))
+    diff = abs(drift_gpu - drift_o)
+    return {"steps": steps, "drift_gpu": drift_gpu, "drift_oracle": drift_o, "abs_diff": diff,
+            "bar": ENERGY_GATE, "pass": bool(diff <= ENERGY_GATE),
+            "oracle_s_init": round(t1 - t0, 2), "oracle_s_per_step": round((t2 - t1) / steps, 3),
+            "oracle_cores": cores, "n": f.shape[1]}
+
+
+def cpu_reference(workload: str, steps: int = 3) -> dict:
+    """The reference's own modules and Simulation (oracle/_ref/libsphref_d3.so:
+    src/bhtree.cpp, src/pre_interaction.cpp, src/gsph/*.cpp, src/timestep.cpp,
+    ... compiled unmodified with g++ -O2 -fopenmp -ffp-contract=off, driven
+    by the Solver::initialize / Solver::integrate sequence), OpenMP on all
+    host cores: initialisation, then `steps` timed steps."""
+    checkers, cores = _checkers()
+    if not checkers.ref_available(3):
+        return None
+    p, f = make_workload(workload, "fp64")
+    r = checkers.Reference(p, f)
+    t0 = time.perf_counter()
+    r.initialize()
+    t1 = time.perf_counter()
+    r.integrate(steps)
+    t2 = time.perf_counter()
+    r.close()
     n = f.shape[1]
-    return {"value": round(n * steps / (t1 - t0), 1), "unit": "particle-steps/s", "cores": cores,
-            "kind": "port", "sample": f"oracle, Sedov 3D GSPH 64^3 (262144 particles), {steps} steps, "
-                                      f"{t1 - t0:.1f} s"}
+    return {"value": round(n * steps / (t2 - t1), 1), "unit": "particle-steps/s", "cores": cores,
+            "kind": "reference",
+            "sample": (f"reference modules (oracle/_ref/libsphref_d3.so, g++ -O2 -fopenmp -ffp-contract=off), "
+                       f"{workload} ({n} particles), FP64 plain guess: initialise {t1 - t0:.1f} s, "
+                       f"then {steps} steps timed, {t2 - t1:.1f} s")}
 
 
 def main() -> None:
@@ -191,7 +270,8 @@ def main() -> None:
     if args.form == "learned":
         mlp_params, train_info = train_learned()
     res = run_form(args.workload, args.form, args.steps, args.warmup, mlp_params)
-    rl = roofline(res, args.steps, args.workload, args.form)
+    kernels = kernel_models(res, args.steps, args.workload, args.form)
+    rl = roofline(kernels)
     out = {
         "metric": "particle-steps/s at 1M 3D GSPH: learned-FP32 vs plain, energy drift <=1e-4",
         "value": round(res["value"], 1),
@@ -207,8 +287,10 @@ def main() -> None:
         "data": "synthetic",
         "config": {"workload": args.workload, "particles": res["n"], "form": args.form,
                    "sph": "gsph-2nd-order", "riemann": "iterative", "kernel": "cubic_spline",
-                   "neighbor_number": 50, "leaf": 32, "periodic": True},
+                   "neighbor_number": 50, "leaf": 32, "periodic": True,
+                   "p_background": ics.SEDOV_P_BACKGROUND, "p_background_reason": P_BACKGROUND_NOTE},
         "roofline": rl,
+        "kernels": kernels,
     }
     st = res["stats"]
     # tree searches per step: the pre-interaction's, plus the FluidForce one
@@ -232,6 +314,9 @@ def main() -> None:
         "force_list_searches": st["flist_searches"],
         "h_beyond_search_radius": st["flist_h_beyond"],
     }
+    if st.get("search_reach_tests"):
+        out["counters"]["search_reach_tests_per_particle"] = round(
+            st["search_reach_tests"] / (max(searches, 1.0) * res["n"] * args.steps), 2)
     if train_info is not None:
         out["training"] = train_info
     out["energy_drift"] = res["drift"]
@@ -254,7 +339,23 @@ def main() -> None:
     out["hbm_copy_probe"] = {"bytes_each_way": nbytes, "ms": round(ms, 4),
                              "GBps_read_plus_write": round(2 * nbytes / (ms * 1e-3) / 1e9, 1)}
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline()
+        # the energy gate at the benchmark size (the GPU drift covers the
+        # warm-up and timed steps from the initialised state), then the
+        # reference's own CPU path on a bounded sample of the same workload
+        gate = accuracy_gate(args.workload, args.warmup + args.steps, res["drift"])
+        out["accuracy"] = gate
+        ref = cpu_reference(args.workload)
+        port_value = round(gate["n"] / gate["oracle_s_per_step"], 1)
+        out["cpu_port"] = {"value": port_value, "unit": "particle-steps/s", "cores": gate["oracle_cores"],
+                           "kind": "port",
+                           "sample": f"oracle (oracle/sph_oracle.cpp, OpenMP), {args.workload}, the "
+                                     f"{gate['steps']} steps of the accuracy gate"}
+        if ref is not None:
+            out["cpu_baseline"] = ref
+            out["cpu_port"]["ratio_to_reference"] = round(port_value / ref["value"], 3)
+            out["gpu_over_reference"] = round(res["value"] / ref["value"], 1)
+        else:
+            out["cpu_baseline"] = dict(out["cpu_port"])
     print(json.dumps(out), flush=True)
 
 

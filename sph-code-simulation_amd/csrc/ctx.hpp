@@ -220,6 +220,8 @@ int tree_dump(sphgpu_ctx *c, int64_t *n_nodes, int32_t *level, int32_t *count, i
 
 // neighbor.hip
 int lists_alloc(sphgpu_ctx *c, Lists &l, int cap);
+/* doubles l's capacity up to the reference's nn * 1000; SPHGPU_E_NEIGHBOR_OVERFLOW at that size */
+int lists_grow(sphgpu_ctx *c, Lists &l);
 void lists_free(Lists &l);
 // is_ij = 0: radius hs[i] (r2 < hs^2);  is_ij = 1: sqrt(r2) < max(h_i, h_j), r2 != 0
 // defer: launch only; the overflow flag is read later (pre_resolve)

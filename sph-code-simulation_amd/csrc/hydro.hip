@@ -1526,11 +1526,8 @@ int pre_resolve(sphgpu_ctx *c)
     if (w[0]) {
         // the search overflowed: grow its lists (the reference's capacity is
         // nn * 1000) and run the pre-interaction again from the same radii
-        const long long want = 2ll * c->lpre.cap;
-        if (want > 1000ll * c->prm.neighbor_number)
-            return fail(c, SPHGPU_E_NEIGHBOR_OVERFLOW, "Neighbor list overflow: increase neighbor_list_size in defines.hpp");
-        int rc = lists_alloc(c, c->lpre, (int)want);
-        if (rc) return fail(c, rc, "neighbour list reallocation failed");
+        int rc = lists_grow(c, c->lpre);
+        if (rc) return rc;
         return pre_core(c, false, false);
     }
     if (c->flist == 1) c->flist = w[2] ? 0 : 2;

@@ -25,6 +25,7 @@
 //   is_ij = 1 :  sqrt(r_ij^2) < max(h_i, h_j)  and  r_ij^2 != 0
 // The group test only prunes; it never decides membership.
 #include <hip/hip_runtime.h>
+#include <algorithm>
 
 #include "ctx.hpp"
 
@@ -623,6 +624,19 @@ int lists_alloc(sphgpu_ctx *c, Lists &l, int cap)
     return 0;
 }
 
+int lists_grow(sphgpu_ctx *c, Lists &l)
+{
+    // the reference's list holds nn * neighbor_list_size = nn * 1000 entries
+    // and fails only when a particle has more (src/bhtree.cpp:300-302,
+    // include/defines.hpp:35): double up to that size, then stop there
+    const long long limit = 1000ll * c->prm.neighbor_number;
+    if (l.cap >= limit)
+        return fail(c, SPHGPU_E_NEIGHBOR_OVERFLOW, "Neighbor list overflow: increase neighbor_list_size in defines.hpp");
+    const long long want = std::min(2ll * l.cap, limit);
+    const int rc = lists_alloc(c, l, (int)want);
+    return rc ? fail(c, rc, "neighbour list reallocation failed") : 0;
+}
+
 void lists_free(Lists &l)
 {
     if (l.idx) hipFree(l.idx);
@@ -686,12 +700,9 @@ int search(sphgpu_ctx *c, Lists &l, int is_ij, const double *radius, bool defer)
         HIPCHK(c, hipMemcpyAsync(&over, c->dflag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
         if (!over) return 0;
-        // grow the list and search again (the reference's capacity is nn * 1000)
-        const long long want = 2ll * l.cap;
-        if (want > 1000ll * c->prm.neighbor_number)
-            return fail(c, SPHGPU_E_NEIGHBOR_OVERFLOW, "Neighbor list overflow: increase neighbor_list_size in defines.hpp");
-        int rc = lists_alloc(c, l, (int)want);
-        if (rc) return fail(c, rc, "neighbour list reallocation failed");
+        // grow the list and search again
+        int rc = lists_grow(c, l);
+        if (rc) return rc;
     }
 }
 

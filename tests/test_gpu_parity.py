@@ -19,7 +19,7 @@ import pytest
 
 import ics
 from checkers import Oracle
-from parity_util import SMALL, force_pairs, rel_err
+from parity_util import SMALL, compare_fields, force_pairs, rel_err
 
 pytestmark = pytest.mark.gpu
 
@@ -126,10 +126,15 @@ def test_leaf_members(case):
     for k in o_leaf:
         cnt = int(rec[k, 1])
         off = int(rec[k, 12])
-        om = np.sort(mem[off:off + cnt])
         s = d["first"][k]
         gm = np.sort(d["members"][s:s + cnt])
-        np.testing.assert_array_equal(om, gm)
+        # the reference's linked-list order: the root list is the particle
+        # array (src/bhtree.cpp:103-107) and every assign() prepends to the
+        # child's list (:245-246), so each level reverses its parent's order;
+        # a leaf at level l (root = 1) holds ascending ids for odd l,
+        # descending for even l
+        want = gm if d["level"][k] % 2 == 1 else gm[::-1]
+        np.testing.assert_array_equal(mem[off:off + cnt], want)
 
 
 @pytest.mark.parametrize("case", list(SMALL))
@@ -231,60 +236,6 @@ def test_force_lists_many_candidates(case):
     assert stats["flist_searches"] == 0
 
 
-def force_errors(gf, of, d):
-    """Acceleration and du/dt errors per particle, normalised by the size of
-    one pair's pressure term, P/(rho h) (and P c/(rho h) for du/dt), plus the
-    oracle value itself.  Net forces of equilibrium regions are roundoff-level
-    sums of such terms, so their relative error is not meaningful."""
-    P, rho, h, c = of[ics.F_PRES], of[ics.F_DENS], of[ics.F_SML], of[ics.F_SOUND]
-    ga, oa = gf[ics.F_ACC:ics.F_ACC + d], of[ics.F_ACC:ics.F_ACC + d]
-    amag = np.sqrt((oa ** 2).sum(axis=0))
-    scale_a = np.abs(P) / (rho * h) + amag
-    e_acc = np.sqrt(((ga - oa) ** 2).sum(axis=0)) / scale_a
-    scale_u = np.abs(P) * c / (rho * h) + np.abs(of[ics.F_DENE])
-    e_dene = np.abs(gf[ics.F_DENE] - of[ics.F_DENE]) / scale_u
-    return e_acc, e_dene
-
-
-def _compare_fields(case, p, gf, of, prec):
-    d = p["dim"]
-    t = p["sph_type"]
-    if prec == ics.PAIR_FP64:
-        tol_rho, tol_f = 1e-9, 1e-9
-    else:
-        tol_rho, tol_f = 5e-5, 5e-5
-    e_sml = rel_err(gf[ics.F_SML], of[ics.F_SML])
-    e_rho = rel_err(gf[ics.F_DENS], of[ics.F_DENS])
-    e_pres = rel_err(gf[ics.F_PRES], of[ics.F_PRES])
-    e_acc, e_dene = force_errors(gf, of, d)
-    print(f"{case} prec={prec}: sml {e_sml.max():.3e} (p99.9 {np.quantile(e_sml, 0.999):.3e}) "
-          f"dens {e_rho.max():.3e} pres {e_pres.max():.3e} acc {e_acc.max():.3e} "
-          f"(p99.9 {np.quantile(e_acc, 0.999):.3e}) dene {e_dene.max():.3e}")
-    nbad = int((e_sml > tol_rho).sum())
-    assert nbad <= max(1, int(1e-3 * len(e_sml))), f"{nbad} smoothing lengths off"
-    assert np.quantile(e_rho, 0.999) < tol_rho
-    assert np.quantile(e_pres, 0.999) < tol_rho
-    assert np.quantile(e_acc, 0.999) < tol_f
-    assert np.quantile(e_dene, 0.999) < tol_f
-    if t == ics.SSPH:
-        e_gh = rel_err(gf[ics.F_GRADH], of[ics.F_GRADH])
-        assert np.quantile(e_gh, 0.999) < tol_rho * 10
-    elif t in (ics.DISPH, ics.GDISPH):
-        # DISPH grad-h (d_pre_interaction.cpp) is a ratio whose denominator
-        # 1 + h/(D n) sum dW/dh nearly cancels on a uniform lattice, so its
-        # relative error is not meaningful; it enters the force only through
-        # f_ij = 1 - gradh_i / (m_j u_j), so it is compared on the m u scale
-        mu = of[ics.F_MASS] * of[ics.F_ENE]
-        e_gh = np.abs(gf[ics.F_GRADH] - of[ics.F_GRADH]) / (np.abs(of[ics.F_GRADH]) + mu)
-        print(f"  gradh on the m u scale {e_gh.max():.3e}")
-        assert np.quantile(e_gh, 0.999) < tol_rho * 10
-    if p["use_gravity"]:
-        e_phi = rel_err(gf[ics.F_PHI], of[ics.F_PHI])
-        print(f"  phi {e_phi.max():.3e}")
-        assert np.quantile(e_phi, 0.999) < (1e-9 if prec == ics.PAIR_FP64 else 1e-5)
-    np.testing.assert_array_equal(gf[ics.F_ID], of[ics.F_ID])
-
-
 @pytest.mark.parametrize("case", list(SMALL))
 @pytest.mark.parametrize("prec", [ics.PAIR_FP64, ics.PAIR_FP32])
 def test_initialize_fields(case, prec):
@@ -295,7 +246,7 @@ def test_initialize_fields(case, prec):
     g = gpu_ctx(p, f, pair_precision=prec)
     g.initialize()
     gf = g.download()
-    _compare_fields(case, p, gf, of, prec)
+    compare_fields(case, p, gf, of, prec)
     # neighbour counts are exact whenever h is (FP64 mode)
     if prec == ics.PAIR_FP64:
         same_h = gf[ics.F_SML] == of[ics.F_SML]
@@ -322,7 +273,7 @@ def test_short_run(case, prec):
     print(f"{case} prec={prec}: dt oracle {o.dt:.6e} gpu {g.dt:.6e}; t {o.time:.6e} {g.time:.6e}")
     print(f"  E oracle {eo[3]:.12e} gpu {eg[3]:.12e} (E0 {e0[3]:.12e})")
     assert abs(g.dt - o.dt) <= (1e-9 if prec == ics.PAIR_FP64 else 1e-4) * o.dt
-    _compare_fields(case, p, gf, of, prec)
+    compare_fields(case, p, gf, of, prec)
     drift_o = (eo[3] - e0[3]) / abs(e0[3])
     drift_g = (eg[3] - e0[3]) / abs(e0[3])
     assert abs(drift_g - drift_o) < (1e-10 if prec == ics.PAIR_FP64 else 1e-5)
@@ -426,3 +377,37 @@ def test_deferred_search_overflow_reruns_pre():
     # pre-interaction kernel ran twice (clipped lists, then full ones)
     assert sa["pre_candidates"] == sb["pre_candidates"] > 20 * 5000
     assert sa["pre_newton_iters"] > sb["pre_newton_iters"]
+
+
+@pytest.mark.parametrize("n1,fits", [(33, True), (37, False)])
+def test_list_grows_to_reference_limit(n1, fits):
+    """A particle whose density is cut 1e6x gets a pre-interaction search
+    radius that covers the whole box, so its candidate list holds every
+    particle.  The reference's list takes nn * 1000 = 50,000 entries
+    (include/defines.hpp:35) and throws only beyond that
+    (src/bhtree.cpp:300-302): with 35,937 particles the GPU lists grow to fit
+    and the fields equal the oracle's; with 50,653 the GPU returns the
+    reference's overflow error."""
+    p, f = ics.sedov_3d(n1=n1)
+    f = f.copy()
+    f[ics.F_DENS, 0] *= 1e-6
+    g = gpu_ctx(p, f, pair_precision=ics.PAIR_FP64)
+    g.build_tree()
+    if not fits:
+        with pytest.raises(sphgpu.SphGpuError, match="Neighbor list overflow"):
+            g.pre_interaction()
+        return
+    g.reset_stats()
+    g.pre_interaction()
+    st = g.stats()
+    n = f.shape[1]
+    assert st["pre_candidates"] >= 2 * n - 1  # particle 0 sees all n
+    o = Oracle(p, f)
+    o.make_tree()
+    o.pre_interaction()
+    of, gf = o.get_fields(), g.download()
+    for r in (ics.F_SML, ics.F_DENS, ics.F_PRES):
+        e = rel_err(gf[r], of[r])
+        assert e.max() < 1e-9, (r, float(e.max()))
+    print(f"n={n}: {st['pre_candidates']} candidates, particle 0 h {gf[ics.F_SML, 0]:.6e} "
+          f"(oracle {of[ics.F_SML, 0]:.6e})")

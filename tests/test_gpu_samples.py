@@ -14,7 +14,7 @@ import ics
 import samples
 import sphgpu
 from checkers import Oracle
-from test_gpu_parity import _compare_fields
+from parity_util import compare_fields
 
 pytestmark = pytest.mark.gpu
 
@@ -54,4 +54,4 @@ def test_sample_matches_oracle(name, cfg, size):
     gf, of = g.download(), o.get_fields()
     assert np.isfinite(gf).all()
     assert abs(g.dt - o.dt) <= 1e-9 * o.dt
-    _compare_fields(f"{name}/{cfg}", p, gf, of, ics.PAIR_FP64)
+    compare_fields(f"{name}/{cfg}", p, gf, of, ics.PAIR_FP64)
