@@ -234,6 +234,19 @@ int sphgpu_riemann_batch(sphgpu_ctx *ctx, int solver, int precision, int n,
                          const double *left, const double *right,
                          double *pstar, double *vstar, int32_t *iters,
                          int32_t *learned_used);
+/* sphgpu_riemann_batch without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory while they
+ * run (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The states are staged in a static device area of the
+ * library (one per GPU, held until the process ends) instead of buffers
+ * allocated per call, so n is limited to SPHGPU_RIEMANN_STATIC_MAX
+ * (SPHGPU_E_ARG beyond it).  All calls on one GPU share that area: calls that
+ * overlap on the same GPU (several host threads or contexts) race. */
+#define SPHGPU_RIEMANN_STATIC_MAX 32768
+int sphgpu_riemann_batch_static(sphgpu_ctx *ctx, int solver, int precision, int n,
+                                const double *left, const double *right,
+                                double *pstar, double *vstar, int32_t *iters,
+                                int32_t *learned_used);
 
 /* ---- learned Riemann starting guess ------------------------------------- */
 /* Pair-state log written by the GSPH force kernel: rows of 7 floats

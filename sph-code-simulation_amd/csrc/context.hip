@@ -534,6 +534,15 @@ int sphgpu_riemann_batch(sphgpu_ctx *c, int solver, int precision, int n, const 
     return riemann_batch(c, solver, precision, n, left, right, pstar, vstar, iters, learned_used);
 }
 
+int sphgpu_riemann_batch_static(sphgpu_ctx *c, int solver, int precision, int n, const double *left,
+                                const double *right, double *pstar, double *vstar, int32_t *iters,
+                                int32_t *learned_used)
+{
+    if (!c || n < 0 || (n > 0 && (!left || !right || !pstar || !vstar))) return SPHGPU_E_ARG;
+    if (n == 0) return 0;
+    return riemann_batch_static(c, solver, precision, n, left, right, pstar, vstar, iters, learned_used);
+}
+
 int sphgpu_enable_timing(sphgpu_ctx *c, int on)
 {
     if (!c) return SPHGPU_E_ARG;

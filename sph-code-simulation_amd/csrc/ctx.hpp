@@ -242,6 +242,10 @@ int fluid_force(sphgpu_ctx *c);
 int riemann_batch(sphgpu_ctx *c, int solver, int precision, int n, const double *left,
                   const double *right, double *pstar, double *vstar, int32_t *iters,
                   int32_t *learned_used);
+// the same without allocating: staging in the library's static device area
+int riemann_batch_static(sphgpu_ctx *c, int solver, int precision, int n, const double *left,
+                         const double *right, double *pstar, double *vstar, int32_t *iters,
+                         int32_t *learned_used);
 
 // gravity.hip
 int gravity(sphgpu_ctx *c);

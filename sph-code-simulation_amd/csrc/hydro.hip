@@ -18,6 +18,8 @@
 #include <climits>
 #include <cmath>
 #include <cstring>
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "ctx.hpp"
@@ -1625,22 +1627,23 @@ int fluid_force(sphgpu_ctx *c)
     return 0;
 }
 
-int riemann_batch(sphgpu_ctx *c, int solver, int precision, int n, const double *left, const double *right,
-                  double *pstar, double *vstar, int32_t *iters, int32_t *learned_used)
+namespace {
+
+// Stages the states in d (10 n doubles: L, R, p*, v*) and di (2 n ints:
+// iterations, learned-guess use), solves and copies the results back.
+int riemann_batch_staged(sphgpu_ctx *c, int solver, int precision, int n, const double *left, const double *right,
+                         double *pstar, double *vstar, int32_t *iters, int32_t *learned_used, double *d, int *di)
 {
-    double *d = nullptr;
-    int *di = nullptr;
     const size_t b = sizeof(double) * (size_t)n;
-    HIPCHK(c, hipMalloc((void **)&d, b * 12));
-    HIPCHK(c, hipMalloc((void **)&di, 2 * sizeof(int) * (size_t)n));
     double *L = d, *R = d + 4ull * n, *P = d + 8ull * n, *V = d + 9ull * n;
     HIPCHK(c, hipMemcpyAsync(L, left, b * 4, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(R, right, b * 4, hipMemcpyHostToDevice, c->stream));
     const dim3 g(grid_for(n, 256)), bl(256);
     const int ref = riemann_refine64(c);
     const MLPDev m = mlp_dev(c);
-    if (precision != SPHGPU_PAIR_FP64 && solver == SPHGPU_RIEMANN_ITERATIVE &&
-        c->prm.riemann_guess == SPHGPU_GUESS_LEARNED && m.w) {
+    const bool use_mlp = precision != SPHGPU_PAIR_FP64 && solver == SPHGPU_RIEMANN_ITERATIVE &&
+                         c->prm.riemann_guess == SPHGPU_GUESS_LEARNED && m.w;
+    if (use_mlp) {
         hipLaunchKernelGGL(k_riemann_batch_learned, g, bl, 0, c->stream, n, (const double *)L, (const double *)R,
                            (float)c->prm.gamma, ref, m.img, P, V, di, di + n);
     } else if (precision == SPHGPU_PAIR_FP64) {
@@ -1659,16 +1662,77 @@ int riemann_batch(sphgpu_ctx *c, int solver, int precision, int n, const double 
     HIPCHK(c, hipMemcpyAsync(vstar, V, b, hipMemcpyDeviceToHost, c->stream));
     if (iters) HIPCHK(c, hipMemcpyAsync(iters, di, sizeof(int) * (size_t)n, hipMemcpyDeviceToHost, c->stream));
     if (learned_used) {
-        if (precision != SPHGPU_PAIR_FP64 && solver == SPHGPU_RIEMANN_ITERATIVE &&
-            c->prm.riemann_guess == SPHGPU_GUESS_LEARNED && m.w)
+        if (use_mlp)
             HIPCHK(c, hipMemcpyAsync(learned_used, di + n, sizeof(int) * (size_t)n, hipMemcpyDeviceToHost, c->stream));
         else
             std::memset(learned_used, 0, sizeof(int32_t) * (size_t)n);
     }
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+// Staging area of the allocation-free batch: 10 doubles and 2 ints per
+// problem for up to SPHGPU_RIEMANN_STATIC_MAX problems (2.75 MiB), one copy
+// per device for the life of the process.  Its name is a host-side stand-in;
+// device code reaches it only through the address hipGetSymbolAddress returns
+// for the current device.
+constexpr size_t kStaticBytes = (size_t)SPHGPU_RIEMANN_STATIC_MAX * (10 * sizeof(double) + 2 * sizeof(int));
+__device__ __align__(256) unsigned char scratch[kStaticBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+// this device's address of `scratch`, looked up once per device (with the
+// device current) and kept for later calls
+int static_scratch(sphgpu_ctx *c, void **out)
+{
+    int dev = -1;
+    HIPCHK(c, hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(c, SPHGPU_E_ARG, "device index beyond the static staging table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return fail(c, SPHGPU_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[dev] = ptr;
+    }
+    *out = g_scratch_ptr[dev];
+    return 0;
+}
+
+} // namespace
+
+int riemann_batch(sphgpu_ctx *c, int solver, int precision, int n, const double *left, const double *right,
+                  double *pstar, double *vstar, int32_t *iters, int32_t *learned_used)
+{
+    double *d = nullptr;
+    int *di = nullptr;
+    HIPCHK(c, hipMalloc((void **)&d, sizeof(double) * 10 * (size_t)n));
+    if (hipMalloc((void **)&di, 2 * sizeof(int) * (size_t)n) != hipSuccess) {
+        hipFree(d);
+        return fail(c, SPHGPU_E_NOMEM, "riemann batch staging allocation failed");
+    }
+    const int rc = riemann_batch_staged(c, solver, precision, n, left, right, pstar, vstar, iters, learned_used, d, di);
     hipFree(d);
     hipFree(di);
-    return 0;
+    return rc;
+}
+
+int riemann_batch_static(sphgpu_ctx *c, int solver, int precision, int n, const double *left, const double *right,
+                         double *pstar, double *vstar, int32_t *iters, int32_t *learned_used)
+{
+    if (n > SPHGPU_RIEMANN_STATIC_MAX)
+        return fail(c, SPHGPU_E_ARG, "sphgpu_riemann_batch_static: n = " + std::to_string(n) +
+                                         " exceeds SPHGPU_RIEMANN_STATIC_MAX = " +
+                                         std::to_string(SPHGPU_RIEMANN_STATIC_MAX));
+    void *base = nullptr;
+    const int rc = static_scratch(c, &base);
+    if (rc) return rc;
+    double *d = (double *)base;
+    int *di = (int *)(d + 10ull * n);
+    return riemann_batch_staged(c, solver, precision, n, left, right, pstar, vstar, iters, learned_used, d, di);
 }
 
 } // namespace sphgpu

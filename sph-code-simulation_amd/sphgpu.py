@@ -15,6 +15,8 @@ import numpy as np
 PKG = Path(__file__).resolve().parent
 LIB_PATH = PKG / "libsphgpu.so"
 NFIELDS = 41
+# problems per call of sphgpu_riemann_batch_static (SPHGPU_RIEMANN_STATIC_MAX)
+RIEMANN_STATIC_MAX = 32768
 
 
 class Params(C.Structure):
@@ -294,6 +296,7 @@ def load_library() -> C.CDLL:
         "sphgpu_neighbor_sets": (ip, [vp, dp, ip, ip, i32p, i32p]),
         "sphgpu_tree_dump": (ip, [vp, i64p, i32p, i32p, i32p, i32p, dp, dp, dp, dp, dp, i32p]),
         "sphgpu_riemann_batch": (ip, [vp, ip, ip, ip, dp, dp, dp, dp, i32p, i32p]),
+        "sphgpu_riemann_batch_static": (ip, [vp, ip, ip, ip, dp, dp, dp, dp, i32p, i32p]),
         "sphgpu_riemann_log_enable": (ip, [vp, C.c_int64, ip]),
         "sphgpu_riemann_log_read": (ip, [vp, fp, C.c_int64, i64p]),
         "sphgpu_mlp_init": (ip, [vp, C.c_uint64]),
@@ -339,7 +342,7 @@ EXPORTED = (
     "sphgpu_gravity", "sphgpu_timestep", "sphgpu_predict", "sphgpu_correct",
     "sphgpu_initialize", "sphgpu_integrate", "sphgpu_energy_momentum",
     "sphgpu_get_stats", "sphgpu_reset_stats", "sphgpu_neighbor_sets",
-    "sphgpu_tree_dump", "sphgpu_riemann_batch", "sphgpu_riemann_log_enable",
+    "sphgpu_tree_dump", "sphgpu_riemann_batch", "sphgpu_riemann_batch_static", "sphgpu_riemann_log_enable",
     "sphgpu_riemann_log_read", "sphgpu_mlp_init", "sphgpu_mlp_set_params",
     "sphgpu_mlp_get_params", "sphgpu_mlp_param_count", "sphgpu_mlp_train",
     "sphgpu_mlp_loss_grad", "sphgpu_mlp_predict", "sphgpu_kernel_times", "sphgpu_copy_probe",
@@ -475,7 +478,10 @@ class Context:
         return out
 
     def riemann_batch(self, solver: int, precision: int, left: np.ndarray, right: np.ndarray,
-                      with_usage: bool = False):
+                      with_usage: bool = False, static: bool = False):
+        """Batched Riemann problems; static=True calls the allocation-free
+        entry point (sphgpu_riemann_batch_static, at most RIEMANN_STATIC_MAX
+        problems)."""
         left = np.ascontiguousarray(left, dtype=np.float64)
         right = np.ascontiguousarray(right, dtype=np.float64)
         n = left.shape[0]
@@ -484,7 +490,8 @@ class Context:
         vs = np.zeros(n)
         it = np.zeros(n, dtype=np.int32)
         used = np.zeros(n, dtype=np.int32)
-        self._check(self.lib.sphgpu_riemann_batch(
+        fn = self.lib.sphgpu_riemann_batch_static if static else self.lib.sphgpu_riemann_batch
+        self._check(fn(
             self.h, int(solver), int(precision), n, _dp(left), _dp(right), _dp(ps), _dp(vs),
             it.ctypes.data_as(C.POINTER(C.c_int32)), used.ctypes.data_as(C.POINTER(C.c_int32))),
             "riemann_batch")
