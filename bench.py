@@ -17,6 +17,12 @@ Prints one JSON line (the driver's contract), with these extra objects:
                 the same initialisation and warm-up + timed steps on the host
                 (FP64, plain guess) and |drift_GPU - drift_oracle| is reported
                 against the 1e-4 bar;
+  evrard_1m     the self-gravitating workload (3D Evrard collapse, 1M
+                particles, DISPH + tree gravity): step time, gravity's kernel
+                time and interactions, energy drift;
+  dropin        the reference's driver and Simulation with the GPU Module
+                adapters in place of its modules (the module substitution),
+                per-phase step time at the benchmark size;
   cpu_baseline  the reference's own CPU modules (oracle/_ref, compiled from
                 its unmodified sources) timed on a bounded sample of the same
                 workload on this host's cores; cpu_port is the oracle's time
@@ -76,6 +82,12 @@ def make_workload(name: str, form: str):
         p, f = ics.sedov_3d(n1=64, riemann_guess=guess)
     elif name == "sedov_3d_gsph_32k":
         p, f = ics.sedov_3d(n1=32, riemann_guess=guess)
+    elif name == "evrard_3d_disph_1m":
+        # the reference's own Evrard lattice (src/sample/evrard.cpp) at
+        # n_side 124: 998,592 particles; on a seeded radial-CDF draw the
+        # reference itself goes NaN within the first steps from 100k
+        # particles on (tests/test_oracle_reference.py)
+        p, f = ics.evrard_lattice_3d(124)
     else:
         raise SystemExit(f"unknown workload {name}")
     p["pair_precision"] = ics.PAIR_FP64 if form == "fp64" else ics.PAIR_FP32
@@ -95,7 +107,7 @@ def train_learned(log_workload: str = "sedov_3d_gsph_262k", log_steps: int = 4) 
     return w, info
 
 
-def run_form(name: str, form: str, steps: int, warmup: int, mlp_params=None):
+def run_form(name: str, form: str, steps: int, warmup: int, mlp_params=None, kernel_table: bool = True):
     p, f = make_workload(name, form)
     n = f.shape[1]
     ctx = sphgpu.Context(p, n)
@@ -108,22 +120,87 @@ def run_form(name: str, form: str, steps: int, warmup: int, mlp_params=None):
         ctx.integrate(warmup)
     e0 = ctx.energy_momentum()
     ctx.reset_stats()
-    ctx.enable_timing(True)
+    # the timed steps carry HIP events around the pair kernels only (each
+    # event adds a gap to the stream)
+    ctx.enable_timing(2)
     t0 = time.perf_counter()
     ctx.integrate(steps)
     t1 = time.perf_counter()
     times = ctx.kernel_times()
-    ctx.enable_timing(False)
+    ctx.enable_timing(0)
     stats = ctx.stats()
     e1 = ctx.energy_momentum()
     sim_t = ctx.time
     # members per particle (the neighbour counts of the last PreInteraction)
     members = float(ctx.download()[ics.F_NEIGHBOR].sum())
+    # a further pass of the same length with events around every kernel
+    # group: the per-kernel table of the whole step (not the headline)
+    times_all, t2, t3 = times, 0.0, 0.0
+    if kernel_table:
+        ctx.enable_timing(1)
+        t2 = time.perf_counter()
+        ctx.integrate(steps)
+        t3 = time.perf_counter()
+        times_all = ctx.kernel_times()
+        ctx.enable_timing(0)
     ctx.close()
     wall = t1 - t0
-    return dict(n=n, wall=wall, value=n * steps / wall, times=times, stats=stats, e0=e0, e1=e1, time=sim_t,
-                members=members,
+    return dict(n=n, wall=wall, value=n * steps / wall, times=times, times_all=times_all, stats=stats, e0=e0, e1=e1,
+                time=sim_t, members=members, ms_all_events=(t3 - t2) / steps * 1e3,
                 drift=float((e1[3] - e_init[3]) / abs(e_init[3])))
+
+
+def evrard_entry(steps: int, warmup: int, kernel_table: bool = True) -> dict:
+    """The self-gravitating workload: 3D Evrard collapse, 998,592 particles
+    on the reference's lattice,
+    DISPH with time-dependent viscosity, Wendland C4, tree gravity at
+    theta = 0.5 (FP32 pairs); whole steps with gravity's own kernel time and
+    the energy drift over the warm-up and timed steps."""
+    res = run_form("evrard_3d_disph_1m", "fp32", steps, warmup, kernel_table=kernel_table)
+    st, t = res["stats"], res["times"]
+    return {"workload": "evrard_3d_disph_1m", "particles": res["n"], "steps": steps, "warmup": warmup,
+            "value": round(res["value"], 1), "unit": "particle-steps/s",
+            "ms_per_step": round(res["wall"] / steps * 1e3, 4), "sim_time": res["time"],
+            "gravity_ms": round(t["gravity"][0], 4) if "gravity" in t else None,
+            "gravity_interactions_per_particle": round(st["gravity_interactions"] / (res["n"] * steps), 1),
+            "energy_drift": res["drift"],
+            "kernel_ms": {k: round(v[0], 4) for k, v in res["times_all"].items()}}
+
+
+def dropin_entry(workload: str, steps: int = 2) -> dict:
+    """The module substitution a reference user makes: the reference's own
+    Simulation and driver sequence (Solver::integrate, src/solver.cpp:966-978,
+    with its host predict/correct and host BHTree build) with the GPU Module
+    adapters (sph-code-simulation_amd/dropin) in place of TimeStep,
+    PreInteraction, FluidForce and GravityForce (oracle/_ref/libsphdropin_d3.so,
+    the reference sources compiled with the adapters).  Each phase of a step
+    is timed on the host, host-device transfers of the adapters included."""
+    checkers, cores = _checkers()
+    if not checkers.dropin_available(3):
+        return None
+    p, f = make_workload(workload, "fp32")
+    d = checkers.DropIn(p, f)
+    t0 = time.perf_counter()
+    d.initialize()
+    t_init = time.perf_counter() - t0
+    phases = ("timestep", "predict", "make_tree", "pre_interaction", "fluid_force", "gravity", "correct")
+    acc = {k: 0.0 for k in phases}
+    for _ in range(steps):
+        for k in phases:
+            t = time.perf_counter()
+            getattr(d, k)()
+            acc[k] += time.perf_counter() - t
+    d.close()
+    step_ms = sum(acc.values()) / steps * 1e3
+    gpu_ms = sum(acc[k] for k in ("timestep", "pre_interaction", "fluid_force", "gravity")) / steps * 1e3
+    n = f.shape[1]
+    return {"workload": workload, "particles": n, "steps": steps, "form": "fp32 pairs, plain guess",
+            "initialize_s": round(t_init, 2), "ms_per_step": round(step_ms, 1),
+            "value": round(n / (step_ms * 1e-3), 1), "unit": "particle-steps/s",
+            "gpu_modules_ms_per_step": round(gpu_ms, 1),
+            "phase_ms": {k: round(v / steps * 1e3, 1) for k, v in acc.items()},
+            "host_cores": cores,
+            "note": "host phases (predict, make_tree, correct) are the reference's own code on the host"}
 
 
 def measured_traffic(kernel: str, workload: str, form: str):
@@ -261,6 +338,10 @@ def main() -> None:
     ap.add_argument("--form", default="learned", choices=["fp32", "fp64", "learned"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-evrard", action="store_true")
+    ap.add_argument("--no-kernel-table", action="store_true",
+                    help="skip the second pass that times every kernel group (profiler runs: the last "
+                         "launches are then the timed steps)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported"}))
@@ -269,7 +350,8 @@ def main() -> None:
     mlp_params, train_info = None, None
     if args.form == "learned":
         mlp_params, train_info = train_learned()
-    res = run_form(args.workload, args.form, args.steps, args.warmup, mlp_params)
+    res = run_form(args.workload, args.form, args.steps, args.warmup, mlp_params,
+                   kernel_table=not args.no_kernel_table)
     kernels = kernel_models(res, args.steps, args.workload, args.form)
     rl = roofline(kernels)
     out = {
@@ -320,7 +402,11 @@ def main() -> None:
     if train_info is not None:
         out["training"] = train_info
     out["energy_drift"] = res["drift"]
-    out["kernel_ms"] = {k: round(v[0], 4) for k, v in res["times"].items()}
+    out["kernel_ms"] = {k: round(v[0], 4) for k, v in res["times_all"].items()}
+    if not args.no_kernel_table:
+        out["kernel_ms_note"] = ("every kernel group, from a further pass of the same number of steps with events "
+                                 "around each group (that pass ran at %.4f ms per step); the timed steps carry "
+                                 "events around the pair kernels only" % res["ms_all_events"])
     if not args.no_fp64 and args.form != "fp64":
         r64 = run_form(args.workload, "fp64", args.steps, args.warmup)
         out["plain_fp64"] = {"value": round(r64["value"], 1), "ms_per_step": round(r64["wall"] / args.steps * 1e3, 4),
@@ -328,7 +414,26 @@ def main() -> None:
                              "drift_difference": abs(res["drift"] - r64["drift"]),
                              "riemann_iters_per_call": round(r64["stats"]["riemann_iters"]
                                                              / max(1, r64["stats"]["riemann_calls"]), 4),
-                             "kernel_ms": {k: round(v[0], 4) for k, v in r64["times"].items()}}
+                             "riemann_wave_max_per_slot": round(r64["stats"]["riemann_wave_max_iters"] * 64
+                                                                / max(1, r64["stats"]["riemann_calls"]), 4),
+                             "kernel_ms": {k: round(v[0], 4) for k, v in r64["times_all"].items()}}
+    if args.form == "learned" and not args.no_fp64:
+        # the same FP32 pair kernels without the network: what the learned
+        # start changes on this workload
+        r32 = run_form(args.workload, "fp32", args.steps, args.warmup)
+        st32 = r32["stats"]
+        out["plain_fp32"] = {
+            "value": round(r32["value"], 1), "ms_per_step": round(r32["wall"] / args.steps * 1e3, 4),
+            "learned_over_plain": round(res["value"] / r32["value"], 4),
+            "riemann_iters_per_call": round(st32["riemann_iters"] / max(1, st32["riemann_calls"]), 4),
+            "riemann_wave_max_per_slot": round(st32["riemann_wave_max_iters"] * 64 / max(1, st32["riemann_calls"]), 4),
+            "fluid_force_ms": round(r32["times"]["fluid_force"][0], 4) if "fluid_force" in r32["times"] else None,
+            "note": ("the network starts the pairs its gate marks hard (pressure ratio above 4 or a velocity jump "
+                     "above the mean sound speed): on this workload a few per mille of the pairs, which do not "
+                     "set the wavefront's largest Newton count; its gain shows on the hard pairs themselves "
+                     "(training.held_out) and not in the step time")}
+    if not args.no_evrard and args.workload != "evrard_3d_disph_1m":
+        out["evrard_1m"] = evrard_entry(args.steps, args.warmup, not args.no_kernel_table)
     # plain copy of known size: attainable HBM bandwidth, and the calibration
     # kernel (k_copy_probe) for the FETCH_SIZE / WRITE_SIZE counters
     pc, _ = ics.sedov_3d(n1=4)
@@ -350,6 +455,9 @@ def main() -> None:
                            "kind": "port",
                            "sample": f"oracle (oracle/sph_oracle.cpp, OpenMP), {args.workload}, the "
                                      f"{gate['steps']} steps of the accuracy gate"}
+        di = dropin_entry(args.workload)
+        if di is not None:
+            out["dropin"] = di
         if ref is not None:
             out["cpu_baseline"] = ref
             out["cpu_port"]["ratio_to_reference"] = round(port_value / ref["value"], 3)

@@ -160,6 +160,9 @@ typedef struct sphgpu_stats {
                                     unusable (over 256 candidates, r = 0 pairs,
                                     list capacity)                             */
     int64_t flist_searches;      /* FluidForce calls that ran the is_ij search */
+    int64_t search_reach_tests;  /* members of the leaves each particle's own
+                                    search sphere reaches (the candidate tests
+                                    of a per-particle tree walk), summed      */
 } sphgpu_stats;
 
 /* ---- lifecycle ------------------------------------------------------------ */
@@ -277,6 +280,9 @@ int sphgpu_mlp_predict(sphgpu_ctx *ctx, const float *rows, int64_t n_rows,
 int sphgpu_kernel_times(sphgpu_ctx *ctx, char *name_buf, int buf_len,
                         double *avg_ms, int64_t *count, int max_kernels,
                         int *n_kernels);
+/* on = 0: no events; 1: every kernel group; 2: only the pair kernels
+ * (neighbour searches, PreInteraction, FluidForce, gravity walk), so that a
+ * timed run carries few events. */
 int sphgpu_enable_timing(sphgpu_ctx *ctx, int on);
 
 /* Measurement hook (no reference counterpart): copies nbytes (rounded down

@@ -19,9 +19,20 @@ int fail(sphgpu_ctx *c, int code, const std::string &msg)
     return code;
 }
 
+// the kernel groups that timing mode 2 keeps
+static bool pair_kernel(const char *name)
+{
+    static const char *const keep[] = {"search_pre", "search_force", "pre_interaction", "fluid_force", "gravity"};
+    for (const char *k : keep)
+        if (std::strcmp(name, k) == 0) return true;
+    return false;
+}
+
 void time_begin(sphgpu_ctx *c, const char *name)
 {
-    if (!c->timing) return;
+    c->timing_open = false;
+    if (!c->timing || (c->timing == 2 && !pair_kernel(name))) return;
+    c->timing_open = true;
     Timer t;
     if (c->event_pool.size() >= 2) {
         t.a = c->event_pool.back(); c->event_pool.pop_back();
@@ -37,7 +48,8 @@ void time_begin(sphgpu_ctx *c, const char *name)
 
 void time_end(sphgpu_ctx *c)
 {
-    if (!c->timing || c->pending.empty()) return;
+    if (!c->timing_open || c->pending.empty()) return;
+    c->timing_open = false;
     hipEventRecord(c->pending.back().b, c->stream);
 }
 
@@ -249,7 +261,8 @@ int sphgpu_create(const sphgpu_params *p, int n, sphgpu_ctx **out)
     ok = ok && alloc((void **)&c->dscal, sizeof(double) * 16) && alloc((void **)&c->dmin, sizeof(unsigned long long) * kStatCopies) &&
          alloc((void **)&c->dstats, sizeof(long long) * kStatSlots * kStatCopies) && alloc((void **)&c->dflag, sizeof(int) * 8);
     ok = ok && hipHostMalloc((void **)&c->hscal, sizeof(double) * 16, hipHostMallocDefault) == hipSuccess;
-    ok = ok && hipHostMalloc((void **)&c->hflag, sizeof(int) * 8, hipHostMallocDefault) == hipSuccess &&
+    ok = ok && hipHostMalloc((void **)&c->hflag, sizeof(int) * 8, hipHostMallocMapped | hipHostMallocCoherent) == hipSuccess &&
+         hipHostGetDevicePointer((void **)&c->hflag_dev, c->hflag, 0) == hipSuccess &&
          hipEventCreateWithFlags(&c->flag_ready, hipEventDisableTiming) == hipSuccess;
     ok = ok && tree_alloc(c) == 0;
     // neighbour lists: about 2.5x the expected candidate count nn * 1.2^D
@@ -325,7 +338,11 @@ int sphgpu_upload(sphgpu_ctx *c, const double *fields, uint64_t mask)
     hipLaunchKernelGGL(k_gather_fields, dim3(grid_for(c->n, 256)), dim3(256), 0, c->stream, fp,
                        (const double *)c->stage, (const int *)c->id, c->n, (unsigned long long)mask);
     HIPCHK(c, hipGetLastError());
-    // any host change of inputs invalidates the FP32 mirrors
+    // new positions make the tree stale (BHTree::make runs again before the
+    // next search); any host change of inputs invalidates the FP32 mirrors
+    // and with them a derived force list
+    const uint64_t pos_rows = SPHGPU_MASK(SPHGPU_F_POS) | SPHGPU_MASK(SPHGPU_F_POS + 1) | SPHGPU_MASK(SPHGPU_F_POS + 2);
+    if (mask & pos_rows) c->tree.valid = false;
     if (refresh_mirrors(c)) return SPHGPU_E_HIP;
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return 0;
@@ -470,6 +487,7 @@ int sphgpu_get_stats(sphgpu_ctx *c, sphgpu_stats *out)
     out->flist_h_beyond = h[18];
     out->flist_other = h[19];
     out->flist_searches = c->flist_searches;
+    out->search_reach_tests = h[20];
     return 0;
 }
 
@@ -547,7 +565,7 @@ int sphgpu_enable_timing(sphgpu_ctx *c, int on)
 {
     if (!c) return SPHGPU_E_ARG;
     collect_times(c);
-    c->timing = on != 0;
+    c->timing = on < 0 ? 0 : on > 2 ? 1 : on;
     c->ktimes.clear();
     return 0;
 }

@@ -48,6 +48,15 @@ template <typename T> struct RecView {
     const V4 *r[R_NREC];
 };
 
+// Node record of the gravity walk: centre of mass, edge^2, mass, member
+// range [start, end), subtree end (skip) and signed level (negative: leaf);
+// 64 bytes, one wave-uniform load per visited node.
+struct alignas(16) GravNode {
+    double cx, cy, cz, l2;
+    double mass, pad;
+    int start, end, skip, level;
+};
+
 struct Tree {
     // per sorted particle
     unsigned long long *key = nullptr, *key_alt = nullptr;
@@ -79,6 +88,9 @@ struct Tree {
     // hierarchy of block reductions over sorted particles (64 and 4096)
     double *bmax1 = nullptr, *bmax2 = nullptr;
     double *bsum1 = nullptr, *bsum2 = nullptr;  // [4][nb] : m, mx, my, mz
+    // gravity walk records (one per node, refreshed by each gravity call)
+    GravNode *gnode = nullptr;
+    int gnode_cap = 0;
     // root geometry (device): center[3], edge
     double *root = nullptr;
     int node_cap = 0;
@@ -156,6 +168,7 @@ struct sphgpu_ctx {
     long long *dstats = nullptr;
     int *dflag = nullptr;        // [0] list overflow, [1] node overflow
     int *hflag = nullptr;        // pinned host copy of dflag
+    int *hflag_dev = nullptr;    // its device address (written by k_flist_order)
     hipEvent_t flag_ready = nullptr;  // recorded after that copy
 
     sphgpu::Tree tree;
@@ -176,7 +189,8 @@ struct sphgpu_ctx {
     bool in_step = false;
 
     // timing
-    bool timing = false;
+    int timing = 0;  // 0 off, 1 every kernel group, 2 pair kernels only
+    bool timing_open = false;  // time_begin recorded an event for this group
     std::vector<sphgpu::Timer> pending;
     std::map<std::string, std::pair<double, long long>> ktimes;
     std::vector<hipEvent_t> event_pool;
@@ -225,7 +239,9 @@ int lists_grow(sphgpu_ctx *c, Lists &l);
 void lists_free(Lists &l);
 // is_ij = 0: radius hs[i] (r2 < hs^2);  is_ij = 1: sqrt(r2) < max(h_i, h_j), r2 != 0
 // defer: launch only; the overflow flag is read later (pre_resolve)
-int search(sphgpu_ctx *c, Lists &l, int is_ij, const double *radius, bool defer = false);
+// flag_cleared: the overflow flag was reset by an earlier launch of the step
+int search(sphgpu_ctx *c, Lists &l, int is_ij, const double *radius, bool defer = false,
+           bool flag_cleared = false);
 // settles a pending force list (flist == 1) from the device flag: 0 = use it
 int flist_check(sphgpu_ctx *c);
 

@@ -8,12 +8,20 @@
 // terms for every member j != i, an unopened node its monopole.  phi is
 // assigned, acc is added to (the walk runs after FluidForce).
 //
-// One thread per particle walks the pre-ordered node array stacklessly
-// (descend = index + 1, prune = skip).  Threads of a wave hold neighbouring
-// particles in key order, so their walks are largely coherent.  The opening
-// decision is taken in FP64 exactly as the reference takes it; the pair and
-// monopole terms run in the pair precision T with compensated sums.
+// A wavefront owns 64 consecutive particles in key order (a compact region)
+// and walks the pre-ordered node array once for all of them: at each node
+// every lane takes the reference's opening decision for its own particle in
+// FP64; lanes that accept the node add its monopole and sit out its subtree,
+// and the wave descends while any lane opens.  An opened leaf's members are
+// loaded once per wave into LDS and every opening lane sums its pair terms
+// from there.  Each particle therefore interacts with exactly the nodes and
+// leaf members of its own reference walk, in the same pre-order, while node
+// records are wave-uniform loads and member records are loaded once per wave
+// instead of once per lane.  The pair and monopole terms run in the pair
+// precision T with compensated sums.
 #include <hip/hip_runtime.h>
+
+#include <climits>
 
 #include "ctx.hpp"
 
@@ -21,33 +29,59 @@ namespace sphgpu {
 
 namespace {
 
-template <typename T> SPH_DEV T hk_f(T r, T h)
+// order LDS accesses between the lanes of one wavefront
+SPH_DEV void wave_lds_fence()
 {
-    const T e = h * T(0.5);
-    const T u = r / e;
-    if (u < T(1.0)) return (T(-0.5) * u * u * (T(1.0 / 3.0) - T(3.0 / 20) * u * u + u * u * u / T(20)) + T(1.4)) / e;
-    else if (u < T(2.0))
-        return T(-1.0) / (T(15) * r) + (-u * u * (T(4.0 / 3.0) - u + T(0.3) * u * u - u * u * u / T(30)) + T(1.6)) / e;
-    else return T(1) / r;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
 }
 
-template <typename T> SPH_DEV T hk_g(T r, T h)
+// Hernquist & Katz (1989) softened potential and force factors
+// (src/bhtree.cpp:319-345), epsilon = h/2: f(r, h) for the potential, g(r, h)
+// for the force (acceleration = -r_ij G m g).  FP64: the reference's
+// expressions.
+SPH_DEV double hk_f(double r, double h)
 {
-    const T e = h * T(0.5);
-    const T u = r / e;
-    if (u < T(1.0)) return (T(4.0 / 3.0) - T(1.2) * u * u + T(0.5) * u * u * u) / (e * e * e);
-    else if (u < T(2.0))
-        return (T(-1.0 / 15) + T(8.0 / 3) * u * u * u - T(3) * u * u * u * u + T(1.2) * u * u * u * u * u -
-                u * u * u * u * u * u / T(6.0)) / (r * r * r);
-    else return T(1) / (r * r * r);
+    const double e = h * 0.5;
+    const double u = r / e;
+    if (u < 1.0) return (-0.5 * u * u * (1.0 / 3.0 - 3.0 / 20 * u * u + u * u * u / 20) + 1.4) / e;
+    else if (u < 2.0) return -1.0 / (15 * r) + (-u * u * (4.0 / 3.0 - u + 0.3 * u * u - u * u * u / 30) + 1.6) / e;
+    else return 1 / r;
+}
+
+SPH_DEV double hk_g(double r, double h)
+{
+    const double e = h * 0.5;
+    const double u = r / e;
+    if (u < 1.0) return (4.0 / 3.0 - 1.2 * u * u + 0.5 * u * u * u) / (e * e * e);
+    else if (u < 2.0)
+        return (-1.0 / 15 + 8.0 / 3 * u * u * u - 3 * u * u * u * u + 1.2 * u * u * u * u * u - u * u * u * u * u * u / 6.0) /
+               (r * r * r);
+    else return 1 / (r * r * r);
+}
+
+// FP32: the same polynomials with the reciprocals 1/e and 1/r formed once per
+// pair side and every branch evaluated (the lanes of a wave take different
+// branches for the same neighbour), selected by u
+SPH_DEV void hk_fg(float r, float inv_r, float inv_e, float &f, float &g)
+{
+    const float u = r * inv_e;
+    const float u2 = u * u, u3 = u2 * u;
+    const float inv_r3 = inv_r * inv_r * inv_r;
+    const float inv_e3 = inv_e * inv_e * inv_e;
+    const float fa = (-0.5f * u2 * (1.0f / 3.0f - (3.0f / 20.0f) * u2 + u3 * (1.0f / 20.0f)) + 1.4f) * inv_e;
+    const float fb = -(1.0f / 15.0f) * inv_r + (-u2 * (4.0f / 3.0f - u + 0.3f * u2 - u3 * (1.0f / 30.0f)) + 1.6f) * inv_e;
+    const float ga = (4.0f / 3.0f - 1.2f * u2 + 0.5f * u3) * inv_e3;
+    const float gb = (-1.0f / 15.0f + u3 * (8.0f / 3.0f - 3.0f * u + 1.2f * u2 - u3 * (1.0f / 6.0f))) * inv_r3;
+    f = u < 1.0f ? fa : (u < 2.0f ? fb : inv_r);
+    g = u < 1.0f ? ga : (u < 2.0f ? gb : inv_r3);
 }
 
 template <typename T> struct GravArgs {
-    int n, n_nodes, node_cap;
+    int n, n_nodes;
     const double *x64[3];
     RecView<T> rv;
-    const int *nstart, *nend, *nskip, *nlevel;
-    const double *com, *edge, *mass;
+    const GravNode *__restrict__ node;
     double *acc[3];
     double *phi;
     double theta2, G;
@@ -55,10 +89,47 @@ template <typename T> struct GravArgs {
     long long *stats;
 };
 
+// one record per node, pre-order: centre of mass, edge^2 (the reference's
+// e * e), mass, member range, subtree end and signed level
+__global__ void k_grav_nodes(int n_nodes, int cap, const double *__restrict__ com, const double *__restrict__ edge,
+                             const double *__restrict__ mass, const int *__restrict__ start,
+                             const int *__restrict__ end, const int *__restrict__ skip, const int *__restrict__ level,
+                             GravNode *__restrict__ out)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n_nodes) return;
+    GravNode g;
+    g.cx = com[k];
+    g.cy = com[(size_t)cap + k];
+    g.cz = com[2 * (size_t)cap + k];
+    g.l2 = edge[k] * edge[k];
+    g.mass = mass[k];
+    g.pad = 0.0;
+    g.start = start[k];
+    g.end = end[k];
+    g.skip = skip[k];
+    g.level = level[k];
+    out[k] = g;
+}
+
+template <typename T> struct GravStage;
+template <> struct GravStage<float> {
+    float4 p[kWave];   // x, y, z, 1/e_j
+    float gm[kWave];   // G m_j
+};
+template <> struct GravStage<double> {
+    double4 p[kWave];  // x, y, z, h_j
+    double gm[kWave];
+};
+
 template <int D, typename T>
 __global__ __launch_bounds__(256) void k_gravity(GravArgs<T> a)
 {
+    __shared__ GravStage<T> stage[4];
+    GravStage<T> &st = stage[(threadIdx.x >> 6) & 3];
+    const int lane = threadIdx.x & (kWave - 1);
     const int i0 = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if ((i0 & ~(kWave - 1)) >= a.n) return;  // whole wave beyond the particles
     const bool valid = i0 < a.n;
     const int i = valid ? i0 : a.n - 1;
     typedef typename V4T<T>::type V4;
@@ -70,56 +141,102 @@ __global__ __launch_bounds__(256) void k_gravity(GravArgs<T> a)
     const V4 pi_ = RP[i];
     const T xi[3] = {pi_.x, pi_.y, pi_.z};
     const T hi = pi_.w;
+    const T inv_ei = T(2) / hi;
     const T G = T(a.G);
     Acc<T> phi, ac[3];
     long long inter = 0;
-    int node = valid ? 0 : a.n_nodes;
+    // the lane takes part in the walk from node `resume` on; a lane that
+    // accepted a node's monopole resumes at the node's subtree end
+    int resume = valid ? 0 : INT_MAX;
+    int node = 0;
     while (node < a.n_nodes) {
-        const double e = a.edge[node];
-        const double l2 = e * e;
-        double d[3];
+        const GravNode g = a.node[node];
+        const bool act = node >= resume;
+        double d[3] = {0, 0, 0};
+        const double cm[3] = {g.cx, g.cy, g.cz};
         double d2 = 0.0;
 #pragma unroll
         for (int k = 0; k < D; ++k) {
-            d[k] = rij_axis<double>(ri[k], a.com[(size_t)k * a.node_cap + node], a.box, k);
+            d[k] = rij_axis<double>(ri[k], cm[k], a.box, k);
             d2 = (k == 0) ? d[k] * d[k] : d2 + d[k] * d[k];
         }
-        const bool open = (l2 > a.theta2 * d2) || (d2 < l2);
-        if (open) {
-            if (a.nlevel[node] < 0) {
-                const int s = a.nstart[node], en = a.nend[node];
-                for (int j = s; j < en; ++j) {
-                    if (j == i) continue;
-                    const V4 pj = RP[j];
-                    const T xj[3] = {pj.x, pj.y, pj.z};
-                    T rij[3];
-                    T r2 = T(0);
-#pragma unroll
-                    for (int k = 0; k < D; ++k) {
-                        rij[k] = rij_axis<T>(xi[k], xj[k], a.box, k);
-                        r2 = (k == 0) ? rij[k] * rij[k] : r2 + rij[k] * rij[k];
-                    }
-                    const T r = tsqrt(r2);
-                    const T hj = pj.w;
-                    const T gm = G * RV[j].w;
-                    phi.add(-(gm * (hk_f<T>(r, hi) + hk_f<T>(r, hj)) * T(0.5)));
-                    const T s2 = gm * (hk_g<T>(r, hi) + hk_g<T>(r, hj)) * T(0.5);
-#pragma unroll
-                    for (int k = 0; k < D; ++k) ac[k].add(-(rij[k] * s2));
-                    ++inter;
-                }
-            }
-            ++node;
-        } else {
+        // BHNode::calc_force opening test (src/bhtree.cpp:350-358)
+        const bool open = act && ((g.l2 > a.theta2 * d2) || (d2 < g.l2));
+        if (act && !open) {
+            // monopole (src/bhtree.cpp:387-392)
             const T r_inv = T(1.0) / tsqrt(T(d2));
-            const T gm = G * T(a.mass[node]);
+            const T gm = G * T(g.mass);
             phi.add(-(gm * r_inv));
             const T s = gm * (r_inv * r_inv * r_inv);
 #pragma unroll
             for (int k = 0; k < D; ++k) ac[k].add(-(T(d[k]) * s));
             ++inter;
-            node = a.nskip[node];
+            resume = g.skip;
         }
+        if (__ballot(open) == 0ull) {
+            node = g.skip;
+            continue;
+        }
+        if (g.level < 0) {
+            // opened leaf: members staged 64 at a time, pair terms with every
+            // member j != i (src/bhtree.cpp:362-378)
+            for (int s0 = g.start; s0 < g.end; s0 += kWave) {
+                const int cnt = min(kWave, g.end - s0);
+                if (lane < cnt) {
+                    const V4 pj = RP[s0 + lane];
+                    const T mj = RV[s0 + lane].w;
+                    if constexpr (sizeof(T) == 4) st.p[lane] = make_float4(pj.x, pj.y, pj.z, 2.0f / pj.w);
+                    else st.p[lane] = pj;
+                    st.gm[lane] = G * mj;
+                }
+                wave_lds_fence();
+                if (open) {
+                    for (int t = 0; t < cnt; ++t) {
+                        if (s0 + t == i) continue;
+                        const typename V4T<T>::type q = st.p[t];
+                        const T xj[3] = {q.x, q.y, q.z};
+                        T rij[3] = {0, 0, 0};
+                        T r2 = T(0);
+#pragma unroll
+                        for (int k = 0; k < D; ++k) {
+                            rij[k] = rij_axis<T>(xi[k], xj[k], a.box, k);
+                            r2 = (k == 0) ? rij[k] * rij[k] : r2 + rij[k] * rij[k];
+                        }
+                        const T r = tsqrt(r2);
+                        const T gm = st.gm[t];
+                        T fsum, gsum;
+                        if constexpr (sizeof(T) == 4) {
+                            const float inv_r = tdiv(1.0f, r);
+                            // beyond both softening lengths (u >= 2 on both
+                            // sides, the usual case in an opened leaf) the
+                            // terms are 1/r and 1/r^3; the polynomials run
+                            // only when some lane of the wave has a near pair
+                            const bool near = r * inv_ei < 2.0f || r * q.w < 2.0f;
+                            if (__ballot(near)) {
+                                float fi, gi, fj, gj;
+                                hk_fg(r, inv_r, inv_ei, fi, gi);
+                                hk_fg(r, inv_r, q.w, fj, gj);
+                                fsum = fi + fj;
+                                gsum = gi + gj;
+                            } else {
+                                fsum = 2.0f * inv_r;
+                                gsum = 2.0f * (inv_r * inv_r * inv_r);
+                            }
+                        } else {
+                            fsum = hk_f(r, hi) + hk_f(r, q.w);
+                            gsum = hk_g(r, hi) + hk_g(r, q.w);
+                        }
+                        phi.add(-(gm * fsum * T(0.5)));
+                        const T s2 = gm * gsum * T(0.5);
+#pragma unroll
+                        for (int k = 0; k < D; ++k) ac[k].add(-(rij[k] * s2));
+                        ++inter;
+                    }
+                }
+                wave_lds_fence();
+            }
+        }
+        ++node;
     }
     if (valid) {
         a.phi[i] = (double)phi.get();
@@ -127,7 +244,7 @@ __global__ __launch_bounds__(256) void k_gravity(GravArgs<T> a)
         for (int k = 0; k < D; ++k) a.acc[k][i] += (double)ac[k].get();
     }
     for (int off = kWave / 2; off > 0; off >>= 1) inter += __shfl_xor(inter, off, kWave);
-    if ((threadIdx.x & (kWave - 1)) == 0) atomicAdd(stat_ptr(a.stats, 10), (unsigned long long)inter);
+    if (lane == 0) atomicAdd(stat_ptr(a.stats, 10), (unsigned long long)inter);
 }
 
 template <typename T> void launch_gravity(sphgpu_ctx *c)
@@ -135,11 +252,9 @@ template <typename T> void launch_gravity(sphgpu_ctx *c)
     GravArgs<T> a;
     a.n = c->n;
     a.n_nodes = c->tree.n_nodes;
-    a.node_cap = c->tree.node_cap;
     for (int k = 0; k < 3; ++k) a.x64[k] = c->pos[k < c->dim ? k : 0];
     a.rv = rec_view<T>(c);
-    a.nstart = c->tree.start; a.nend = c->tree.end; a.nskip = c->tree.skip; a.nlevel = c->tree.level;
-    a.com = c->tree.com; a.edge = c->tree.edge; a.mass = c->tree.mass;
+    a.node = c->tree.gnode;
     for (int k = 0; k < 3; ++k) a.acc[k] = c->acc[k];
     a.phi = c->sc[S_PHI];
     a.theta2 = c->prm.theta * c->prm.theta;
@@ -162,6 +277,20 @@ int gravity(sphgpu_ctx *c)
         int rc = refresh_mirrors(c);
         if (rc) return rc;
     }
+    Tree &t = c->tree;
+    if (t.gnode_cap < t.n_nodes) {
+        if (t.gnode) hipFree(t.gnode);
+        t.gnode = nullptr;
+        t.gnode_cap = 0;
+        const int cap = t.n_nodes + t.n_nodes / 4 + 64;
+        if (hipMalloc((void **)&t.gnode, sizeof(GravNode) * (size_t)cap) != hipSuccess)
+            return fail(c, SPHGPU_E_NOMEM, "gravity node records");
+        t.gnode_cap = cap;
+    }
+    time_begin(c, "gravity_nodes");
+    hipLaunchKernelGGL(k_grav_nodes, dim3(grid_for(t.n_nodes, 256)), dim3(256), 0, c->stream, t.n_nodes, t.node_cap,
+                       t.com, t.edge, t.mass, t.start, t.end, t.skip, t.level, t.gnode);
+    time_end(c);
     time_begin(c, "gravity");
     if (c->prm.pair_precision == SPHGPU_PAIR_FP64) launch_gravity<double>(c);
     else launch_gravity<float>(c);

@@ -65,6 +65,14 @@ template <typename T> struct PreArgs {
 // neighbour loops handle candidates in groups of this many
 constexpr int kGroup = 4;
 
+// entries q0 .. q0+kGroup-1 of particle i's list (entries at or past n
+// repeat entry n - 1); the kGroup loads are issued together
+SPH_DEV void list_group(const int *__restrict__ idx, int i, int q0, int n, int cap, int jg[kGroup])
+{
+#pragma unroll
+    for (int u = 0; u < kGroup; ++u) jg[u] = idx[list_slot(i, min(q0 + u, n - 1), cap)];
+}
+
 // single components of a particle record: the mass (velocity record, w) and
 // the specific internal energy (thermo record, w)
 template <typename V4> SPH_DEV auto rec_mass(const V4 *rv, int j) -> decltype(rv->w)
@@ -126,8 +134,6 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
     const V4 *__restrict__ RV = a.rv.r[R_VELM];
     const V4 *__restrict__ RT = a.rv.r[R_THERMO];
     const int nc = valid ? a.cnt[i] : 0;
-    // this particle's candidate column: entry q at pcol[q * kWave]
-    const int *const pcol = a.idx + list_slot(i, 0, a.cap);
     const V4 pi_ = RP[i], vi_ = RV[i], ti_ = RT[i];
     const T xi[3] = {pi_.x, pi_.y, pi_.z};
     const T vi[3] = {vi_.x, vi_.y, vi_.z};
@@ -156,8 +162,7 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
                 int jg[kGroup];
                 V4 pg[kGroup];
                 T wg[kGroup];
-#pragma unroll
-                for (int u = 0; u < kGroup; ++u) jg[u] = pcol[min(q0 + u, nc - 1) * kWave];
+                list_group(a.idx, i, q0, nc, a.cap, jg);
 #pragma unroll
                 for (int u = 0; u < kGroup; ++u) pg[u] = RP[jg[u]];
 #pragma unroll
@@ -224,7 +229,7 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
     // unusable for this step (the force searches instead).
     const bool fl = a.fl_idx != nullptr;
     const bool fl_rows = fl && nc > 256;
-    int *const fcol = fl_rows ? a.fl_idx + list_slot(i, 0, a.fl_cap) : nullptr;
+
     int qself = -1, nrow = 0;
     bool coincident = false;
     // FP32 r^2 at or below z2 may come from coincident positions (their
@@ -257,8 +262,7 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
     V4 pg[kGroup], vg[kGroup];
     T cg[kGroup], ug[kGroup];
     bool memb[kGroup];
-#pragma unroll
-    for (int u = 0; u < kGroup; ++u) jg[u] = pcol[min(q0 + u, nc - 1) * kWave];
+    list_group(a.idx, i, q0, nc, a.cap, jg);
 #pragma unroll
     for (int u = 0; u < kGroup; ++u) pg[u] = RP[jg[u]];
 #pragma unroll
@@ -296,7 +300,7 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
                 if (sizeof(T) == 4 && !((float)r2 > z2)) nz = r2_64(a.x64, xi64, j, a.box, D) != 0.0;
                 coincident = coincident || !nz;
                 if (fl_rows && nz) {
-                    if (nrow < a.fl_cap) fcol[nrow * kWave] = j;
+                    if (nrow < a.fl_cap) a.fl_idx[list_slot(i, nrow, a.fl_cap)] = j;
                     ++nrow;
                 }
             }
@@ -423,8 +427,7 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
     // taken in the density pass; what is left is writing the force-list rows
     // (row r of every lane = its r-th member, from the member bits, so a
     // store covers one 256-byte row) and finishing the sums.
-    int *const frow = (fl && !fl_rows) ? a.fl_idx + list_slot(i, 0, a.fl_cap) : nullptr;
-    bool rows_pending = frow != nullptr;
+    bool rows_pending = fl && !fl_rows;
 
     if (kGrad && grad_on) {
         // MUSCL gradients (summed in the density pass)
@@ -469,7 +472,7 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
             }
             const int qc = (wd << 6) + __ffsll((long long)m) - 1;
             m &= m - 1ull;
-            frow[r * kWave] = pcol[qc * kWave];
+            a.fl_idx[list_slot(i, r, a.fl_cap)] = a.idx[list_slot(i, qc, a.cap)];
         }
         rows_pending = false;
     }
@@ -538,6 +541,12 @@ struct FlistArgs {
     Box box;
     int *flag;
     long long *stats;
+    // k_flist_order also hands the list flags to the host (mapped pinned
+    // memory) and stores the h / v_sig minimum
+    const int *dflag;
+    int *hflag;
+    const unsigned long long *minbits;
+    double *dscal;
 };
 
 template <int D, bool FP64>
@@ -553,14 +562,12 @@ __global__ __launch_bounds__(256) void k_flist_scatter(FlistArgs a)
     for (int k = 0; k < D; ++k) xi64[k] = a.x64[k][i];
     const float xf[3] = {pi.x, pi.y, pi.z};
     const float mk = fp32_margin_k(a.root, D);
-    const int *const pcol = a.idx + list_slot(i, 0, a.cap);
     // members in groups of kGroup, each group's loads issued together
     for (int q0 = 0; q0 < na; q0 += kGroup) {
     int jg[kGroup];
     float4 pg[kGroup];
     double hg[kGroup];
-#pragma unroll
-    for (int u = 0; u < kGroup; ++u) jg[u] = pcol[min(q0 + u, na - 1) * kWave];
+    list_group(a.idx, i, q0, na, a.cap, jg);
 #pragma unroll
     for (int u = 0; u < kGroup; ++u) pg[u] = a.posf[jg[u]];
     // FP32 rounding is monotone: rounded values that differ order h_i and
@@ -602,6 +609,15 @@ constexpr int kRunSort = 16;
 
 __global__ __launch_bounds__(256) void k_flist_order(FlistArgs a)
 {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        // every writer of the flags (search, pre-interaction, scatter) and of
+        // the minima ran in earlier launches: their values are final here
+        unsigned long long v = a.minbits[0];
+        for (int k = 1; k < kStatCopies; ++k) v = a.minbits[k] < v ? a.minbits[k] : v;
+        a.dscal[2] = bitsd(v);
+        for (int k = 0; k < 3; ++k) __hip_atomic_store(&a.hflag[k], a.dflag[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        __threadfence_system();
+    }
     const int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     int nt = 0;
     if (i < a.n) {
@@ -673,8 +689,17 @@ __global__ __launch_bounds__(256) void k_initial_density(int n, int cap, const i
 // smoothing-length guess (pre_interaction.cpp:61-64, 186-189)
 template <int D>
 __global__ void k_hguess(int n, const double *__restrict__ mass, const double *__restrict__ dens, int nn,
-                         double kappa, double *__restrict__ hs)
+                         double kappa, double *__restrict__ hs, unsigned long long *minbits, int *flags)
 {
+    // with flags: the pre-interaction's resets (search overflow, force-list
+    // flag and the h / v_sig minima) ride on this launch
+    if (flags && blockIdx.x == 0) {
+        if (threadIdx.x < kStatCopies) minbits[threadIdx.x] = 0x7fefffffffffffffull;  // DBL_MAX
+        if (threadIdx.x == 0) {
+            flags[0] = 0;
+            flags[2] = 0;
+        }
+    }
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const double A = area_const<D>();
@@ -836,11 +861,10 @@ __global__ __launch_bounds__(256) void k_force_sph(ForceArgs<T> a)
     Acc<T> ac[3], de;
     const int nc = a.cnt[i];
     // the next pair index is loaded one pair ahead
-    const int *const pcol = a.idx + list_slot(i, 0, a.cap);
-    int j_next = nc > 0 ? pcol[0] : 0;
+    int j_next = nc > 0 ? a.idx[list_slot(i, 0, a.cap)] : 0;
     for (int q = 0; q < nc; ++q) {
         const int j = j_next;
-        if (q + 1 < nc) j_next = pcol[(q + 1) * kWave];
+        if (q + 1 < nc) j_next = a.idx[list_slot(i, q + 1, a.cap)];
         const V4 pj = RP[j], vj = RV[j], tj = RT[j], aj = RA[j];
         T d[3];
         const T r = tsqrt(rec_r2<D, T>(pj, xi, a.box, d));
@@ -1043,14 +1067,13 @@ SPH_DEV void force_gsph_slots(const ForceArgs<T> &a, int i, int nc, int nmax, co
     long long iters = 0, wave_max_sum = 0, calls = 0, fallbacks = 0, learned = 0;
     // the next slot's pair index is loaded one slot ahead, so its record
     // gathers do not wait for it
-    const int *const pcol = a.idx + list_slot(i, 0, a.cap);
-    int j_next = nc > 0 ? pcol[0] : 0;
+    int j_next = nc > 0 ? a.idx[list_slot(i, 0, a.cap)] : 0;
     for (int q = 0; q < nmax; ++q) {
         const bool act = q < nc;
         int it = 0;
         int j = 0;
         const int j_cur = j_next;
-        if (q + 1 < nc) j_next = pcol[(q + 1) * kWave];
+        if (q + 1 < nc) j_next = a.idx[list_slot(i, q + 1, a.cap)];
         T d[3] = {0, 0, 0}, e[3] = {0, 0, 0}, r = T(1), r_inv = T(1);
         T left[4] = {0, 1, 1, 1}, right[4] = {0, 1, 1, 1};
         T rhoj = T(1), hj = T(1), mj = T(0);
@@ -1373,6 +1396,10 @@ template <int D> void launch_flist(sphgpu_ctx *c)
     a.box = c->box;
     a.flag = c->dflag + 2;
     a.stats = c->dstats;
+    a.dflag = c->dflag;
+    a.hflag = c->hflag_dev;
+    a.minbits = c->dmin;
+    a.dscal = c->dscal;
     const dim3 g(grid_for(c->n, 256)), b(256);
     time_begin(c, "force_lists");
     if (c->prm.pair_precision == SPHGPU_PAIR_FP64) hipLaunchKernelGGL((k_flist_scatter<D, true>), g, b, 0, c->stream, a);
@@ -1428,17 +1455,18 @@ template <int D> void launch_initial_density(sphgpu_ctx *c, const double *const 
     }
 }
 
-template <int D> void launch_hguess(sphgpu_ctx *c, double kappa, double *out)
+template <int D> void launch_hguess(sphgpu_ctx *c, double kappa, double *out, bool resets)
 {
     hipLaunchKernelGGL((k_hguess<D>), dim3(grid_for(c->n, 256)), dim3(256), 0, c->stream, c->n,
-                       (const double *)c->sc[S_MASS], (const double *)c->sc[S_DENS], c->prm.neighbor_number, kappa, out);
+                       (const double *)c->sc[S_MASS], (const double *)c->sc[S_DENS], c->prm.neighbor_number, kappa, out,
+                       resets ? c->dmin : nullptr, resets ? c->dflag : nullptr);
 }
 
-static void hguess(sphgpu_ctx *c, double kappa, double *out)
+static void hguess(sphgpu_ctx *c, double kappa, double *out, bool resets = false)
 {
-    if (c->dim == 1) launch_hguess<1>(c, kappa, out);
-    else if (c->dim == 2) launch_hguess<2>(c, kappa, out);
-    else launch_hguess<3>(c, kappa, out);
+    if (c->dim == 1) launch_hguess<1>(c, kappa, out, resets);
+    else if (c->dim == 2) launch_hguess<2>(c, kappa, out, resets);
+    else launch_hguess<3>(c, kappa, out, resets);
 }
 
 // ---- force dispatch ----
@@ -1561,26 +1589,34 @@ static int pre_core(sphgpu_ctx *c, bool defer, bool guess_h)
         HIPCHK(c, hipStreamSynchronize(c->stream));
         c->pre_first = false;
     }
+    const bool fl = c->prm.iterative_sml != 0;
     if (guess_h) {
+        // the guess launch also clears the search's overflow flag, the
+        // force-list flag and the h / v_sig minima
         time_begin(c, "pre_hguess");
-        hguess(c, kappa, c->hs);
+        hguess(c, kappa, c->hs, true);
         time_end(c);
     }
-    if ((rc = search(c, c->lpre, 0, c->hs, defer))) return rc;
-    const bool fl = c->prm.iterative_sml != 0;
-    hipLaunchKernelGGL(k_minbits_init, dim3(1), dim3(kStatCopies), 0, c->stream, c->dmin, fl ? c->dflag + 2 : nullptr);
+    if ((rc = search(c, c->lpre, 0, c->hs, defer, guess_h))) return rc;
+    if (!guess_h)
+        hipLaunchKernelGGL(k_minbits_init, dim3(1), dim3(kStatCopies), 0, c->stream, c->dmin, fl ? c->dflag + 2 : nullptr);
     time_begin(c, "pre_interaction");
     if (c->prm.pair_precision == SPHGPU_PAIR_FP64) dispatch_pre<double>(c);
     else dispatch_pre<float>(c);
     time_end(c);
-    hipLaunchKernelGGL(k_store_hvs, dim3(1), dim3(1), 0, c->stream, (const unsigned long long *)c->dmin, c->dscal);
     HIPCHK(c, hipGetLastError());
     if (fl) {
+        // the list-ordering launch stores h / v_sig and hands the list flags
+        // to the host; flag_ready marks them readable
         if (c->dim == 1) launch_flist<1>(c);
         else if (c->dim == 2) launch_flist<2>(c);
         else launch_flist<3>(c);
         HIPCHK(c, hipGetLastError());
+        HIPCHK(c, hipEventRecord(c->flag_ready, c->stream));
         c->flist = 1;  // checked (one flag read) by the first fluid_force
+    } else {
+        hipLaunchKernelGGL(k_store_hvs, dim3(1), dim3(1), 0, c->stream, (const unsigned long long *)c->dmin, c->dscal);
+        HIPCHK(c, hipGetLastError());
     }
     c->have_pre = true;
     return tree_set_kernel(c);
@@ -1597,8 +1633,6 @@ int fluid_force(sphgpu_ctx *c)
         // overflow, or a force list that needs the is_ij search); the host
         // reads the same flags meanwhile.  After a skip, nothing was written
         // and the checked path below runs.
-        HIPCHK(c, hipMemcpyAsync(c->hflag, c->dflag, 3 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(c, hipEventRecord(c->flag_ready, c->stream));
         time_begin(c, "fluid_force");
         if (c->prm.pair_precision == SPHGPU_PAIR_FP64) dispatch_force<double>(c, c->dflag);
         else dispatch_force<float>(c, c->dflag);

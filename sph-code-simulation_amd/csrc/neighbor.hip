@@ -211,6 +211,7 @@ __global__ __launch_bounds__(256, 4) void k_search(SearchArgs a)
     // this particle's column of the list: entry q at col[64 q]
     int *const col = a.idx + list_slot(ii, 0, a.cap);
     int visits = 0, tests = 0, windows = 0, leaves = 0;
+    int reach = 0;
     const int p_end = min(group * kWave + kWave, a.n);
     int sub = group * kWave;
     // candidate buffer: lane q holds the q-th pending candidate index
@@ -455,8 +456,12 @@ __global__ __launch_bounds__(256, 4) void k_search(SearchArgs a)
                 const float e = fmaxf(abs_min_image_f(lc3[q] - xf[q], a.box.periodic, Lf[q]) - lh[q], 0.f);
                 dist2 = (q == 0) ? e * e : dist2 + e * e;
             }
-            if (__ballot(mine && dist2 <= rl * rl)) {
+            const bool own = mine && dist2 <= rl * rl;
+            if (__ballot(own)) {
                 ++leaves;
+                // members this lane's own sphere would have to test in a
+                // per-particle walk (the search's algorithmic work)
+                if (own) reach += rdl(lk.y, o) - rdl(lk.x, o);
                 int s = rdl(lk.x, o);
                 const int e = rdl(lk.y, o);
                 while (s < e) {
@@ -600,6 +605,8 @@ __global__ __launch_bounds__(256, 4) void k_search(SearchArgs a)
     int c = valid ? count : 0;
     for (int off = kWave / 2; off > 0; off >>= 1) c += __shfl_xor(c, off, kWave);
     for (int off = kWave / 2; off > 0; off >>= 1) nexact += __shfl_xor(nexact, off, kWave);
+    long long reach_w = reach;
+    for (int off = kWave / 2; off > 0; off >>= 1) reach_w += __shfl_xor(reach_w, off, kWave);
     if (lane == 0) {
         atomicAdd(stat_ptr(a.stats, 17), (unsigned long long)nexact);
         atomicAdd(stat_ptr(a.stats, IS_IJ ? 4 : 0), (unsigned long long)c);
@@ -607,6 +614,7 @@ __global__ __launch_bounds__(256, 4) void k_search(SearchArgs a)
         atomicAdd(stat_ptr(a.stats, 14), (unsigned long long)tests);
         atomicAdd(stat_ptr(a.stats, 15), (unsigned long long)windows);
         atomicAdd(stat_ptr(a.stats, 16), (unsigned long long)leaves);
+        atomicAdd(stat_ptr(a.stats, 20), (unsigned long long)reach_w);
     }
 }
 
@@ -660,7 +668,7 @@ int flist_check(sphgpu_ctx *c)
     return c->flist == 2 ? 0 : 1;
 }
 
-int search(sphgpu_ctx *c, Lists &l, int is_ij, const double *radius, bool defer)
+int search(sphgpu_ctx *c, Lists &l, int is_ij, const double *radius, bool defer, bool flag_cleared)
 {
     Tree &t = c->tree;
     if (!t.valid) return fail(c, SPHGPU_E_STATE, "neighbour search without a tree");
@@ -681,7 +689,8 @@ int search(sphgpu_ctx *c, Lists &l, int is_ij, const double *radius, bool defer)
         a.idx = l.idx; a.cnt = l.cnt; a.cap = l.cap;
         a.overflow = c->dflag;
         a.stats = c->dstats;
-        HIPCHK(c, hipMemsetAsync(c->dflag, 0, sizeof(int), c->stream));
+        if (!flag_cleared) HIPCHK(c, hipMemsetAsync(c->dflag, 0, sizeof(int), c->stream));
+        flag_cleared = false;  // a repeated search after growing the lists clears it here
         const int groups = (c->n + kWave - 1) / kWave;
         const dim3 grid(grid_for((long long)groups * kWave, 256)), block(256);
         time_begin(c, is_ij ? "search_force" : "search_pre");

@@ -222,9 +222,79 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_tile_sums(unsigned *__res
     if (threadIdx.x == 0) tile_sum[ntiles] = carry;
 }
 
-__global__ __launch_bounds__(kScanThreads) void k_scan_apply(const unsigned *in, int n, const unsigned *__restrict__ tile_off,
-                                                             unsigned *out)
+// one workgroup scans a short array in place or out of place, tile by tile:
+// each thread loads 8 consecutive values of a tile of 8192 (coalesced
+// 16-byte loads), the thread sums are scanned across the workgroup, a carry
+// runs from tile to tile, and each thread writes its 8 running values (a
+// tile is read before it is written, so in == out is allowed); total ->
+// *total_out (and *total_out2) when given
+constexpr int kSingleThreads = 1024;
+constexpr int kSingleItems = 8;
+constexpr int kSingleTile = kSingleThreads * kSingleItems;
+constexpr int kSingleMax = kSingleTile * 24;
+
+__global__ __launch_bounds__(kSingleThreads) void k_scan_single(const unsigned *in, unsigned *out, int n,
+                                                                unsigned *total_out, unsigned *total_out2)
 {
+    __shared__ unsigned wsum[kSingleThreads / kWave];
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+    unsigned carry = 0;
+    for (int t0 = 0; t0 < n; t0 += kSingleTile) {
+        const int b = t0 + threadIdx.x * kSingleItems;
+        unsigned v[kSingleItems];
+        if (b + kSingleItems <= n) {
+            const uint4 a0 = *reinterpret_cast<const uint4 *>(in + b);
+            const uint4 a1 = *reinterpret_cast<const uint4 *>(in + b + 4);
+            v[0] = a0.x; v[1] = a0.y; v[2] = a0.z; v[3] = a0.w;
+            v[4] = a1.x; v[5] = a1.y; v[6] = a1.z; v[7] = a1.w;
+        } else {
+#pragma unroll
+            for (int k = 0; k < kSingleItems; ++k) v[k] = b + k < n ? in[b + k] : 0u;
+        }
+        unsigned s = 0;
+#pragma unroll
+        for (int k = 0; k < kSingleItems; ++k) s += v[k];
+        unsigned x = s;
+        for (int off = 1; off < kWave; off <<= 1) {
+            const unsigned y = __shfl_up(x, off, kWave);
+            if (lane >= off) x += y;
+        }
+        if (lane == kWave - 1) wsum[w] = x;
+        __syncthreads();
+        unsigned run = carry + x - s, total = 0;
+        for (int k = 0; k < kSingleThreads / kWave; ++k) {
+            if (k < w) run += wsum[k];
+            total += wsum[k];
+        }
+        __syncthreads();
+        carry += total;
+        unsigned o[kSingleItems];
+#pragma unroll
+        for (int k = 0; k < kSingleItems; ++k) {
+            o[k] = run;
+            run += v[k];
+        }
+        if (b + kSingleItems <= n) {
+            *reinterpret_cast<uint4 *>(out + b) = make_uint4(o[0], o[1], o[2], o[3]);
+            *reinterpret_cast<uint4 *>(out + b + 4) = make_uint4(o[4], o[5], o[6], o[7]);
+        } else {
+#pragma unroll
+            for (int k = 0; k < kSingleItems; ++k)
+                if (b + k < n) out[b + k] = o[k];
+        }
+    }
+    if (threadIdx.x == 0 && total_out) *total_out = carry;
+    if (threadIdx.x == 0 && total_out2) *total_out2 = carry;
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_apply(const unsigned *in, int n, const unsigned *__restrict__ tile_off,
+                                                             unsigned *out, int ntiles, unsigned *total_out,
+                                                             unsigned *total_out2)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (total_out) *total_out = tile_off[ntiles];
+        if (total_out2) *total_out2 = tile_off[ntiles];
+    }
     __shared__ unsigned tile[kScanTile];
     __shared__ unsigned wsum[kScanThreads / kWave];
     const long long base = (long long)blockIdx.x * kScanTile;
@@ -770,14 +840,22 @@ __global__ void k_node_mass(const int *__restrict__ nstart, const int *__restric
 } // namespace
 
 // exclusive scan of in[0..n) into out[0..n), with out[n] = total when
-// write_total (out then has n + 1 entries); tmp holds ceil(n / tile) + 1 values
-static void scan_counts(hipStream_t st, const unsigned *in, unsigned *out, int n, unsigned *tmp, bool write_total)
+// write_total (out then has n + 1 entries) and the total also in *total_copy
+// when given; tmp holds ceil(n / tile) + 1 values.  Short arrays take one
+// workgroup (one launch), longer ones the three tile passes.
+static void scan_counts(hipStream_t st, const unsigned *in, unsigned *out, int n, unsigned *tmp, bool write_total,
+                        unsigned *total_copy = nullptr)
 {
+    if (n <= kSingleMax) {
+        hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(kSingleThreads), 0, st, in, out, n,
+                           write_total ? out + n : nullptr, total_copy);
+        return;
+    }
     const int nt = (n + kScanTile - 1) / kScanTile;
     hipLaunchKernelGGL(k_scan_tiles, dim3(nt), dim3(kScanThreads), 0, st, in, n, tmp);
     hipLaunchKernelGGL(k_scan_tile_sums, dim3(1), dim3(kScanThreads), 0, st, tmp, nt);
-    hipLaunchKernelGGL(k_scan_apply, dim3(nt), dim3(kScanThreads), 0, st, in, n, (const unsigned *)tmp, out);
-    if (write_total) hipMemcpyAsync(out + n, tmp + nt, sizeof(unsigned), hipMemcpyDeviceToDevice, st);
+    hipLaunchKernelGGL(k_scan_apply, dim3(nt), dim3(kScanThreads), 0, st, in, n, (const unsigned *)tmp, out, nt,
+                       write_total ? out + n : nullptr, total_copy);
 }
 
 // ------------------------------------------------------------------------------
@@ -828,7 +906,7 @@ void tree_free(sphgpu_ctx *c)
     Tree &t = c->tree;
     void *ps[] = {t.key, t.key_alt, t.perm, t.perm_alt, t.kdig, t.lam, t.sync_words, t.leaf_end, t.ncount, t.nbase, t.start,
                   t.end, t.skip, t.level, t.center, t.edge, t.ks, t.mass, t.com, t.bmax1, t.bmax2, t.bsum1,
-                  t.bsum2, t.root, t.hist, t.nbox, t.nlink, t.scan_tmp, t.nhalf, t.nchild, t.child, t.ccount};
+                  t.bsum2, t.root, t.hist, t.nbox, t.nlink, t.scan_tmp, t.nhalf, t.nchild, t.child, t.ccount, t.gnode};
     for (void *p : ps)
         if (p) hipFree(p);
     if (t.host_words) hipHostFree(t.host_words);
@@ -942,9 +1020,11 @@ int tree_build(sphgpu_ctx *c)
     hipLaunchKernelGGL(k_lambda, dim3(grid_for(n, kLambdaThreads)), dim3(kLambdaThreads), lam_lds,
                        c->stream, (const unsigned char *)t.kdig, sh,
                        t.lam, t.leaf_end, t.ncount);
-    scan_counts(c->stream, (const unsigned *)t.ncount, (unsigned *)t.nbase, n, t.scan_tmp, true);
+    // the node count goes to nbase[n] and to sync_words[0] (read by the fill
+    // on the device and copied to the host)
+    scan_counts(c->stream, (const unsigned *)t.ncount, (unsigned *)t.nbase, n, t.scan_tmp, true,
+                (unsigned *)t.sync_words);
     HIPCHK(c, hipGetLastError());
-    HIPCHK(c, hipMemcpyAsync(t.sync_words, t.nbase + n, sizeof(int), hipMemcpyDeviceToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(t.host_words, t.sync_words, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipEventRecord(t.words_ready, c->stream));
     hipLaunchKernelGGL(k_fill, dim3(grid_for(n, 128)), dim3(128), 0, c->stream, (const unsigned long long *)t.key,

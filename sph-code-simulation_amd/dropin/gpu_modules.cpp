@@ -13,6 +13,7 @@
 //   TimeStep        reads acc, sml, h_per_v_sig; writes dt (src/timestep.cpp:18-38)
 #include "gpu_modules.hpp"
 
+#include <cstring>
 #include <map>
 #include <mutex>
 #include <string>
@@ -89,6 +90,7 @@ std::shared_ptr<Device> Device::attach(const std::shared_ptr<Simulation> &sim, c
     const int rc = sphgpu_create(&q, d->m_num, &d->m_ctx);
     if (rc != 0) THROW_ERROR("sphgpu_create failed (", rc, ")");
     d->m_block.assign((size_t)SPHGPU_NFIELDS * d->m_num, 0.0);
+    d->m_stage.assign((size_t)SPHGPU_NFIELDS * d->m_num, 0.0);
     g_devices[sim.get()] = d;
     return d;
 }
@@ -103,57 +105,104 @@ void Device::check(int rc, const char *what) const
     if (rc != 0) THROW_ERROR(what, " failed (", rc, "): ", sphgpu_last_error(m_ctx));
 }
 
-void Device::upload(Simulation &sim, uint64_t mask)
+uint64_t Device::upload(Simulation &sim, uint64_t mask)
 {
     auto &ps = sim.get_particles();
-    const size_t n = m_num;
-    double *b = m_block.data();
-    for (size_t i = 0; i < n; ++i) {
+    const long long n = m_num;
+    double *b = m_stage.data();
+    // only the rows asked for are packed, in one pass over the particles
+    const bool vec_pos = (mask & kVec(SPHGPU_F_POS)) != 0, vec_vel = (mask & kVec(SPHGPU_F_VEL)) != 0,
+               vec_velp = (mask & kVec(SPHGPU_F_VEL_P)) != 0, vec_acc = (mask & kVec(SPHGPU_F_ACC)) != 0;
+#pragma omp parallel for schedule(static)
+    for (long long i = 0; i < n; ++i) {
         const SPHParticle &p = ps[i];
         for (int k = 0; k < DIM; ++k) {
-            b[(SPHGPU_F_POS + k) * n + i] = p.pos[k];
-            b[(SPHGPU_F_VEL + k) * n + i] = p.vel[k];
-            b[(SPHGPU_F_VEL_P + k) * n + i] = p.vel_p[k];
-            b[(SPHGPU_F_ACC + k) * n + i] = p.acc[k];
+            if (vec_pos) b[(SPHGPU_F_POS + k) * n + i] = p.pos[k];
+            if (vec_vel) b[(SPHGPU_F_VEL + k) * n + i] = p.vel[k];
+            if (vec_velp) b[(SPHGPU_F_VEL_P + k) * n + i] = p.vel_p[k];
+            if (vec_acc) b[(SPHGPU_F_ACC + k) * n + i] = p.acc[k];
         }
-        b[SPHGPU_F_MASS * n + i] = p.mass;
-        b[SPHGPU_F_DENS * n + i] = p.dens;
-        b[SPHGPU_F_PRES * n + i] = p.pres;
-        b[SPHGPU_F_ENE * n + i] = p.ene;
-        b[SPHGPU_F_ENE_P * n + i] = p.ene_p;
-        b[SPHGPU_F_DENE * n + i] = p.dene;
-        b[SPHGPU_F_SML * n + i] = p.sml;
-        b[SPHGPU_F_SOUND * n + i] = p.sound;
-        b[SPHGPU_F_BALSARA * n + i] = p.balsara;
-        b[SPHGPU_F_ALPHA * n + i] = p.alpha;
-        b[SPHGPU_F_GRADH * n + i] = p.gradh;
-        b[SPHGPU_F_PHI * n + i] = p.phi;
-        b[SPHGPU_F_NEIGHBOR * n + i] = p.neighbor;
+#define PK(F, m) if (mask & M(F)) b[(F) * n + i] = p.m
+        PK(SPHGPU_F_MASS, mass); PK(SPHGPU_F_DENS, dens); PK(SPHGPU_F_PRES, pres); PK(SPHGPU_F_ENE, ene);
+        PK(SPHGPU_F_ENE_P, ene_p); PK(SPHGPU_F_DENE, dene); PK(SPHGPU_F_SML, sml); PK(SPHGPU_F_SOUND, sound);
+        PK(SPHGPU_F_BALSARA, balsara); PK(SPHGPU_F_ALPHA, alpha); PK(SPHGPU_F_GRADH, gradh); PK(SPHGPU_F_PHI, phi);
+        PK(SPHGPU_F_NEIGHBOR, neighbor);
+#undef PK
     }
     if (m_gsph && (mask & kGrad)) {
         const auto &gd = sim.get_vector_array("grad_density");
         const auto &gp = sim.get_vector_array("grad_pressure");
         for (int a = 0; a < DIM; ++a) {
             const auto &gv = sim.get_vector_array("grad_velocity_" + std::to_string(a));
-            for (size_t i = 0; i < n; ++i)
+#pragma omp parallel for schedule(static)
+            for (long long i = 0; i < n; ++i)
                 for (int k = 0; k < DIM; ++k) b[(SPHGPU_F_GRAD_V + 3 * a + k) * n + i] = gv[i][k];
         }
-        for (size_t i = 0; i < n; ++i)
+#pragma omp parallel for schedule(static)
+        for (long long i = 0; i < n; ++i)
             for (int k = 0; k < DIM; ++k) {
                 b[(SPHGPU_F_GRAD_D + k) * n + i] = gd[i][k];
                 b[(SPHGPU_F_GRAD_P + k) * n + i] = gp[i][k];
             }
+    } else {
+        mask &= ~kGrad;
     }
-    check(sphgpu_upload(m_ctx, b, mask), "sphgpu_upload");
+    // rows whose host values equal the device's stay where they are (axes
+    // beyond DIM are never packed and stay as they are)
+    for (int k = DIM; k < 3; ++k)
+        mask &= ~(M(SPHGPU_F_POS + k) | M(SPHGPU_F_VEL + k) | M(SPHGPU_F_VEL_P + k) | M(SPHGPU_F_ACC + k) |
+                  M(SPHGPU_F_GRAD_D + k) | M(SPHGPU_F_GRAD_P + k));
+    uint64_t send = 0;
+    for (int f = 0; f < SPHGPU_NFIELDS; ++f) {
+        if (!(mask & M(f))) continue;
+        double *dst = m_block.data() + (size_t)f * n;
+        const double *src = b + (size_t)f * n;
+        if ((m_synced & M(f)) && std::memcmp(dst, src, sizeof(double) * (size_t)n) == 0) {
+            ++rows_skipped;
+            continue;
+        }
+        std::memcpy(dst, src, sizeof(double) * (size_t)n);
+        send |= M(f);
+    }
+    if (send) {
+        check(sphgpu_upload(m_ctx, m_block.data(), send), "sphgpu_upload");
+        m_synced |= send;
+        rows_up += (uint64_t)__builtin_popcountll(send);
+        // new positions need a new tree; new smoothing lengths new kernel sizes
+        if (send & kVec(SPHGPU_F_POS)) m_tree = false;
+        if (send & M(SPHGPU_F_SML)) m_ks = false;
+    }
+    return send;
+}
+
+void Device::ensure_tree()
+{
+    if (m_tree) return;
+    check(sphgpu_build_tree(m_ctx), "sphgpu_build_tree");
+    ++tree_builds;
+    m_tree = true;
+    m_ks = false;
+}
+
+void Device::ensure_kernel_size()
+{
+    ensure_tree();
+    if (m_ks) return;
+    check(sphgpu_set_kernel(m_ctx), "sphgpu_set_kernel");
+    m_ks = true;
 }
 
 void Device::download(Simulation &sim, uint64_t mask)
 {
+    if (!m_gsph) mask &= ~kGrad;
     double *b = m_block.data();
     check(sphgpu_download(m_ctx, b, mask), "sphgpu_download");
+    m_synced |= mask;
+    rows_down += (uint64_t)__builtin_popcountll(mask);
     auto &ps = sim.get_particles();
-    const size_t n = m_num;
-    for (size_t i = 0; i < n; ++i) {
+    const long long n = m_num;
+#pragma omp parallel for schedule(static)
+    for (long long i = 0; i < n; ++i) {
         SPHParticle &p = ps[i];
         for (int k = 0; k < DIM; ++k) {
             if (mask & M(SPHGPU_F_ACC + k)) p.acc[k] = b[(SPHGPU_F_ACC + k) * n + i];
@@ -169,10 +218,12 @@ void Device::download(Simulation &sim, uint64_t mask)
         auto &gp = sim.get_vector_array("grad_pressure");
         for (int a = 0; a < DIM; ++a) {
             auto &gv = sim.get_vector_array("grad_velocity_" + std::to_string(a));
-            for (size_t i = 0; i < n; ++i)
+#pragma omp parallel for schedule(static)
+            for (long long i = 0; i < n; ++i)
                 for (int k = 0; k < DIM; ++k) gv[i][k] = b[(SPHGPU_F_GRAD_V + 3 * a + k) * n + i];
         }
-        for (size_t i = 0; i < n; ++i)
+#pragma omp parallel for schedule(static)
+        for (long long i = 0; i < n; ++i)
             for (int k = 0; k < DIM; ++k) {
                 gd[i][k] = b[(SPHGPU_F_GRAD_D + k) * n + i];
                 gp[i][k] = b[(SPHGPU_F_GRAD_P + k) * n + i];
@@ -194,11 +245,13 @@ void PreInteraction::calculation(std::shared_ptr<Simulation> sim)
                         M(SPHGPU_F_ENE) | M(SPHGPU_F_SOUND) | M(SPHGPU_F_ALPHA) | M(SPHGPU_F_BALSARA);
     m_dev->upload(*sim, in);
     m_dev->check(sphgpu_set_dt(m_dev->ctx(), sim->get_dt()), "sphgpu_set_dt");
-    m_dev->check(sphgpu_build_tree(m_dev->ctx()), "sphgpu_build_tree");
+    m_dev->ensure_tree();
+    // the pre-interaction ends with set_kernel over its new smoothing lengths
     m_dev->check(sphgpu_pre_interaction(m_dev->ctx()), "sphgpu_pre_interaction");
     const uint64_t out = M(SPHGPU_F_SML) | M(SPHGPU_F_DENS) | M(SPHGPU_F_PRES) | M(SPHGPU_F_GRADH) |
                          M(SPHGPU_F_NEIGHBOR) | M(SPHGPU_F_BALSARA) | M(SPHGPU_F_ALPHA) | kGrad;
     m_dev->download(*sim, out);
+    m_dev->mark_kernel_size();
     double hvs = 0.0;
     m_dev->check(sphgpu_get_h_per_v_sig(m_dev->ctx(), &hvs), "sphgpu_get_h_per_v_sig");
     sim->set_h_per_v_sig(hvs);
@@ -222,8 +275,7 @@ void FluidForce::calculation(std::shared_ptr<Simulation> sim)
                         M(SPHGPU_F_GRADH) | M(SPHGPU_F_BALSARA) | M(SPHGPU_F_ALPHA) | kGrad;
     m_dev->upload(*sim, in);
     m_dev->check(sphgpu_set_dt(m_dev->ctx(), sim->get_dt()), "sphgpu_set_dt");
-    m_dev->check(sphgpu_build_tree(m_dev->ctx()), "sphgpu_build_tree");
-    m_dev->check(sphgpu_set_kernel(m_dev->ctx()), "sphgpu_set_kernel");
+    m_dev->ensure_kernel_size();
     m_dev->check(sphgpu_fluid_force(m_dev->ctx()), "sphgpu_fluid_force");
     m_dev->download(*sim, kVec(SPHGPU_F_ACC) | M(SPHGPU_F_DENE));
 }
@@ -241,7 +293,7 @@ void GravityForce::calculation(std::shared_ptr<Simulation> sim)
     if (!m_dev) m_dev = Device::attach(sim, *m_param);
     const uint64_t in = kVec(SPHGPU_F_POS) | M(SPHGPU_F_MASS) | M(SPHGPU_F_SML) | kVec(SPHGPU_F_ACC);
     m_dev->upload(*sim, in);
-    m_dev->check(sphgpu_build_tree(m_dev->ctx()), "sphgpu_build_tree");
+    m_dev->ensure_tree();
     m_dev->check(sphgpu_gravity(m_dev->ctx()), "sphgpu_gravity");
     m_dev->download(*sim, kVec(SPHGPU_F_ACC) | M(SPHGPU_F_PHI));
 }

@@ -7,10 +7,15 @@
 // together with the reference sources (they include its headers); see the
 // integration notes for the two-line change in Solver::initialize.
 //
-// Each call uploads the fields the module reads from the Simulation's AoS
-// particles, runs the module on the device and writes the fields it produces
-// back, so GPU and CPU modules can be mixed freely.  All GPU modules of one
-// Simulation share one device context.
+// Each call hands the module's input fields from the Simulation's AoS
+// particles to the device, runs the module there and writes the fields it
+// produces back, so GPU and CPU modules can be mixed freely.  All GPU modules
+// of one Simulation share one device context, which stays resident: a field
+// row crosses to the device only when its host values differ from the ones
+// the device already holds (the driver's predict/correct and any CPU module
+// change rows; a GPU module's own outputs do not need to come back), and the
+// device tree is rebuilt only when the positions changed, so FluidForce and
+// GravityForce reuse PreInteraction's tree and its derived pair lists.
 #pragma once
 
 #include <cstdint>
@@ -39,17 +44,32 @@ public:
                                           const SPHParameters &param);
     ~Device();
     sphgpu_ctx *ctx() const { return m_ctx; }
-    // copy the selected fields between the Simulation's particles and the device
-    void upload(Simulation &sim, uint64_t mask);
+    // hand the selected fields of the Simulation's particles to the device
+    // (only rows whose values changed cross); returns the rows sent
+    uint64_t upload(Simulation &sim, uint64_t mask);
+    // copy the selected device fields into the Simulation's particles
     void download(Simulation &sim, uint64_t mask);
+    // the device tree over the current positions: built unless the last
+    // build saw the same positions
+    void ensure_tree();
+    // the device's kernel_size values over the current smoothing lengths
+    void ensure_kernel_size();
+    // the kernel sizes were just set on the device (by the pre-interaction)
+    void mark_kernel_size() { m_ks = m_tree; }
     void check(int rc, const char *what) const;  // THROW_ERROR on failure
+    // rows uploaded / downloaded and tree builds, for measurement
+    uint64_t rows_up = 0, rows_down = 0, rows_skipped = 0, tree_builds = 0;
 
 private:
     Device() = default;
     sphgpu_ctx *m_ctx = nullptr;
     int m_num = 0;
     bool m_gsph = false;
-    std::vector<double> m_block;  // [SPHGPU_NFIELDS][n]
+    std::vector<double> m_block;  // device values of the synced rows, [SPHGPU_NFIELDS][n]
+    std::vector<double> m_stage;  // host values being handed over
+    uint64_t m_synced = 0;        // rows of m_block equal to the device's
+    bool m_tree = false;          // the device tree matches the device positions
+    bool m_ks = false;            // its kernel sizes match the device's h
 };
 
 sphgpu_params to_sphgpu_params(const SPHParameters &p);

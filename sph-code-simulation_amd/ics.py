@@ -204,10 +204,42 @@ def evrard_3d(n: int = 1_000_000, seed: int = DEFAULT_SEED) -> tuple[dict, np.nd
     return p, _finish(f, p)
 
 
+def evrard_lattice_3d(n_side: int = 124) -> tuple[dict, np.ndarray]:
+    """3D Evrard collapse on the reference's own initial lattice
+    (src/sample/evrard.cpp:12-86): cell centres of an n_side^3 grid on
+    [-1, 1]^3 inside the unit sphere, stretched radially r -> r^1.5 so that
+    rho ~ 1/r; equal masses 1/n, u = 0.05 G, rho = 1/(2 pi r).  Vectorised
+    with the generator's arithmetic (tests/test_samples.py compares it with
+    the loop form).  n_side = 124 gives 998,024 particles.  Parameters as in
+    evrard_3d (sample/evrard/evrard.json)."""
+    p, _ = evrard_3d(n=8)
+    dx = 2.0 / n_side
+    c = (np.arange(n_side, dtype=np.float64) + 0.5) * dx - 1.0
+    x, y, z = np.meshgrid(c, c, c, indexing="ij")
+    x, y, z = x.ravel(), y.ravel(), z.ravel()
+    r0 = np.sqrt(x * x + y * y + z * z)
+    keep = ~(r0 > 1.0)
+    x, y, z, r0 = x[keep], y[keep], z[keep], r0[keep]
+    # libm's pow, as the generator calls it (numpy's vectorised power may
+    # round differently in the last place)
+    s = np.array([math.pow(v, 1.5) / v if v > 0.0 else 1.0 for v in r0.tolist()])
+    x, y, z = x * s, y * s, z * s
+    n = x.size
+    f = empty_fields(n)
+    f[F_POS], f[F_POS + 1], f[F_POS + 2] = x, y, z
+    f[F_MASS] = 1.0 / n
+    u = 0.05 * p["gravity_constant"]
+    dens = 1.0 / (2.0 * math.pi * np.sqrt(x * x + y * y + z * z))
+    f[F_DENS] = dens
+    f[F_ENE] = u
+    f[F_PRES] = (p["gamma"] - 1.0) * dens * u
+    return p, _finish(f, p)
+
+
 WORKLOADS = {
     "sod_1d_ssph_500": sod_1d,
     "khi_2d_disph_65k": khi_2d,
     "sedov_3d_gsph_262k": sedov_3d,
     "sedov_3d_gsph_1m": lambda **kw: sedov_3d(n1=100, **kw),
-    "evrard_3d_disph_1m": evrard_3d,
+    "evrard_3d_disph_1m": evrard_lattice_3d,
 }

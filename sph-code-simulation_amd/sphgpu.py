@@ -48,7 +48,8 @@ class Stats(C.Structure):
         "search_member_tests")] + [
         ("max_depth", C.c_int32), ("pad", C.c_int32),
         ("search_windows", C.c_int64), ("search_leaves", C.c_int64), ("search_exact", C.c_int64),
-        ("flist_h_beyond", C.c_int64), ("flist_other", C.c_int64), ("flist_searches", C.c_int64)]
+        ("flist_h_beyond", C.c_int64), ("flist_other", C.c_int64), ("flist_searches", C.c_int64),
+        ("search_reach_tests", C.c_int64)]
 
     def as_dict(self) -> dict:
         return {n: getattr(self, n) for n, _ in self._fields_ if n != "pad"}
@@ -550,7 +551,9 @@ class Context:
         return out
 
     # ---- timing ------------------------------------------------------------
-    def enable_timing(self, on: bool = True):
+    def enable_timing(self, on: int = 1):
+        """0 off, 1 events around every kernel group, 2 around the pair
+        kernels only (searches, PreInteraction, FluidForce, gravity)."""
         self._check(self.lib.sphgpu_enable_timing(self.h, int(on)), "enable_timing")
 
     def copy_probe(self, nbytes: int) -> float:

@@ -1,0 +1,125 @@
+"""Whole runs on the GPU against the oracle.
+
+* The CPU-runnable workload to its end: 1D Sod (SSPH, cubic spline, 32
+  neighbours, 500 particles) from t = 0 to t = 0.2, the reference's own
+  shock-tube end time (src/sample/shock_tube.cpp; the oracle is bit-exact
+  with the reference on this run, tests/test_oracle_reference.py).  Both runs
+  take the same number of steps (the oracle's count to t >= 0.2); the GPU's
+  dt sequence follows from its own fields.
+* The 3D Evrard collapse (DISPH, Wendland C4, time-dependent viscosity,
+  self-gravity with theta = 0.5) over a longer window than the per-step tests:
+  energy conservation and the energy components against the oracle.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import ics
+import sphgpu
+from checkers import Oracle
+from parity_util import force_errors, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def _run_pair(p, f, steps):
+    o = Oracle(p, f)
+    o.initialize()
+    e_o0 = o.energy()
+    g = sphgpu.Context(p, f.shape[1])
+    g.upload(f)
+    g.initialize()
+    e_g0 = g.energy_momentum()
+    o.integrate(steps)
+    g.integrate(steps)
+    return o, g, e_o0, e_g0
+
+
+def _sod_steps() -> int:
+    p, f = ics.sod_1d()
+    o = Oracle(p, f)
+    o.initialize()
+    steps = 0
+    while o.time < 0.2:
+        o.integrate(1)
+        steps += 1
+    o.close()
+    return steps
+
+
+@pytest.mark.parametrize("prec", [ics.PAIR_FP64, ics.PAIR_FP32])
+def test_sod_to_t_end(prec):
+    """C1 to t = 0.2.  FP64 pairs: every field within 1e-8 of the oracle
+    after the whole run (the per-step bar is 1e-9; the differences are
+    summation order and the ulp of pow, carried through all steps).  FP32
+    pairs: density, pressure, velocity and energy within 2e-3 of the
+    field's scale at every particle, and the total energy within 1e-6."""
+    steps = _sod_steps()
+    p, f = ics.sod_1d()
+    p["pair_precision"] = prec
+    o, g, e_o0, e_g0 = _run_pair(p, f, steps)
+    gf, of = g.download(), o.get_fields()
+    t_o, t_g = o.time, g.time
+    e_o1, e_g1 = o.energy(), g.energy_momentum()
+    g.close()
+    o.close()
+    assert np.isfinite(gf).all()
+    assert t_o >= 0.2
+    drift_o = (e_o1[3] - e_o0[3]) / abs(e_o0[3])
+    drift_g = (e_g1[3] - e_g0[3]) / abs(e_g0[3])
+    e_rho = rel_err(gf[ics.F_DENS], of[ics.F_DENS])
+    e_pres = rel_err(gf[ics.F_PRES], of[ics.F_PRES])
+    e_ene = rel_err(gf[ics.F_ENE], of[ics.F_ENE])
+    # velocities on the scale of the post-shock flow speed
+    v_scale = np.abs(of[ics.F_VEL]).max()
+    e_vel = np.abs(gf[ics.F_VEL] - of[ics.F_VEL]) / v_scale
+    e_x = np.abs(gf[ics.F_POS] - of[ics.F_POS])
+    print(f"sod prec={prec}: {steps} steps, t {t_o:.6f} (gpu {t_g:.6f}); dens {e_rho.max():.3e} "
+          f"pres {e_pres.max():.3e} ene {e_ene.max():.3e} vel {e_vel.max():.3e} x {e_x.max():.3e}; "
+          f"drift oracle {drift_o:.6e} gpu {drift_g:.6e}")
+    if prec == ics.PAIR_FP64:
+        tol, tol_e = 1e-8, 1e-12
+        assert abs(t_g - t_o) <= 1e-12
+    else:
+        tol, tol_e = 2e-3, 1e-6
+        assert abs(t_g - t_o) <= 1e-4 * t_o
+    for name, e in (("dens", e_rho), ("pres", e_pres), ("ene", e_ene), ("vel", e_vel)):
+        assert e.max() < tol, f"{name}: {e.max():.3e} >= {tol:.1e}"
+    assert e_x.max() < tol * 1e-2
+    assert abs(drift_g - drift_o) < tol_e
+    np.testing.assert_array_equal(gf[ics.F_ID], of[ics.F_ID])
+
+
+def test_evrard_energy_window():
+    """Evrard collapse, 20k particles, 100 steps (to t ~ 0.4) (DISPH, Wendland, time-
+    dependent AV, gravity theta 0.5), FP32 pairs: kinetic, thermal and
+    potential energy within 1e-4 of the oracle's (relative to |E_pot|), the
+    drift of the total within 1e-4 of the oracle's drift, and the fields of
+    the last step within the per-step FP32 bars at p99."""
+    p, f = ics.evrard_3d(n=20000)
+    p["pair_precision"] = ics.PAIR_FP32
+    steps = 100
+    o, g, e_o0, e_g0 = _run_pair(p, f, steps)
+    e_o1, e_g1 = o.energy(), g.energy_momentum()
+    gf, of = g.download(), o.get_fields()
+    t_o, t_g = o.time, g.time
+    g.close()
+    o.close()
+    assert np.isfinite(gf).all()
+    scale = abs(e_o1[2])
+    comp = [abs(e_g1[k] - e_o1[k]) / scale for k in range(3)]
+    drift_o = (e_o1[3] - e_o0[3]) / abs(e_o0[3])
+    drift_g = (e_g1[3] - e_g0[3]) / abs(e_g0[3])
+    e_rho = rel_err(gf[ics.F_DENS], of[ics.F_DENS])
+    e_phi = rel_err(gf[ics.F_PHI], of[ics.F_PHI])
+    e_acc, _ = force_errors(gf, of, 3)
+    print(f"evrard 20k {steps} steps: t {t_o:.5f} (gpu {t_g:.5f}); K/U/W err {comp[0]:.2e} {comp[1]:.2e} "
+          f"{comp[2]:.2e}; drift oracle {drift_o:.4e} gpu {drift_g:.4e}; dens p99 {np.quantile(e_rho, 0.99):.2e} "
+          f"max {e_rho.max():.2e}; phi max {e_phi.max():.2e}; acc p99 {np.quantile(e_acc, 0.99):.2e}")
+    assert abs(t_g - t_o) <= 1e-4 * t_o
+    assert max(comp) < 1e-4
+    assert abs(drift_g - drift_o) < 1e-4
+    assert np.quantile(e_rho, 0.99) < 1e-4
+    assert np.quantile(e_phi, 0.99) < 1e-4
+    assert np.quantile(e_acc, 0.99) < 1e-3

@@ -125,3 +125,18 @@ def test_load_sample_by_key():
     assert f.shape[1] == 100 and p["sph_type"] == ics.GDISPH and p["gsph_2nd_order"] == 0
     with pytest.raises(ValueError):
         samples.load_sample(None, {"N": 10})
+
+
+def test_evrard_lattice_workload_equals_sample_generator():
+    """ics.evrard_lattice_3d (vectorised, the 1M Evrard workload at
+    n_side 124) builds the same fields as the loop restatement of
+    src/sample/evrard.cpp in samples.py, bit for bit."""
+    import json
+    import numpy as np
+    p, f = ics.evrard_lattice_3d(22)
+    j = json.loads((Path(__file__).resolve().parent / "golden" / "sample_configs" / "evrard.json").read_text())
+    j["N"] = 22
+    _, g = samples.load_sample("evrard", j)
+    assert f.shape == g.shape
+    for row in (ics.F_POS, ics.F_POS + 1, ics.F_POS + 2, ics.F_MASS, ics.F_DENS, ics.F_PRES, ics.F_ENE):
+        np.testing.assert_array_equal(f[row], g[row])
