@@ -6,8 +6,8 @@ network and mean-squared loss in FP64 numpy, with the backward pass written
 out by hand and itself pinned by central finite differences
 (tests/test_mlp_oracle.py).
 
-Network 6 -> 64 -> 64 -> 1 with ReLU; parameters in one vector
-[W1 (64x6), b1, W2 (64x64), b2, w3, b3].  Features of a pair row
+Network 6 -> 64 -> 64 -> 64 -> 1 with ReLU; parameters in one vector
+[W1 (64x6), b1, W2 (64x64), b2, W3 (64x64), b3, w4, b4] (8833 values).  Features of a pair row
 (u_L, rho_L, p_L, u_R, rho_R, p_R, p*): with pair means rho_m, p_m, u_m and
 c_m = sqrt(gamma p_m / rho_m): (rho/rho_m, p/p_m, (u-u_m)/c_m) for L and R;
 target log(p*/p_m).
@@ -22,8 +22,10 @@ OFF_B1 = OFF_W1 + H * IN
 OFF_W2 = OFF_B1 + H
 OFF_B2 = OFF_W2 + H * H
 OFF_W3 = OFF_B2 + H
-OFF_B3 = OFF_W3 + H
-N_PARAMS = OFF_B3 + 1
+OFF_B3 = OFF_W3 + H * H
+OFF_W4 = OFF_B3 + H
+OFF_B4 = OFF_W4 + H
+N_PARAMS = OFF_B4 + 1
 
 
 def unpack(w: np.ndarray):
@@ -32,9 +34,11 @@ def unpack(w: np.ndarray):
     b1 = w[OFF_B1:OFF_W2]
     W2 = w[OFF_W2:OFF_B2].reshape(H, H)
     b2 = w[OFF_B2:OFF_W3]
-    w3 = w[OFF_W3:OFF_B3]
-    b3 = w[OFF_B3]
-    return W1, b1, W2, b2, w3, b3
+    W3 = w[OFF_W3:OFF_B3].reshape(H, H)
+    b3 = w[OFF_B3:OFF_W4]
+    w4 = w[OFF_W4:OFF_B4]
+    b4 = w[OFF_B4]
+    return W1, b1, W2, b2, W3, b3, w4, b4
 
 
 def features(rows: np.ndarray, gamma: float):
@@ -49,29 +53,34 @@ def features(rows: np.ndarray, gamma: float):
 
 
 def forward(w: np.ndarray, x: np.ndarray):
-    W1, b1, W2, b2, w3, b3 = unpack(w)
+    W1, b1, W2, b2, W3, b3, w4, b4 = unpack(w)
     z1 = x @ W1.T + b1
     h1 = np.maximum(z1, 0.0)
     z2 = h1 @ W2.T + b2
     h2 = np.maximum(z2, 0.0)
-    out = h2 @ w3 + b3
-    return out, (z1, h1, z2, h2)
+    z3 = h2 @ W3.T + b3
+    h3 = np.maximum(z3, 0.0)
+    out = h3 @ w4 + b4
+    return out, (z1, h1, z2, h2, z3, h3)
 
 
 def loss_grad(w: np.ndarray, rows: np.ndarray, gamma: float):
     """Mean squared error of log(p*/p_m) and its gradient (FP64)."""
     x, p_m = features(rows, gamma)
     y = np.log(np.asarray(rows, dtype=np.float64)[:, 6] / p_m)
-    out, (z1, h1, z2, h2) = forward(w, x)
-    W1, b1, W2, b2, w3, b3 = unpack(w)
+    out, (z1, h1, z2, h2, z3, h3) = forward(w, x)
+    W1, b1, W2, b2, W3, b3, w4, b4 = unpack(w)
     B = len(y)
     d = out - y
     loss = float(np.mean(d * d))
     dout = 2.0 * d / B
     g = np.zeros(N_PARAMS)
-    g[OFF_W3:OFF_B3] = h2.T @ dout
-    g[OFF_B3] = dout.sum()
-    dz2 = np.outer(dout, w3) * (z2 > 0)
+    g[OFF_W4:OFF_B4] = h3.T @ dout
+    g[OFF_B4] = dout.sum()
+    dz3 = np.outer(dout, w4) * (z3 > 0)
+    g[OFF_W3:OFF_B3] = (dz3.T @ h2).ravel()
+    g[OFF_B3:OFF_W4] = dz3.sum(axis=0)
+    dz2 = (dz3 @ W3) * (z2 > 0)
     g[OFF_W2:OFF_B2] = (dz2.T @ h1).ravel()
     g[OFF_B2:OFF_W3] = dz2.sum(axis=0)
     dz1 = (dz2 @ W2) * (z1 > 0)

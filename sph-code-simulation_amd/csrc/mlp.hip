@@ -105,14 +105,19 @@ struct TrainLDS {
     bf16_t W1[kMlpH][kLd1];     // [out][in]
     bf16_t H1[kTile][kLd];      // relu(z1) [row][h]
     bf16_t H1T[kMlpH][kLd];     // [h][row]
+    bf16_t H2[kTile][kLd];      // relu(z2) [row][h]
+    bf16_t H2T[kMlpH][kLd];     // [h][row]
     bf16_t W2[kMlpH][kLd];      // [out][in]
     bf16_t W2T[kMlpH][kLd];     // [in][out]
-    bf16_t dZ2[kTile][kLd];     // [row][out]
-    bf16_t dZ2T[kMlpH][kLd];    // [out][row]
-    bf16_t dZ1T[kMlpH][kLd];    // [h][row]
-    float F[kTile][kMlpH + 1];  // relu(z2), later dz1 (FP32)
+    bf16_t W3[kMlpH][kLd];      // [out][in]
+    bf16_t W3T[kMlpH][kLd];     // [in][out]
+    bf16_t DA[kTile][kLd];      // dz3 [row][out]
+    bf16_t DAT[kMlpH][kLd];     // dz3, later dz1 [out][row]
+    bf16_t DB[kTile][kLd];      // dz2 [row][out]
+    bf16_t DBT[kMlpH][kLd];     // dz2 [out][row]
+    float F[kTile][kMlpH + 1];  // relu(z3), later dz2, then dz1 (FP32)
     float y[kTile], dout[kTile], wt[kTile], loss[kTile];
-    float b1[kMlpH], b2[kMlpH], w3[kMlpH];
+    float b1[kMlpH], b2[kMlpH], b3[kMlpH], w4[kMlpH];
 };
 
 // C (M x N) = A (M x K) * B (K x N) with A row-major [M][lda] and B given by
@@ -150,14 +155,18 @@ __global__ __launch_bounds__(256) void k_mlp_fwd_bwd(const float *__restrict__ r
     }
     for (int e = tid; e < kMlpH * kMlpH; e += blockDim.x) {
         const int o = e / kMlpH, k = e % kMlpH;
-        const bf16_t v = (bf16_t)w[kOffW2 + o * kMlpH + k];
-        L.W2[o][k] = v;
-        L.W2T[k][o] = v;
+        const bf16_t v2 = (bf16_t)w[kOffW2 + o * kMlpH + k];
+        L.W2[o][k] = v2;
+        L.W2T[k][o] = v2;
+        const bf16_t v3 = (bf16_t)w[kOffW3 + o * kMlpH + k];
+        L.W3[o][k] = v3;
+        L.W3T[k][o] = v3;
     }
     if (tid < kMlpH) {
         L.b1[tid] = w[kOffB1 + tid];
         L.b2[tid] = w[kOffB2 + tid];
-        L.w3[tid] = w[kOffW3 + tid];
+        L.b3[tid] = w[kOffB3 + tid];
+        L.w4[tid] = w[kOffW4 + tid];
     }
     if (tid < kTile) {
         const long long r = base + tid;
@@ -179,64 +188,87 @@ __global__ __launch_bounds__(256) void k_mlp_fwd_bwd(const float *__restrict__ r
         L.wt[tid] = wt;
     }
     __syncthreads();
-    // layer 1: z1 = X W1^T + b1
+    // forward: z1 = X W1^T + b1, z2 = H1 W2^T + b2, z3 = H2 W3^T + b3
     wave_gemm(&L.X[0][0], kLd1, &L.W1[0][0], kLd1, kTile, kMlpH, kK1, [&](int r, int c, float v) {
         const float h = fmaxf(v + L.b1[c], 0.f);
         L.H1[r][c] = (bf16_t)h;
         L.H1T[c][r] = (bf16_t)h;
     });
     __syncthreads();
-    // layer 2: z2 = H1 W2^T + b2
     wave_gemm(&L.H1[0][0], kLd, &L.W2[0][0], kLd, kTile, kMlpH, kMlpH, [&](int r, int c, float v) {
-        L.F[r][c] = fmaxf(v + L.b2[c], 0.f);
+        const float h = fmaxf(v + L.b2[c], 0.f);
+        L.H2[r][c] = (bf16_t)h;
+        L.H2T[c][r] = (bf16_t)h;
+    });
+    __syncthreads();
+    wave_gemm(&L.H2[0][0], kLd, &L.W3[0][0], kLd, kTile, kMlpH, kMlpH, [&](int r, int c, float v) {
+        L.F[r][c] = fmaxf(v + L.b3[c], 0.f);
     });
     __syncthreads();
     const float inv_b = 1.0f / (float)batch;
     if (tid < kTile) {
-        float out = w[kOffB3];
-        for (int o = 0; o < kMlpH; ++o) out = __builtin_fmaf(L.w3[o], L.F[tid][o], out);
+        float out = w[kOffB4];
+        for (int o = 0; o < kMlpH; ++o) out = __builtin_fmaf(L.w4[o], L.F[tid][o], out);
         const float d = out - L.y[tid];
         L.loss[tid] = L.wt[tid] * d * d;
         L.dout[tid] = L.wt[tid] * 2.f * d * inv_b;
     }
     __syncthreads();
     float *gp = gpart + (size_t)blockIdx.x * kMlpParams;
+    // output layer: dw4 = sum dout relu(z3), db4 = sum dout; db3 = sum dz3
     if (tid < kMlpH) {
         float s = 0.f, sb = 0.f;
         for (int r = 0; r < kTile; ++r) {
             s = __builtin_fmaf(L.dout[r], L.F[r][tid], s);
-            const float dz = (L.F[r][tid] > 0.f) ? L.dout[r] * L.w3[tid] : 0.f;
-            sb += dz;
+            sb += (L.F[r][tid] > 0.f) ? L.dout[r] * L.w4[tid] : 0.f;
         }
-        gp[kOffW3 + tid] = s;
-        gp[kOffB2 + tid] = sb;
+        gp[kOffW4 + tid] = s;
+        gp[kOffB3 + tid] = sb;
     }
     if (tid == 0) {
         float s = 0.f, ls = 0.f;
         for (int r = 0; r < kTile; ++r) { s += L.dout[r]; ls += L.loss[r]; }
-        gp[kOffB3] = s;
+        gp[kOffB4] = s;
         lpart[blockIdx.x] = ls * inv_b;
     }
     for (int e = tid; e < kTile * kMlpH; e += blockDim.x) {
         const int r = e / kMlpH, o = e % kMlpH;
-        const float dz = (L.F[r][o] > 0.f) ? L.dout[r] * L.w3[o] : 0.f;
+        const float dz = (L.F[r][o] > 0.f) ? L.dout[r] * L.w4[o] : 0.f;
         const bf16_t v = (bf16_t)dz;
-        L.dZ2[r][o] = v;
-        L.dZ2T[o][r] = v;
+        L.DA[r][o] = v;
+        L.DAT[o][r] = v;
     }
     __syncthreads();
-    // dW2 = dZ2^T H1 ; dH1 = dZ2 W2 (masked by relu'(z1) -> dz1)
-    wave_gemm(&L.dZ2T[0][0], kLd, &L.H1T[0][0], kLd, kMlpH, kMlpH, kTile, [&](int r, int c, float v) {
-        gp[kOffW2 + r * kMlpH + c] = v;
+    // dW3 = dZ3^T H2 ; dH2 = dZ3 W3, masked by relu'(z2) -> dz2
+    wave_gemm(&L.DAT[0][0], kLd, &L.H2T[0][0], kLd, kMlpH, kMlpH, kTile, [&](int r, int c, float v) {
+        gp[kOffW3 + r * kMlpH + c] = v;
     });
-    wave_gemm(&L.dZ2[0][0], kLd, &L.W2T[0][0], kLd, kTile, kMlpH, kMlpH, [&](int r, int c, float v) {
-        const float dz = ((float)L.H1[r][c] > 0.f) ? v : 0.f;
+    wave_gemm(&L.DA[0][0], kLd, &L.W3T[0][0], kLd, kTile, kMlpH, kMlpH, [&](int r, int c, float v) {
+        const float dz = ((float)L.H2[r][c] > 0.f) ? v : 0.f;
         L.F[r][c] = dz;
-        L.dZ1T[c][r] = (bf16_t)dz;
+        L.DB[r][c] = (bf16_t)dz;
+        L.DBT[c][r] = (bf16_t)dz;
     });
     __syncthreads();
-    // dW1 = dZ1^T X (first 16 input columns computed, 6 used)
-    wave_gemm(&L.dZ1T[0][0], kLd, &L.XT[0][0], kLd, kMlpH, 16, kTile, [&](int r, int c, float v) {
+    // db2 = sum dz2 ; dW2 = dZ2^T H1
+    if (tid < kMlpH) {
+        float s = 0.f;
+        for (int r = 0; r < kTile; ++r) s += L.F[r][tid];
+        gp[kOffB2 + tid] = s;
+    }
+    wave_gemm(&L.DBT[0][0], kLd, &L.H1T[0][0], kLd, kMlpH, kMlpH, kTile, [&](int r, int c, float v) {
+        gp[kOffW2 + r * kMlpH + c] = v;
+    });
+    __syncthreads();
+    // dH1 = dZ2 W2, masked by relu'(z1) -> dz1
+    wave_gemm(&L.DB[0][0], kLd, &L.W2T[0][0], kLd, kTile, kMlpH, kMlpH, [&](int r, int c, float v) {
+        const float dz = ((float)L.H1[r][c] > 0.f) ? v : 0.f;
+        L.F[r][c] = dz;
+        L.DAT[c][r] = (bf16_t)dz;
+    });
+    __syncthreads();
+    // db1 = sum dz1 ; dW1 = dZ1^T X (first 16 input columns computed, 6 used)
+    wave_gemm(&L.DAT[0][0], kLd, &L.XT[0][0], kLd, kMlpH, 16, kTile, [&](int r, int c, float v) {
         if (c < kMlpIn) gp[kOffW1 + r * kMlpIn + c] = v;
     });
     if (tid < kMlpH) {
@@ -292,7 +324,8 @@ int sphgpu_mlp_init(sphgpu_ctx *c, uint64_t seed)
         n3(0.f, std::sqrt(1.0f / kMlpH));
     for (int i = 0; i < kMlpH * kMlpIn; ++i) w[kOffW1 + i] = n1(rng);
     for (int i = 0; i < kMlpH * kMlpH; ++i) w[kOffW2 + i] = n2(rng);
-    for (int i = 0; i < kMlpH; ++i) w[kOffW3 + i] = n3(rng);
+    for (int i = 0; i < kMlpH * kMlpH; ++i) w[kOffW3 + i] = n2(rng);
+    for (int i = 0; i < kMlpH; ++i) w[kOffW4 + i] = n3(rng);
     HIPCHK(c, hipMemcpy(c->mlp->w, w.data(), sizeof(float) * kMlpParams, hipMemcpyHostToDevice));
     c->mlp->trained = true;
     c->mlp->img_dirty = true;

@@ -83,20 +83,34 @@ __global__ void k_bbox_partial(const double *__restrict__ x, const double *__res
         }
 }
 
-__global__ void k_bbox_final(const double *__restrict__ part, int nblocks, int dim, double *__restrict__ root)
+// final bounds over the block partials: 256 threads, then a tree in LDS
+// (min and max are exact in any order)
+__global__ __launch_bounds__(256) void k_bbox_final(const double *__restrict__ part, int nblocks, int dim,
+                                                    double *__restrict__ root)
 {
-    if (threadIdx.x != 0) return;
+    __shared__ double smin[3][256], smax[3][256];
     double mn[3] = {1.7976931348623157e308, 1.7976931348623157e308, 1.7976931348623157e308};
     double mx[3] = {-1.7976931348623157e308, -1.7976931348623157e308, -1.7976931348623157e308};
-    for (int b = 0; b < nblocks; ++b)
+    for (int b = threadIdx.x; b < nblocks; b += blockDim.x)
         for (int k = 0; k < 3; ++k) {
             if (part[b * 6 + k] < mn[k]) mn[k] = part[b * 6 + k];
             if (part[b * 6 + 3 + k] > mx[k]) mx[k] = part[b * 6 + 3 + k];
         }
+    for (int k = 0; k < 3; ++k) { smin[k][threadIdx.x] = mn[k]; smax[k][threadIdx.x] = mx[k]; }
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s)
+            for (int k = 0; k < 3; ++k) {
+                if (smin[k][threadIdx.x + s] < smin[k][threadIdx.x]) smin[k][threadIdx.x] = smin[k][threadIdx.x + s];
+                if (smax[k][threadIdx.x + s] > smax[k][threadIdx.x]) smax[k][threadIdx.x] = smax[k][threadIdx.x + s];
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
     double l = 0.0;
     for (int k = 0; k < dim; ++k) {
-        root[k] = (mx[k] + mn[k]) * 0.5;
-        const double range = mx[k] - mn[k];
+        root[k] = (smax[k][0] + smin[k][0]) * 0.5;
+        const double range = smax[k][0] - smin[k][0];
         if (l < range) l = range;
     }
     for (int k = dim; k < 3; ++k) root[k] = 0.0;
@@ -970,7 +984,7 @@ int tree_build(sphgpu_ctx *c)
     if (!c->prm.periodic) {
         const int nbb = std::min(c->red_blocks, grid_for(n, 256));
         hipLaunchKernelGGL(k_bbox_partial, dim3(nbb), dim3(256), 0, c->stream, px, py, pz, n, c->dim, c->red);
-        hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(64), 0, c->stream, (const double *)c->red, nbb, c->dim, t.root);
+        hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(256), 0, c->stream, (const double *)c->red, nbb, c->dim, t.root);
     }
     hipLaunchKernelGGL(k_keys, dim3(grid_for(n, 256)), dim3(256), 0, c->stream, px, py, pz, (const double *)t.root, sh,
                        t.key, t.perm, t.sync_words);

@@ -31,7 +31,7 @@ def _network(p, n1=24, steps=3):
     q = dict(p, pair_precision=ics.PAIR_FP32)
     _, f = ics.sedov_3d(n1=n1)
     rows = learned.log_pairs(q, f, steps=steps, capacity=4_000_000)
-    w, info = learned.train(q, rows, epochs=10, batch=1024, lr=2e-3)
+    w, info = learned.train(q, rows)
     print("network:", info)
     assert info["loss_final"] < 0.2 * info["loss_initial"]
     return w

@@ -56,7 +56,7 @@ def test_params_struct_layout_matches_header():
 
 def test_mlp_param_count():
     lib = sphgpu.load_library()
-    assert lib.sphgpu_mlp_param_count() == 6 * 64 + 64 + 64 * 64 + 64 + 64 + 1
+    assert lib.sphgpu_mlp_param_count() == 6 * 64 + 64 + 64 * 64 + 64 + 64 * 64 + 64 + 64 + 1  # 6-64-64-64-1
 
 
 def test_create_rejects_bad_parameters_without_touching_device():

@@ -30,7 +30,7 @@ def test_backward_matches_finite_differences():
     w = rng.normal(0, 0.3, M.N_PARAMS)
     rows = _rows(50)
     loss, g = M.loss_grad(w, rows, 5.0 / 3.0)
-    idx = np.concatenate([rng.choice(M.N_PARAMS, 200, replace=False), [M.OFF_B3, M.OFF_B1, M.OFF_W3]])
+    idx = np.concatenate([rng.choice(M.N_PARAMS, 200, replace=False), [M.OFF_B4, M.OFF_B3, M.OFF_B1, M.OFF_W4, M.OFF_W3]])
     eps = 1e-6
     for k in idx:
         wp = w.copy(); wp[k] += eps

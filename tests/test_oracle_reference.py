@@ -128,6 +128,39 @@ def test_reference_sedov_low_background_pressure_breaks(solver, n1, pbg):
     assert np.isfinite(u2).all() and (u2 > 0).all()
 
 
+@pytest.mark.parametrize("solver", ["hll", "iterative"])
+def test_reference_strong_shock_preset_goes_nan(solver):
+    """The reference's strong-shock sample (src/sample/strong_shock.cpp,
+    pressure ratio 1e4) with its GSPH preset goes NaN before t_end: the
+    preset's neighborNumber 5.2 is read by an int key as the default 50, and
+    the second-order GSPH blows up at step 85 (HLL) or 84 (iterative).  The
+    oracle follows it step for step, so no strong-contrast run of this sample
+    exists to train or test the learned guess on."""
+    import json
+    from pathlib import Path
+    import samples
+    if not ref_available(1):
+        pytest.skip("oracle/_ref not built")
+    j = json.loads((Path(__file__).resolve().parent / "golden" / "sample_configs"
+                    / "strong_shock_gsph_cubic.json").read_text())
+    j["riemannSolver"] = solver
+    p, f = samples.load_sample("strong_shock", j)
+    assert p["neighbor_number"] == 50
+    o, r = _both(p, f)
+    o.initialize()
+    r.initialize()
+    first_bad = None
+    for step in range(1, 121):
+        o.integrate(1)
+        r.integrate(1)
+        fr = r.get_fields()
+        np.testing.assert_array_equal(np.isfinite(o.get_fields()), np.isfinite(fr))
+        if not np.isfinite(fr).all():
+            first_bad = step
+            break
+    assert first_bad == (85 if solver == "hll" else 84)
+
+
 def _gdisph_case(name):
     from parity_util import SMALL
     if name in SMALL:
