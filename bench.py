@@ -168,6 +168,34 @@ def evrard_entry(steps: int, warmup: int, kernel_table: bool = True) -> dict:
             "kernel_ms": {k: round(v[0], 4) for k, v in res["times_all"].items()}}
 
 
+def evrard_full_run(t_end: float = 0.8, wall_cap_s: float = 120.0) -> dict:
+    """The Evrard workload as a whole run: 998,592 particles from t = 0 to
+    t_end (through the collapse, FP32 pairs, FP64 integration), with the
+    conservation of total energy and momentum over the run.  Steps go in
+    batches of 20 and the run stops early (reported) past wall_cap_s."""
+    p, f = make_workload("evrard_3d_disph_1m", "fp32")
+    ctx = sphgpu.Context(p, f.shape[1])
+    ctx.upload(f)
+    ctx.initialize()
+    e0 = ctx.energy_momentum()
+    steps = 0
+    t0 = time.perf_counter()
+    while ctx.time < t_end and time.perf_counter() - t0 < wall_cap_s:
+        ctx.integrate(20)
+        steps += 20
+    e1 = ctx.energy_momentum()
+    wall = time.perf_counter() - t0
+    t_sim = ctx.time
+    ctx.close()
+    return {"t_end": t_end, "t_reached": t_sim, "steps": steps, "wall_s": round(wall, 2),
+            "ms_per_step": round(wall / max(steps, 1) * 1e3, 3),
+            "energy_start": [float(v) for v in e0[:4]], "energy_end": [float(v) for v in e1[:4]],
+            "energy_drift": float((e1[3] - e0[3]) / abs(e0[3])),
+            "momentum_end": [float(v) for v in e1[4:7]],
+            "note": "energies are kinetic, thermal, potential, total; at 14,328 particles the same run "
+                    "tracks the oracle to t = 0.8 (tests/test_gpu_long_runs.py)"}
+
+
 def dropin_entry(workload: str, steps: int = 2) -> dict:
     """The module substitution a reference user makes: the reference's own
     Simulation and driver sequence (Solver::integrate, src/solver.cpp:966-978,
@@ -340,6 +368,7 @@ def main() -> None:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-evrard", action="store_true")
+    ap.add_argument("--no-evrard-run", action="store_true", help="skip the Evrard run from t = 0 to 0.8")
     ap.add_argument("--no-kernel-table", action="store_true",
                     help="skip the second pass that times every kernel group (profiler runs: the last "
                          "launches are then the timed steps)")
@@ -435,6 +464,8 @@ def main() -> None:
                      "(training.held_out) and not in the step time")}
     if not args.no_evrard and args.workload != "evrard_3d_disph_1m":
         out["evrard_1m"] = evrard_entry(args.steps, args.warmup, not args.no_kernel_table)
+        if not args.no_evrard_run:
+            out["evrard_1m"]["whole_run"] = evrard_full_run()
     # plain copy of known size: attainable HBM bandwidth, and the calibration
     # kernel (k_copy_probe) for the FETCH_SIZE / WRITE_SIZE counters
     pc, _ = ics.sedov_3d(n1=4)

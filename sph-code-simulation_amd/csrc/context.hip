@@ -265,9 +265,13 @@ int sphgpu_create(const sphgpu_params *p, int n, sphgpu_ctx **out)
          hipHostGetDevicePointer((void **)&c->hflag_dev, c->hflag, 0) == hipSuccess &&
          hipEventCreateWithFlags(&c->flag_ready, hipEventDisableTiming) == hipSuccess;
     ok = ok && tree_alloc(c) == 0;
-    // neighbour lists: about 2.5x the expected candidate count nn * 1.2^D
+    // neighbour lists: about 5x the expected candidate count nn * 1.2^D
+    // (a shock compresses the gas ahead of the smoothing lengths, and a list
+    // that overflows costs a second search in that step; the slot-major
+    // layout touches only the slots a particle fills, so capacity costs
+    // memory, 4 B per slot and particle, not traffic)
     const double kap = p->iterative_sml ? std::pow(1.2, p->dim) : 1.0;
-    int cap = (int)std::ceil(p->neighbor_number * kap * 2.5 / 32.0) * 32;
+    int cap = (int)std::ceil(p->neighbor_number * kap * 5.0 / 64.0) * 64;
     cap = std::max(cap, 64);
     ok = ok && lists_alloc(c, c->lpre, cap) == 0 && lists_alloc(c, c->lforce, cap) == 0;
     ok = ok && mlp_create(c) == 0;

@@ -123,3 +123,63 @@ def test_evrard_energy_window():
     assert np.quantile(e_rho, 0.99) < 1e-4
     assert np.quantile(e_phi, 0.99) < 1e-4
     assert np.quantile(e_acc, 0.99) < 1e-3
+
+
+def test_evrard_lattice_to_collapse():
+    """The Evrard collapse on the reference's own lattice (n_side 30, 14,328
+    particles; DISPH, Wendland, time-dependent AV, gravity theta 0.5) from
+    t = 0 to t = 0.8, the workload's end time, through the bounce of the
+    central region.  FP64 pairs take the oracle's step count and stay on its
+    trajectory: time within 1e-7, energies within 1e-6 of |E_pot|, drift
+    within 1e-7, density p99 within 1e-5.  FP32 pairs leave the trajectory in the collapse (their
+    dt sequence differs in the last digits and the bounce amplifies it, so
+    they reach t = 0.8 a few steps later); run to the same time, the energy
+    components agree to 5e-4 of |E_pot|, the drift of the total to 1e-4 (the
+    accuracy gate's bar) and the median density to 1e-3."""
+    p, f = ics.evrard_lattice_3d(n_side=30)
+    o = Oracle(p, f)
+    o.initialize()
+    e_o0 = o.energy()
+    steps = 0
+    while o.time < 0.8:
+        o.integrate(1)
+        steps += 1
+    e_o1, of = o.energy(), o.get_fields()
+    t_o = o.time
+    o.close()
+    drift_o = (e_o1[3] - e_o0[3]) / abs(e_o0[3])
+    scale = abs(e_o1[2])
+    for prec in (ics.PAIR_FP64, ics.PAIR_FP32):
+        q = dict(p)
+        q["pair_precision"] = prec
+        g = sphgpu.Context(q, f.shape[1])
+        g.upload(f)
+        g.initialize()
+        e_g0 = g.energy_momentum()
+        if prec == ics.PAIR_FP64:
+            g.integrate(steps)
+            gsteps = steps
+        else:
+            gsteps = 0
+            while g.time < 0.8 and gsteps < 2 * steps:
+                g.integrate(1)
+                gsteps += 1
+        e_g1, gf, t_g = g.energy_momentum(), g.download(), g.time
+        g.close()
+        assert np.isfinite(gf).all()
+        comp = [abs(e_g1[k] - e_o1[k]) / scale for k in range(3)]
+        drift_g = (e_g1[3] - e_g0[3]) / abs(e_g0[3])
+        e_rho = rel_err(gf[ics.F_DENS], of[ics.F_DENS])
+        print(f"evrard lattice 30^3 to t 0.8, pairs {'fp64' if prec else 'fp32'}: steps oracle {steps} gpu {gsteps}, "
+              f"t {t_o:.6f} / {t_g:.6f}; K/U/W err {comp[0]:.2e} {comp[1]:.2e} {comp[2]:.2e}; drift oracle "
+              f"{drift_o:.4e} gpu {drift_g:.4e}; dens p50 {np.median(e_rho):.2e} p99 {np.quantile(e_rho, 0.99):.2e}")
+        if prec == ics.PAIR_FP64:
+            assert abs(t_g - t_o) <= 1e-7 * t_o
+            assert max(comp) < 1e-6
+            assert abs(drift_g - drift_o) < 1e-7
+            assert np.quantile(e_rho, 0.99) < 1e-5
+        else:
+            assert abs(t_g - t_o) <= 0.01 * t_o
+            assert max(comp) < 5e-4
+            assert abs(drift_g - drift_o) < 1e-4
+            assert np.median(e_rho) < 1e-3
