@@ -473,6 +473,33 @@ template <> struct Acc<double> {
     SPH_DEV double get() const { return s; }
 };
 
+// Compensated sum in blocks, for long sums whose terms come one per loop
+// step: each term costs one add into a plain partial p, and fold() adds the
+// partial into the TwoSum pair (s, c) as Acc does.  With at most B terms per
+// block the partial's rounding stays below (B - 1) u sum |x_k| (first order),
+// against 2 u for Acc and (n - 1) u for a plain sum of all n terms.
+template <typename T> struct BlockAcc {
+    T s, c, p;
+    SPH_DEV BlockAcc() : s(T(0)), c(T(0)), p(T(0)) {}
+    SPH_DEV void add(T x) { p += x; }
+    SPH_DEV void fold()
+    {
+        const T t = s + p;
+        const T z = t - s;
+        c += (s - (t - z)) + (p - z);
+        s = t;
+        p = T(0);
+    }
+    SPH_DEV T get() const { return s + (c + p); }
+};
+template <> struct BlockAcc<double> {
+    double s;
+    SPH_DEV BlockAcc() : s(0.0) {}
+    SPH_DEV void add(double x) { s += x; }
+    SPH_DEV void fold() {}
+    SPH_DEV double get() const { return s; }
+};
+
 // Order-preserving integer image of a non-negative double, for atomicMin/Max.
 SPH_DEV unsigned long long dbits(double x) { return (unsigned long long)__double_as_longlong(x); }
 SPH_DEV double bitsd(unsigned long long b) { return __longlong_as_double((long long)b); }

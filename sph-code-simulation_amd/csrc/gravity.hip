@@ -18,7 +18,7 @@
 // leaf members of its own reference walk, in the same pre-order, while node
 // records are wave-uniform loads and member records are loaded once per wave
 // instead of once per lane.  The pair and monopole terms run in the pair
-// precision T with compensated sums.
+// precision T with compensated sums taken in blocks of at most 16 terms.
 #include <hip/hip_runtime.h>
 
 #include <climits>
@@ -143,13 +143,25 @@ __global__ __launch_bounds__(256) void k_gravity(GravArgs<T> a)
     const T hi = pi_.w;
     const T inv_ei = T(2) / hi;
     const T G = T(a.G);
-    Acc<T> phi, ac[3];
+    // compensated sums in blocks of at most 16 terms (fold below)
+    BlockAcc<T> phi, ac[3];
+    auto fold = [&]() {
+        phi.fold();
+#pragma unroll
+        for (int k = 0; k < D; ++k) ac[k].fold();
+    };
+    int since_fold = 0;  // node steps since the last fold (wave-uniform)
     long long inter = 0;
     // the lane takes part in the walk from node `resume` on; a lane that
     // accepted a node's monopole resumes at the node's subtree end
     int resume = valid ? 0 : INT_MAX;
     int node = 0;
     while (node < a.n_nodes) {
+        // a lane adds at most one monopole per node step
+        if (++since_fold == 16) {
+            fold();
+            since_fold = 0;
+        }
         const GravNode g = a.node[node];
         const bool act = node >= resume;
         double d[3] = {0, 0, 0};
@@ -191,7 +203,9 @@ __global__ __launch_bounds__(256) void k_gravity(GravArgs<T> a)
                 }
                 wave_lds_fence();
                 if (open) {
+                    fold();
                     for (int t = 0; t < cnt; ++t) {
+                        if ((t & 15) == 15) fold();
                         if (s0 + t == i) continue;
                         const typename V4T<T>::type q = st.p[t];
                         const T xj[3] = {q.x, q.y, q.z};
@@ -232,6 +246,7 @@ __global__ __launch_bounds__(256) void k_gravity(GravArgs<T> a)
                         for (int k = 0; k < D; ++k) ac[k].add(-(rij[k] * s2));
                         ++inter;
                     }
+                    fold();
                 }
                 wave_lds_fence();
             }

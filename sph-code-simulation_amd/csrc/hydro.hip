@@ -155,16 +155,21 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
         for (; it < 10;) {
             const T hb = h;
             Acc<T> dn, ddn;
-            // candidates in groups of kGroup: the group's index, position
-            // and mass loads are issued together, so a group waits for
-            // memory once; the sums run in candidate order
+            // candidates in groups of kGroup: the group's position and mass
+            // loads are issued together, and the next group's indices are
+            // loaded one group ahead, so a group waits for memory once; the
+            // sums run in candidate order
+            int jn[kGroup];
+            if (nc > 0) list_group(a.idx, i, 0, nc, a.cap, jn);
             for (int q0 = 0; q0 < nc; q0 += kGroup) {
                 int jg[kGroup];
                 V4 pg[kGroup];
                 T wg[kGroup];
-                list_group(a.idx, i, q0, nc, a.cap, jg);
+#pragma unroll
+                for (int u = 0; u < kGroup; ++u) jg[u] = jn[u];
 #pragma unroll
                 for (int u = 0; u < kGroup; ++u) pg[u] = RP[jg[u]];
+                if (q0 + kGroup < nc) list_group(a.idx, i, q0 + kGroup, nc, a.cap, jn);
 #pragma unroll
                 for (int u = 0; u < kGroup; ++u)
                     wg[u] = (TYPE == SPHGPU_DISPH || TYPE == SPHGPU_GDISPH) ? T(1) : rec_mass(RV, jg[u]);
@@ -255,16 +260,21 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
     const bool bal_on = kBal && (do_bal || a.td_av);
     const bool disph_w = (TYPE == SPHGPU_DISPH) || (TYPE == SPHGPU_GDISPH && !do_bal);
     T bdiv = T(0), brot[3] = {0, 0, 0};
+    int jn[kGroup];
+    if (nc > 0) list_group(a.idx, i, 0, nc, a.cap, jn);
     for (int q0 = 0; q0 < nc; q0 += kGroup) {
-    // a group of candidates: indices and positions first, then the records
-    // of the members, each set of loads issued together
+    // a group of candidates: positions first (the indices were loaded one
+    // group ahead), then the records of the members, each set of loads
+    // issued together
     int jg[kGroup];
     V4 pg[kGroup], vg[kGroup];
     T cg[kGroup], ug[kGroup];
     bool memb[kGroup];
-    list_group(a.idx, i, q0, nc, a.cap, jg);
+#pragma unroll
+    for (int u = 0; u < kGroup; ++u) jg[u] = jn[u];
 #pragma unroll
     for (int u = 0; u < kGroup; ++u) pg[u] = RP[jg[u]];
+    if (q0 + kGroup < nc) list_group(a.idx, i, q0 + kGroup, nc, a.cap, jn);
 #pragma unroll
     for (int u = 0; u < kGroup; ++u) {
         T d[3];
@@ -858,11 +868,17 @@ __global__ __launch_bounds__(256) void k_force_sph(ForceArgs<T> a)
     }
     const T dt = (TYPE == SPHGPU_GDISPH) ? T(a.dscal[0]) : T(0);
     const T gamma = T(a.gamma);
-    Acc<T> ac[3], de;
+    // compensated sums in blocks of 16 pairs
+    BlockAcc<T> ac[3], de;
     const int nc = a.cnt[i];
     // the next pair index is loaded one pair ahead
     int j_next = nc > 0 ? a.idx[list_slot(i, 0, a.cap)] : 0;
     for (int q = 0; q < nc; ++q) {
+        if ((q & 15) == 15) {
+#pragma unroll
+            for (int k = 0; k < D; ++k) ac[k].fold();
+            de.fold();
+        }
         const int j = j_next;
         if (q + 1 < nc) j_next = a.idx[list_slot(i, q + 1, a.cap)];
         const V4 pj = RP[j], vj = RV[j], tj = RT[j], aj = RA[j];
@@ -1057,7 +1073,7 @@ SPH_DEV int solve_pair(const T l[4], const T r[4], T gamma, int refine64, const 
 template <int D, int K, typename T, int SOLVER, bool LEARNED>
 SPH_DEV void force_gsph_slots(const ForceArgs<T> &a, int i, int nc, int nmax, const T xi[3], const T vi[3],
                               const T gdi[3], const T gpi[3], const T gvi[3][3], T hi, T rhoi, T presi, T ci,
-                              T gamma, T rho2_inv_i, T dt, Acc<T> ac[3], Acc<T> &de, const MlpWeightsBF16 *wl,
+                              T gamma, T rho2_inv_i, T dt, BlockAcc<T> ac[3], BlockAcc<T> &de, const MlpWeightsBF16 *wl,
                               MlpWaveLDS *wsc, bool valid)
 {
     typedef typename V4T<T>::type V4;
@@ -1069,6 +1085,12 @@ SPH_DEV void force_gsph_slots(const ForceArgs<T> &a, int i, int nc, int nmax, co
     // gathers do not wait for it
     int j_next = nc > 0 ? a.idx[list_slot(i, 0, a.cap)] : 0;
     for (int q = 0; q < nmax; ++q) {
+        // the sums are compensated in blocks of 16 slots
+        if ((q & 15) == 15) {
+#pragma unroll
+            for (int k = 0; k < D; ++k) ac[k].fold();
+            de.fold();
+        }
         const bool act = q < nc;
         int it = 0;
         int j = 0;
@@ -1258,7 +1280,7 @@ __global__ __launch_bounds__(256) void k_force_gsph(ForceArgs<T> a)
     const T gamma = T(a.gamma);
     const T rho2_inv_i = tdiv(T(1.0), sqr(rhoi));
     const T dt = T(a.dscal[0]);
-    Acc<T> ac[3], de;
+    BlockAcc<T> ac[3], de;
     const int nc = valid ? a.cnt[i] : 0;
     // the wave runs as many pair slots as its longest list
     int nmax = nc;
