@@ -64,6 +64,7 @@ template <typename T> struct PreArgs {
 
 // neighbour loops handle candidates in groups of this many
 constexpr int kGroup = 4;
+static_assert(16 % kGroup == 0, "blocked sums fold every 16 candidates, at group starts");
 
 // entries q0 .. q0+kGroup-1 of particle i's list (entries at or past n
 // repeat entry n - 1); the kGroup loads are issued together
@@ -154,7 +155,7 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
         int it = 0;
         for (; it < 10;) {
             const T hb = h;
-            Acc<T> dn, ddn;
+            BlockAcc<T> dn, ddn;  // compensated in blocks of 16 candidates
             // candidates in groups of kGroup: the group's position and mass
             // loads are issued together, and the next group's indices are
             // loaded one group ahead, so a group waits for memory once; the
@@ -162,6 +163,10 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
             int jn[kGroup];
             if (nc > 0) list_group(a.idx, i, 0, nc, a.cap, jn);
             for (int q0 = 0; q0 < nc; q0 += kGroup) {
+                if ((q0 & 15) == 0) {
+                    dn.fold();
+                    ddn.fold();
+                }
                 int jg[kGroup];
                 V4 pg[kGroup];
                 T wg[kGroup];
@@ -223,7 +228,8 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
     // membership bits of the first 256 candidates, for the force-list rows
     // written after this pass
     unsigned long long mb[4] = {0, 0, 0, 0};
-    Acc<T> dens_a, dhd_a, n_a, pres_a, dhp_a, dhn_a;
+    // compensated in blocks of 16 candidates
+    BlockAcc<T> dens_a, dhd_a, n_a, pres_a, dhp_a, dhn_a;
     T vsig = ci * T(2.0);
     int w = 0;
     // members other than coincident ones start the force pair list: with
@@ -263,6 +269,9 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
     int jn[kGroup];
     if (nc > 0) list_group(a.idx, i, 0, nc, a.cap, jn);
     for (int q0 = 0; q0 < nc; q0 += kGroup) {
+    if ((q0 & 15) == 0) {
+        dens_a.fold(); dhd_a.fold(); n_a.fold(); pres_a.fold(); dhp_a.fold(); dhn_a.fold();
+    }
     // a group of candidates: positions first (the indices were loaded one
     // group ahead), then the records of the members, each set of loads
     // issued together
