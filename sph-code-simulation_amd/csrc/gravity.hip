@@ -122,7 +122,14 @@ template <> struct GravStage<double> {
     double gm[kWave];
 };
 
-template <int D, typename T>
+// minimum-image component for a periodic box, the plain difference otherwise
+// (P is the box's periodic flag, fixed per launch)
+template <bool P, typename T> SPH_DEV T grav_rij(T xi, T xj, const Box &b, int k)
+{
+    return P ? rij_axis<T>(xi, xj, b, k) : xi - xj;
+}
+
+template <int D, typename T, bool P>
 __global__ __launch_bounds__(256) void k_gravity(GravArgs<T> a)
 {
     __shared__ GravStage<T> stage[4];
@@ -169,14 +176,14 @@ __global__ __launch_bounds__(256) void k_gravity(GravArgs<T> a)
         double d2 = 0.0;
 #pragma unroll
         for (int k = 0; k < D; ++k) {
-            d[k] = rij_axis<double>(ri[k], cm[k], a.box, k);
+            d[k] = grav_rij<P, double>(ri[k], cm[k], a.box, k);
             d2 = (k == 0) ? d[k] * d[k] : d2 + d[k] * d[k];
         }
         // BHNode::calc_force opening test (src/bhtree.cpp:350-358)
         const bool open = act && ((g.l2 > a.theta2 * d2) || (d2 < g.l2));
         if (act && !open) {
             // monopole (src/bhtree.cpp:387-392)
-            const T r_inv = T(1.0) / tsqrt(T(d2));
+            const T r_inv = tdiv(T(1.0), tsqrt(T(d2)));
             const T gm = G * T(g.mass);
             phi.add(-(gm * r_inv));
             const T s = gm * (r_inv * r_inv * r_inv);
@@ -213,7 +220,7 @@ __global__ __launch_bounds__(256) void k_gravity(GravArgs<T> a)
                         T r2 = T(0);
 #pragma unroll
                         for (int k = 0; k < D; ++k) {
-                            rij[k] = rij_axis<T>(xi[k], xj[k], a.box, k);
+                            rij[k] = grav_rij<P, T>(xi[k], xj[k], a.box, k);
                             r2 = (k == 0) ? rij[k] * rij[k] : r2 + rij[k] * rij[k];
                         }
                         const T r = tsqrt(r2);
@@ -277,9 +284,11 @@ template <typename T> void launch_gravity(sphgpu_ctx *c)
     a.box = c->box;
     a.stats = c->dstats;
     const dim3 g(grid_for(c->n, 256)), b(256);
-    if (c->dim == 1) hipLaunchKernelGGL((k_gravity<1, T>), g, b, 0, c->stream, a);
-    else if (c->dim == 2) hipLaunchKernelGGL((k_gravity<2, T>), g, b, 0, c->stream, a);
-    else hipLaunchKernelGGL((k_gravity<3, T>), g, b, 0, c->stream, a);
+#define LAUNCH(D)                                                                          \
+    if (c->box.periodic) hipLaunchKernelGGL((k_gravity<D, T, true>), g, b, 0, c->stream, a); \
+    else hipLaunchKernelGGL((k_gravity<D, T, false>), g, b, 0, c->stream, a);
+    if (c->dim == 1) { LAUNCH(1) } else if (c->dim == 2) { LAUNCH(2) } else { LAUNCH(3) }
+#undef LAUNCH
 }
 
 } // namespace
