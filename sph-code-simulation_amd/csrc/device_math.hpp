@@ -46,6 +46,9 @@ template <typename T> SPH_DEV T tabs(T x) { return x < T(0) ? -x : x; }
 // particles, pressures and densities) never are; sqrt(0) = 0 stays exact
 SPH_DEV float tsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 SPH_DEV double tsqrt(double x) { return __builtin_sqrt(x); }
+// 1 / sqrt(x): the hardware estimate in FP32 (about 1 ulp), IEEE in FP64
+SPH_DEV float trsqrt(float x) { return __builtin_amdgcn_rsqf(x); }
+SPH_DEV double trsqrt(double x) { return 1.0 / __builtin_sqrt(x); }
 SPH_DEV float texp(float x) { return __expf(x); }
 SPH_DEV double texp(double x) { return exp(x); }
 
@@ -290,25 +293,41 @@ template <typename T> struct TwoShock {
         T pn = p + tdiv((ustar_r - ustar_l) * (z_l * z_r), z_r - z_l);
         return (T(1e-25) < pn) ? pn : T(1e-25);  // std::max(smallp, p)
     }
-    // step() that also returns its Newton factor F = z_l z_r / (z_r - z_l)
-    // (the FP32 form, which refines the last iterate in FP64 with F; the
-    // update is written p + (u*_R - u*_L) F)
+    // The FP32 form's Newton step, which also returns its Newton factor
+    // F = z_l z_r / (z_r - z_l) (the last iterate is refined in FP64 with F;
+    // the update is written p + (u*_R - u*_L) F).  The same quantities as
+    // step() with three reciprocal-type instructions instead of nine: with
+    // X_K = 1 + gamma1 (p - p_K) / p_K and s_K = 1/sqrt(X_K),
+    // w_K = c_K X_K s_K and 1/w_K = s_K / c_K; with A_K = 4 V_K w_K^3 and
+    // B_K = 4 V_K w_K^2 - gamma2 (p - p_K), z_l = -A_l / B_l and
+    // z_r = A_r / B_r, so F = -A_l A_r / (A_r B_l + A_l B_r).  X_K >=
+    // (gamma - 1) / (2 gamma) > 0 for p >= 0.  prep_f() forms the per-pair
+    // reciprocals first.
+    T ip_l, ip_r, ic_l, ic_r;
+    SPH_DEV void prep_f()
+    {
+        ip_l = tdiv(T(1.0), p_l);
+        ip_r = tdiv(T(1.0), p_r);
+        ic_l = tdiv(T(1.0), cl);
+        ic_r = tdiv(T(1.0), cr);
+    }
     SPH_DEV T step_f(T p, T &F) const
     {
-        T w_l = T(1.0) + tdiv(gamma1 * (p - p_l), p_l);
-        w_l = cl * tsqrt(w_l);
-        T w_r = T(1.0) + tdiv(gamma1 * (p - p_r), p_r);
-        w_r = cr * tsqrt(w_r);
-        T z_l = T(4.0) * V_l * w_l * w_l;
-        z_l = tdiv(-z_l * w_l, z_l - gamma2 * (p - p_l));
-        T z_r = T(4.0) * V_r * w_r * w_r;
-        z_r = tdiv(z_r * w_r, z_r - gamma2 * (p - p_r));
-        const T ustar_l = u_l - tdiv(p - p_l, w_l);
-        const T ustar_r = u_r + tdiv(p - p_r, w_r);
-        F = tdiv(z_l * z_r, z_r - z_l);
+        const T dl = p - p_l, dr = p - p_r;
+        const T X_l = T(1.0) + gamma1 * dl * ip_l;
+        const T X_r = T(1.0) + gamma1 * dr * ip_r;
+        const T s_l = trsqrt(X_l), s_r = trsqrt(X_r);
+        const T w_l = cl * X_l * s_l, w_r = cr * X_r * s_r;
+        const T A_l = T(4.0) * V_l * w_l * w_l * w_l, A_r = T(4.0) * V_r * w_r * w_r * w_r;
+        const T B_l = T(4.0) * V_l * w_l * w_l - gamma2 * dl;
+        const T B_r = T(4.0) * V_r * w_r * w_r - gamma2 * dr;
+        F = -(A_l * A_r) * tdiv(T(1.0), A_r * B_l + A_l * B_r);
+        const T ustar_l = u_l - dl * (s_l * ic_l);
+        const T ustar_r = u_r + dr * (s_r * ic_r);
         const T pn = p + (ustar_r - ustar_l) * F;
         return (T(1e-25) < pn) ? pn : T(1e-25);
     }
+    // the convergence test |dp| / p < 1e-6 as |dp| < 1e-6 p (p > 0)
     SPH_DEV int iterate_f(T &p, T &F) const
     {
         int it = 0;
@@ -316,7 +335,7 @@ template <typename T> struct TwoShock {
             const T p_old = p;
             p = step_f(p, F);
             it = iter + 1;
-            if (tdiv(tabs(p - p_old), p) < T(1e-6)) break;
+            if (tabs(p - p_old) < T(1e-6) * p) break;
         }
         return it;
     }

@@ -1006,6 +1006,7 @@ SPH_DEV int solve_pair(const T l[4], const T r[4], T gamma, int refine64, const 
         // scales the small correction), and v* comes from the same FP64
         // evaluation
         T F = T(0);
+        ts.prep_f();
         if (guess) {
             const T g = *guess;
             bool conv = false;
@@ -1016,7 +1017,7 @@ SPH_DEV int solve_pair(const T l[4], const T r[4], T gamma, int refine64, const 
                     const T p_old = p;
                     p = ts.step_f(p, F);
                     ++it;
-                    if (tdiv(tabs(p - p_old), p) < T(1e-6)) { conv = true; break; }
+                    if (tabs(p - p_old) < T(1e-6) * p) { conv = true; break; }
                 }
             }
             if (!conv) {
