@@ -1091,9 +1091,23 @@ SPH_DEV void force_gsph_slots(const ForceArgs<T> &a, int i, int nc, int nmax, co
     const V4 *__restrict__ RV = a.rv.r[R_VELM];
     const V4 *__restrict__ RT = a.rv.r[R_THERMO];
     long long iters = 0, wave_max_sum = 0, calls = 0, fallbacks = 0, learned = 0;
-    // the next slot's pair index is loaded one slot ahead, so its record
-    // gathers do not wait for it
+    // The next slot's pair index is loaded one slot ahead, so its record
+    // gathers do not wait for it; in the FP32 form the next slot's records
+    // are loaded one slot ahead too (the slot's arithmetic covers their
+    // latency), and the index two slots ahead.
+    constexpr bool kAhead = sizeof(T) == 4;
     int j_next = nc > 0 ? a.idx[list_slot(i, 0, a.cap)] : 0;
+    V4 n_p{}, n_v{}, n_t{}, n_g[4] = {};
+    int j_after = 0;
+    if (kAhead && nc > 0) {
+        n_p = RP[j_next];
+        n_v = RV[j_next];
+        n_t = RT[j_next];
+        if (a.gsph2)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) n_g[b] = a.rv.r[R_G0 + b][j_next];
+        if (nc > 1) j_after = a.idx[list_slot(i, 1, a.cap)];
+    }
     for (int q = 0; q < nmax; ++q) {
         // the sums are compensated in blocks of 16 slots
         if ((q & 15) == 15) {
@@ -1105,13 +1119,32 @@ SPH_DEV void force_gsph_slots(const ForceArgs<T> &a, int i, int nc, int nmax, co
         int it = 0;
         int j = 0;
         const int j_cur = j_next;
-        if (q + 1 < nc) j_next = a.idx[list_slot(i, q + 1, a.cap)];
+        V4 c_p{}, c_v{}, c_t{}, c_g[4] = {};
+        if (kAhead) {
+            c_p = n_p;
+            c_v = n_v;
+            c_t = n_t;
+#pragma unroll
+            for (int b = 0; b < 4; ++b) c_g[b] = n_g[b];
+            if (q + 1 < nc) {
+                j_next = j_after;
+                n_p = RP[j_next];
+                n_v = RV[j_next];
+                n_t = RT[j_next];
+                if (a.gsph2)
+#pragma unroll
+                    for (int b = 0; b < 4; ++b) n_g[b] = a.rv.r[R_G0 + b][j_next];
+                if (q + 2 < nc) j_after = a.idx[list_slot(i, q + 2, a.cap)];
+            }
+        } else if (q + 1 < nc) {
+            j_next = a.idx[list_slot(i, q + 1, a.cap)];
+        }
         T d[3] = {0, 0, 0}, e[3] = {0, 0, 0}, r = T(1), r_inv = T(1);
         T left[4] = {0, 1, 1, 1}, right[4] = {0, 1, 1, 1};
         T rhoj = T(1), hj = T(1), mj = T(0);
         if (act) {
             j = j_cur;
-            const V4 pj = RP[j], vjr = RV[j], tj = RT[j];
+            const V4 pj = kAhead ? c_p : RP[j], vjr = kAhead ? c_v : RV[j], tj = kAhead ? c_t : RT[j];
             r = tsqrt(rec_r2<D, T>(pj, xi, a.box, d));
             r_inv = tdiv(T(1.0), r);
             const T vj[3] = {vjr.x, vjr.y, vjr.z};
@@ -1128,7 +1161,8 @@ SPH_DEV void force_gsph_slots(const ForceArgs<T> &a, int i, int nc, int nmax, co
             const T presj = tj.y, cj = tj.z;
             if (a.gsph2) {
                 // neighbour gradients: (gd, gp, gv) packed in four records
-                const V4 g0 = a.rv.r[R_G0][j], g1 = a.rv.r[R_G1][j], g2 = a.rv.r[R_G2][j], g3 = a.rv.r[R_G3][j];
+                const V4 g0 = kAhead ? c_g[0] : a.rv.r[R_G0][j], g1 = kAhead ? c_g[1] : a.rv.r[R_G1][j],
+                         g2 = kAhead ? c_g[2] : a.rv.r[R_G2][j], g3 = kAhead ? c_g[3] : a.rv.r[R_G3][j];
                 const T gj[16] = {g0.x, g0.y, g0.z, g0.w, g1.x, g1.y, g1.z, g1.w,
                                   g2.x, g2.y, g2.z, g2.w, g3.x, g3.y, g3.z, g3.w};
                 // MUSCL reconstruction, g_fluid_force.cpp:103-140
