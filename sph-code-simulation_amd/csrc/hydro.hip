@@ -160,24 +160,51 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
             // loads are issued together, and the next group's indices are
             // loaded one group ahead, so a group waits for memory once; the
             // sums run in candidate order
+            // (in the FP32 form the next group's positions and masses are
+            // loaded one group ahead as well, its indices two ahead)
+            constexpr bool kAhead = sizeof(T) == 4;
+            constexpr bool kMassW = !(TYPE == SPHGPU_DISPH || TYPE == SPHGPU_GDISPH);
             int jn[kGroup];
+            V4 pn[kGroup];
+            T wn[kGroup];
             if (nc > 0) list_group(a.idx, i, 0, nc, a.cap, jn);
+            if (kAhead && nc > 0) {
+#pragma unroll
+                for (int u = 0; u < kGroup; ++u) pn[u] = RP[jn[u]];
+#pragma unroll
+                for (int u = 0; u < kGroup; ++u) wn[u] = kMassW ? rec_mass(RV, jn[u]) : T(1);
+                if (kGroup < nc) list_group(a.idx, i, kGroup, nc, a.cap, jn);
+            }
             for (int q0 = 0; q0 < nc; q0 += kGroup) {
                 if ((q0 & 15) == 0) {
                     dn.fold();
                     ddn.fold();
                 }
-                int jg[kGroup];
                 V4 pg[kGroup];
                 T wg[kGroup];
+                if (kAhead) {
 #pragma unroll
-                for (int u = 0; u < kGroup; ++u) jg[u] = jn[u];
+                    for (int u = 0; u < kGroup; ++u) {
+                        pg[u] = pn[u];
+                        wg[u] = wn[u];
+                    }
+                    if (q0 + kGroup < nc) {
 #pragma unroll
-                for (int u = 0; u < kGroup; ++u) pg[u] = RP[jg[u]];
-                if (q0 + kGroup < nc) list_group(a.idx, i, q0 + kGroup, nc, a.cap, jn);
+                        for (int u = 0; u < kGroup; ++u) pn[u] = RP[jn[u]];
 #pragma unroll
-                for (int u = 0; u < kGroup; ++u)
-                    wg[u] = (TYPE == SPHGPU_DISPH || TYPE == SPHGPU_GDISPH) ? T(1) : rec_mass(RV, jg[u]);
+                        for (int u = 0; u < kGroup; ++u) wn[u] = kMassW ? rec_mass(RV, jn[u]) : T(1);
+                        if (q0 + 2 * kGroup < nc) list_group(a.idx, i, q0 + 2 * kGroup, nc, a.cap, jn);
+                    }
+                } else {
+                    int jg[kGroup];
+#pragma unroll
+                    for (int u = 0; u < kGroup; ++u) jg[u] = jn[u];
+#pragma unroll
+                    for (int u = 0; u < kGroup; ++u) pg[u] = RP[jg[u]];
+                    if (q0 + kGroup < nc) list_group(a.idx, i, q0 + kGroup, nc, a.cap, jn);
+#pragma unroll
+                    for (int u = 0; u < kGroup; ++u) wg[u] = kMassW ? rec_mass(RV, jg[u]) : T(1);
+                }
 #pragma unroll
                 for (int u = 0; u < kGroup; ++u) {
                     T r2;
