@@ -293,8 +293,20 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
     const bool bal_on = kBal && (do_bal || a.td_av);
     const bool disph_w = (TYPE == SPHGPU_DISPH) || (TYPE == SPHGPU_GDISPH && !do_bal);
     T bdiv = T(0), brot[3] = {0, 0, 0};
-    int jn[kGroup];
+    // (FP32 form: the next group's positions are loaded one group ahead and
+    // its indices two ahead)
+    constexpr bool kAheadP = sizeof(T) == 4;
+    int jn[kGroup], jc[kGroup];
+    V4 pn[kGroup];
     if (nc > 0) list_group(a.idx, i, 0, nc, a.cap, jn);
+    if (kAheadP && nc > 0) {
+#pragma unroll
+        for (int u = 0; u < kGroup; ++u) {
+            jc[u] = jn[u];
+            pn[u] = RP[jn[u]];
+        }
+        if (kGroup < nc) list_group(a.idx, i, kGroup, nc, a.cap, jn);
+    }
     for (int q0 = 0; q0 < nc; q0 += kGroup) {
     if ((q0 & 15) == 0) {
         dens_a.fold(); dhd_a.fold(); n_a.fold(); pres_a.fold(); dhp_a.fold(); dhn_a.fold();
@@ -306,11 +318,27 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
     V4 pg[kGroup], vg[kGroup];
     T cg[kGroup], ug[kGroup];
     bool memb[kGroup];
+    if (kAheadP) {
 #pragma unroll
-    for (int u = 0; u < kGroup; ++u) jg[u] = jn[u];
+        for (int u = 0; u < kGroup; ++u) {
+            jg[u] = jc[u];
+            pg[u] = pn[u];
+        }
+        if (q0 + kGroup < nc) {
 #pragma unroll
-    for (int u = 0; u < kGroup; ++u) pg[u] = RP[jg[u]];
-    if (q0 + kGroup < nc) list_group(a.idx, i, q0 + kGroup, nc, a.cap, jn);
+            for (int u = 0; u < kGroup; ++u) {
+                jc[u] = jn[u];
+                pn[u] = RP[jn[u]];
+            }
+            if (q0 + 2 * kGroup < nc) list_group(a.idx, i, q0 + 2 * kGroup, nc, a.cap, jn);
+        }
+    } else {
+#pragma unroll
+        for (int u = 0; u < kGroup; ++u) jg[u] = jn[u];
+#pragma unroll
+        for (int u = 0; u < kGroup; ++u) pg[u] = RP[jg[u]];
+        if (q0 + kGroup < nc) list_group(a.idx, i, q0 + kGroup, nc, a.cap, jn);
+    }
 #pragma unroll
     for (int u = 0; u < kGroup; ++u) {
         T d[3];
@@ -907,8 +935,20 @@ __global__ __launch_bounds__(256) void k_force_sph(ForceArgs<T> a)
     // compensated sums in blocks of 16 pairs
     BlockAcc<T> ac[3], de;
     const int nc = a.cnt[i];
-    // the next pair index is loaded one pair ahead
+    // the next pair index is loaded one pair ahead; in the FP32 form the
+    // next pair's four records are loaded one pair ahead too and the index
+    // two ahead
+    constexpr bool kAhead = sizeof(T) == 4;
     int j_next = nc > 0 ? a.idx[list_slot(i, 0, a.cap)] : 0;
+    V4 n_p{}, n_v{}, n_t{}, n_a{};
+    int j_after = 0;
+    if (kAhead && nc > 0) {
+        n_p = RP[j_next];
+        n_v = RV[j_next];
+        n_t = RT[j_next];
+        n_a = RA[j_next];
+        if (nc > 1) j_after = a.idx[list_slot(i, 1, a.cap)];
+    }
     for (int q = 0; q < nc; ++q) {
         if ((q & 15) == 15) {
 #pragma unroll
@@ -916,8 +956,27 @@ __global__ __launch_bounds__(256) void k_force_sph(ForceArgs<T> a)
             de.fold();
         }
         const int j = j_next;
-        if (q + 1 < nc) j_next = a.idx[list_slot(i, q + 1, a.cap)];
-        const V4 pj = RP[j], vj = RV[j], tj = RT[j], aj = RA[j];
+        V4 pj, vj, tj, aj;
+        if (kAhead) {
+            pj = n_p;
+            vj = n_v;
+            tj = n_t;
+            aj = n_a;
+            if (q + 1 < nc) {
+                j_next = j_after;
+                n_p = RP[j_next];
+                n_v = RV[j_next];
+                n_t = RT[j_next];
+                n_a = RA[j_next];
+                if (q + 2 < nc) j_after = a.idx[list_slot(i, q + 2, a.cap)];
+            }
+        } else {
+            if (q + 1 < nc) j_next = a.idx[list_slot(i, q + 1, a.cap)];
+            pj = RP[j];
+            vj = RV[j];
+            tj = RT[j];
+            aj = RA[j];
+        }
         T d[3];
         const T r = tsqrt(rec_r2<D, T>(pj, xi, a.box, d));
         // GDISPH: HLL p* of the pair (right = i, left = j)
