@@ -253,9 +253,9 @@ __global__ __launch_bounds__(kSingleThreads) void k_scan_single(const unsigned *
     __shared__ unsigned wsum[kSingleThreads / kWave];
     const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
     unsigned carry = 0;
-    for (int t0 = 0; t0 < n; t0 += kSingleTile) {
+    // tile t + 1 is loaded while tile t is scanned
+    auto load_tile = [&](int t0, unsigned v[kSingleItems]) {
         const int b = t0 + threadIdx.x * kSingleItems;
-        unsigned v[kSingleItems];
         if (b + kSingleItems <= n) {
             const uint4 a0 = *reinterpret_cast<const uint4 *>(in + b);
             const uint4 a1 = *reinterpret_cast<const uint4 *>(in + b + 4);
@@ -265,6 +265,15 @@ __global__ __launch_bounds__(kSingleThreads) void k_scan_single(const unsigned *
 #pragma unroll
             for (int k = 0; k < kSingleItems; ++k) v[k] = b + k < n ? in[b + k] : 0u;
         }
+    };
+    unsigned vn[kSingleItems];
+    if (n > 0) load_tile(0, vn);
+    for (int t0 = 0; t0 < n; t0 += kSingleTile) {
+        const int b = t0 + threadIdx.x * kSingleItems;
+        unsigned v[kSingleItems];
+#pragma unroll
+        for (int k = 0; k < kSingleItems; ++k) v[k] = vn[k];
+        if (t0 + kSingleTile < n) load_tile(t0 + kSingleTile, vn);
         unsigned s = 0;
 #pragma unroll
         for (int k = 0; k < kSingleItems; ++k) s += v[k];
