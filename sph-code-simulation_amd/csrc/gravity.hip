@@ -148,6 +148,17 @@ __global__ __launch_bounds__(256) void k_gravity(GravArgs<T> a)
     const V4 pi_ = RP[i];
     const T xi[3] = {pi_.x, pi_.y, pi_.z};
     const T hi = pi_.w;
+    // half-box ties of the FP32 minimum image: the FP32 positions and their
+    // difference carry at most 4 u X of rounding (X the largest coordinate
+    // magnitude); 2^-18 (X + L) covers it
+    float tie_band[3] = {0.f, 0.f, 0.f};
+    if constexpr (P) {
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            const double X = fmax(fabs(a.box.lo[k]), fabs(a.box.hi[k]));
+            tie_band[k] = (float)((X + a.box.range[k]) * 3.814697265625e-06);
+        }
+    }
     const T inv_ei = T(2) / hi;
     const T G = T(a.G);
     // compensated sums in blocks of at most 16 terms (fold below)
@@ -219,10 +230,23 @@ __global__ __launch_bounds__(256) void k_gravity(GravArgs<T> a)
                         T rij[3] = {0, 0, 0};
                         T r2 = T(0);
 #pragma unroll
-                        for (int k = 0; k < D; ++k) {
-                            rij[k] = grav_rij<P, T>(xi[k], xj[k], a.box, k);
-                            r2 = (k == 0) ? rij[k] * rij[k] : r2 + rij[k] * rij[k];
+                        for (int k = 0; k < D; ++k) rij[k] = grav_rij<P, T>(xi[k], xj[k], a.box, k);
+                        if constexpr (P && sizeof(T) == 4) {
+                            // a component within FP32 rounding of half the
+                            // box takes its image from the FP64 positions, as
+                            // the reference chooses it (the image, and with
+                            // it the sign, flips there)
+                            bool tie = false;
+#pragma unroll
+                            for (int k = 0; k < D; ++k)
+                                tie = tie || fabsf(fabsf(xi[k] - xj[k]) - 0.5f * a.box.rangef[k]) <= tie_band[k];
+                            if (tie)
+#pragma unroll
+                                for (int k = 0; k < D; ++k)
+                                    rij[k] = (T)rij_axis<double>(ri[k], a.x64[k][s0 + t], a.box, k);
                         }
+#pragma unroll
+                        for (int k = 0; k < D; ++k) r2 = (k == 0) ? rij[k] * rij[k] : r2 + rij[k] * rij[k];
                         const T r = tsqrt(r2);
                         const T gm = st.gm[t];
                         T fsum, gsum;

@@ -15,6 +15,14 @@ def _gdisph(builder, second_order):
     return make
 
 
+def _with(make, **kw):
+    def f():
+        p, fl = make()
+        p.update(kw)
+        return p, fl
+    return f
+
+
 SMALL = {
     "sod_1d": lambda: ics.sod_1d(),
     "khi_2d": lambda: ics.khi_2d(nx=32),
@@ -25,6 +33,10 @@ SMALL = {
     # its GDISPH runs first order)
     "khi_2d_gdisph": _gdisph(lambda: ics.khi_2d(nx=32), True),
     "sedov_3d_gdisph": _gdisph(lambda: ics.sedov_3d(n1=16), False),
+    # tree gravity in a periodic box: the walk takes minimum-image distances
+    # to node centres of mass and leaf members (src/bhtree.cpp:347-393 with
+    # Periodic::calc_r_ij)
+    "sedov_3d_gravity_periodic": _with(lambda: ics.sedov_3d(n1=12), use_gravity=1),
 }
 
 

@@ -161,6 +161,22 @@ def test_reference_strong_shock_preset_goes_nan(solver):
     assert first_bad == (85 if solver == "hll" else 84)
 
 
+def test_periodic_gravity_bit_exact():
+    """Tree gravity in a periodic box (the 12^3 Sedov with self-gravity):
+    node and leaf distances through Periodic::calc_r_ij
+    (src/bhtree.cpp:347-393), oracle equal to the reference bit for bit over
+    initialisation and 3 steps."""
+    from parity_util import SMALL
+    p, f = SMALL["sedov_3d_gravity_periodic"]()
+    o, r = _both(p, f)
+    o.initialize()
+    r.initialize()
+    _same(o, r, "init")
+    o.integrate(3)
+    r.integrate(3)
+    _same(o, r, "3 steps")
+
+
 def _gdisph_case(name):
     from parity_util import SMALL
     if name in SMALL:
