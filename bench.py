@@ -287,6 +287,26 @@ def kernel_models(res: dict, steps: int, workload: str, form: str) -> dict:
     return out
 
 
+# compulsory HBM bytes per particle-step of the whole step (each particle's
+# fields moved once per pass: time step, predict, tree keys, sort and
+# permutation, nodes, PreInteraction, FluidForce, correct), the upper end of
+# the estimate 0.94-1.4 kB
+STEP_COMPULSORY_BYTES = 1400.0
+
+
+def path_roofline(kernels: dict, res: dict, steps: int, form: str) -> dict:
+    """Whole-step bound: the priced pair kernels at the vector peak plus the
+    step's compulsory HBM bytes at the HBM peak, over the measured step."""
+    peak = PEAK_FP32_TFLOPS if form != "fp64" else PEAK_FP64_TFLOPS
+    t_valu = sum(e["flop_per_launch"] * e["launches_per_step"] for e in kernels.values()) / (peak * 1e12)
+    t_hbm = STEP_COMPULSORY_BYTES * res["n"] / (PEAK_HBM_GBS * 1e9)
+    t_meas = res["wall"] / steps
+    return {"t_bound_ms": round((t_valu + t_hbm) * 1e3, 4), "t_measured_ms": round(t_meas * 1e3, 4),
+            "frac": round((t_valu + t_hbm) / t_meas, 4),
+            "note": ("T_bound = sum over the pair kernels of algorithmic FLOP / vector peak + %d B per "
+                     "particle-step of compulsory HBM traffic / 8 TB/s" % int(STEP_COMPULSORY_BYTES))}
+
+
 def roofline(kernels: dict) -> dict:
     """The dominant priced kernel (largest time per step)."""
     dom = max(kernels, key=lambda k: kernels[k]["avg_ms"] * kernels[k]["launches_per_step"])
@@ -403,6 +423,7 @@ def main() -> None:
                    "p_background": ics.SEDOV_P_BACKGROUND, "p_background_reason": P_BACKGROUND_NOTE},
         "roofline": rl,
         "kernels": kernels,
+        "path_roofline": path_roofline(kernels, res, args.steps, args.form),
     }
     st = res["stats"]
     # tree searches per step: the pre-interaction's, plus the FluidForce one
@@ -462,6 +483,16 @@ def main() -> None:
                      "above the mean sound speed): on this workload a few per mille of the pairs, which do not "
                      "set the wavefront's largest Newton count; its gain shows on the hard pairs themselves "
                      "(training.held_out) and not in the step time")}
+    if not args.no_fp64 and args.workload == "sedov_3d_gsph_1m":
+        # the 64^3 Sedov (262,144 particles), plain guess: FP32 against FP64
+        c3 = {}
+        for form in ("fp32", "fp64"):
+            r = run_form("sedov_3d_gsph_262k", form, args.steps, args.warmup, kernel_table=False)
+            c3[form] = {"value": round(r["value"], 1), "ms_per_step": round(r["wall"] / args.steps * 1e3, 4),
+                        "energy_drift": r["drift"]}
+        c3["ratio"] = round(c3["fp32"]["value"] / c3["fp64"]["value"], 3)
+        c3["particles"] = 262144
+        out["sedov_262k"] = c3
     if not args.no_evrard and args.workload != "evrard_3d_disph_1m":
         out["evrard_1m"] = evrard_entry(args.steps, args.warmup, not args.no_kernel_table)
         if not args.no_evrard_run:
