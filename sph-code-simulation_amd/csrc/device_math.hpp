@@ -222,6 +222,29 @@ template <int D, typename T> struct Kern<D, SPHGPU_GAUSSIAN, T> {
     }
 };
 
+// dwc for the FP32 pair loops with the reciprocals 1/r and 1/h supplied by
+// the caller (1/h_i is formed once per particle, 1/r once per pair): the same
+// expressions as Kern::dwc with their divisions written as products.
+template <int D, int K> SPH_DEV float dwc_rcp(float r, float r_inv, float ih)
+{
+    if (K == SPHGPU_CUBIC_SPLINE) {
+        if (r == 0.f) return 0.f;
+        const float ih_ = 2.0f * ih;  // 1 / (h / 2)
+        const float q = r * ih_;
+        const float c = float(-Sigma<D>::cubic) * powh<D>(ih_) * ih_ * r_inv;
+        return c * (0.75f * sqr(pos_part(2.0f - q)) - 3.0f * sqr(pos_part(1.0f - q)));
+    } else if (K == SPHGPU_WENDLAND) {
+        const float q = r * ih;
+        const float c = float(-56.0 / 3.0 * Sigma<D>::c4) * powh<D>(ih) * sqr(ih);
+        return c * pow5(pos_part(1.0f - q)) * (1.0f + 5.0f * q);
+    } else {
+        if (r == 0.f) return 0.f;
+        const float q = r * ih;
+        const float w_val = float(Sigma<D>::gauss) * powh<D>(ih) * texp(-(q * q));
+        return -2.0f * w_val * q * ih * r_inv;
+    }
+}
+
 // van Leer limiter, src/gsph/g_fluid_force.cpp:33-41
 template <typename T> SPH_DEV T limiter(T dq1, T dq2)
 {
@@ -296,34 +319,37 @@ template <typename T> struct TwoShock {
     // The FP32 form's Newton step, which also returns its Newton factor
     // F = z_l z_r / (z_r - z_l) (the last iterate is refined in FP64 with F;
     // the update is written p + (u*_R - u*_L) F).  The same quantities as
-    // step() with three reciprocal-type instructions instead of nine: with
-    // X_K = 1 + gamma1 (p - p_K) / p_K and s_K = 1/sqrt(X_K),
-    // w_K = c_K X_K s_K and 1/w_K = s_K / c_K; with A_K = 4 V_K w_K^3 and
-    // B_K = 4 V_K w_K^2 - gamma2 (p - p_K), z_l = -A_l / B_l and
-    // z_r = A_r / B_r, so F = -A_l A_r / (A_r B_l + A_l B_r).  X_K >=
-    // (gamma - 1) / (2 gamma) > 0 for p >= 0.  prep_f() forms the per-pair
-    // reciprocals first.
-    T ip_l, ip_r, ic_l, ic_r;
-    SPH_DEV void prep_f()
+    // step() with three reciprocal-type instructions instead of nine and no
+    // per-pair reciprocals: w_K^2 = c_K^2 (1 + gamma1 (p - p_K) / p_K) =
+    // Y_K = gamma rho_K (p_K + gamma1 (p - p_K)), so with s_K = 1/sqrt(Y_K),
+    // 1/w_K = s_K and w_K = Y_K s_K.  With A_K = 4 V_K w_K^3 and
+    // B_K = 4 V_K w_K^2 - gamma2 (p - p_K), z_l = -A_l / B_l, z_r = A_r / B_r
+    // and F = -A_l A_r / (A_r B_l + A_l B_r), which is unchanged when A_K and
+    // B_K are both scaled by rho_K: rho_K A_K = 4 Y_K w_K and rho_K B_K =
+    // 4 Y_K - gamma2 rho_K (p - p_K).  Y_K > 0 for p >= 0 (p_K + gamma1
+    // (p - p_K) = (1 - gamma1) p_K + gamma1 p with gamma1 < 1).  prep_f()
+    // forms the per-pair products first.
+    T gr_l, gr_r, g2r_l, g2r_r;
+    SPH_DEV void prep_f(T gamma, T rho_l, T rho_r)
     {
-        ip_l = tdiv(T(1.0), p_l);
-        ip_r = tdiv(T(1.0), p_r);
-        ic_l = tdiv(T(1.0), cl);
-        ic_r = tdiv(T(1.0), cr);
+        gr_l = gamma * rho_l;
+        gr_r = gamma * rho_r;
+        g2r_l = gamma2 * rho_l;
+        g2r_r = gamma2 * rho_r;
     }
     SPH_DEV T step_f(T p, T &F) const
     {
         const T dl = p - p_l, dr = p - p_r;
-        const T X_l = T(1.0) + gamma1 * dl * ip_l;
-        const T X_r = T(1.0) + gamma1 * dr * ip_r;
-        const T s_l = trsqrt(X_l), s_r = trsqrt(X_r);
-        const T w_l = cl * X_l * s_l, w_r = cr * X_r * s_r;
-        const T A_l = T(4.0) * V_l * w_l * w_l * w_l, A_r = T(4.0) * V_r * w_r * w_r * w_r;
-        const T B_l = T(4.0) * V_l * w_l * w_l - gamma2 * dl;
-        const T B_r = T(4.0) * V_r * w_r * w_r - gamma2 * dr;
+        const T Y_l = gr_l * (p_l + gamma1 * dl);
+        const T Y_r = gr_r * (p_r + gamma1 * dr);
+        const T s_l = trsqrt(Y_l), s_r = trsqrt(Y_r);
+        const T w_l = Y_l * s_l, w_r = Y_r * s_r;
+        const T A_l = T(4.0) * Y_l * w_l, A_r = T(4.0) * Y_r * w_r;
+        const T B_l = T(4.0) * Y_l - g2r_l * dl;
+        const T B_r = T(4.0) * Y_r - g2r_r * dr;
         F = -(A_l * A_r) * tdiv(T(1.0), A_r * B_l + A_l * B_r);
-        const T ustar_l = u_l - dl * (s_l * ic_l);
-        const T ustar_r = u_r + dr * (s_r * ic_r);
+        const T ustar_l = u_l - dl * s_l;
+        const T ustar_r = u_r + dr * s_r;
         const T pn = p + (ustar_r - ustar_l) * F;
         return (T(1e-25) < pn) ? pn : T(1e-25);
     }

@@ -919,6 +919,7 @@ __global__ __launch_bounds__(256) void k_force_sph(ForceArgs<T> a)
     const T g2u_i = sqr(gm1) * ui;
     const T g2upp_i = tdiv(g2u_i, presi);
     const T m_u_inv = tdiv(T(1.0), mi * ui);
+    const T ihi = tdiv(T(1.0), hi);
     // GDISPH: the particle's own MUSCL gradients (gd, gp, gv records)
     T gi[16];
 #pragma unroll
@@ -997,8 +998,16 @@ __global__ __launch_bounds__(256) void k_force_sph(ForceArgs<T> a)
             hll_solver<T>(left, right, pstar, vstar);
         }
         const T hj = pj.w, rhoj = tj.x, presj = tj.y, mj = vj.w, uj = tj.w;
-        const T cwi = Kern<D, K, T>::dwc(r, hi);
-        const T cwj = Kern<D, K, T>::dwc(r, hj);
+        T cwi, cwj;
+        if constexpr (sizeof(T) == 4) {
+            // the Wendland gradient needs no 1/r
+            const T r_inv = (K == SPHGPU_WENDLAND) ? T(0) : tdiv(T(1.0), r);
+            cwi = dwc_rcp<D, K>(r, r_inv, ihi);
+            cwj = dwc_rcp<D, K>(r, r_inv, tdiv(T(1.0), hj));
+        } else {
+            cwi = Kern<D, K, T>::dwc(r, hi);
+            cwj = Kern<D, K, T>::dwc(r, hj);
+        }
         const T vjv[3] = {vj.x, vj.y, vj.z};
         T dwi[3], dwj[3], dwij[3], vij[3];
 #pragma unroll
@@ -1092,7 +1101,7 @@ SPH_DEV int solve_pair(const T l[4], const T r[4], T gamma, int refine64, const 
         // scales the small correction), and v* comes from the same FP64
         // evaluation
         T F = T(0);
-        ts.prep_f();
+        ts.prep_f(gamma, l[1], r[1]);
         if (guess) {
             const T g = *guess;
             bool conv = false;
@@ -1182,6 +1191,7 @@ SPH_DEV void force_gsph_slots(const ForceArgs<T> &a, int i, int nc, int nmax, co
     // are loaded one slot ahead too (the slot's arithmetic covers their
     // latency), and the index two slots ahead.
     constexpr bool kAhead = sizeof(T) == 4;
+    const T ihi = tdiv(T(1.0), hi);
     int j_next = nc > 0 ? a.idx[list_slot(i, 0, a.cap)] : 0;
     V4 n_p{}, n_v{}, n_t{}, n_g[4] = {};
     int j_after = 0;
@@ -1231,8 +1241,16 @@ SPH_DEV void force_gsph_slots(const ForceArgs<T> &a, int i, int nc, int nmax, co
         if (act) {
             j = j_cur;
             const V4 pj = kAhead ? c_p : RP[j], vjr = kAhead ? c_v : RV[j], tj = kAhead ? c_t : RT[j];
-            r = tsqrt(rec_r2<D, T>(pj, xi, a.box, d));
-            r_inv = tdiv(T(1.0), r);
+            const T r2 = rec_r2<D, T>(pj, xi, a.box, d);
+            if constexpr (sizeof(T) == 4) {
+                // 1/r from one reciprocal square root, r = r^2 / r (list
+                // pairs have r > 0)
+                r_inv = trsqrt(r2);
+                r = r2 > T(0) ? r2 * r_inv : T(0);
+            } else {
+                r = tsqrt(r2);
+                r_inv = tdiv(T(1.0), r);
+            }
             const T vj[3] = {vjr.x, vjr.y, vjr.z};
             T ve_i = T(0), ve_j = T(0);
 #pragma unroll
@@ -1344,8 +1362,14 @@ SPH_DEV void force_gsph_slots(const ForceArgs<T> &a, int i, int nc, int nmax, co
                 }
             }
             // Godunov force, g_fluid_force.cpp:158-165
-            const T cwi = Kern<D, K, T>::dwc(r, hi);
-            const T cwj = Kern<D, K, T>::dwc(r, hj);
+            T cwi, cwj;
+            if constexpr (sizeof(T) == 4) {
+                cwi = dwc_rcp<D, K>(r, r_inv, ihi);
+                cwj = dwc_rcp<D, K>(r, r_inv, tdiv(T(1.0), hj));
+            } else {
+                cwi = Kern<D, K, T>::dwc(r, hi);
+                cwj = Kern<D, K, T>::dwc(r, hj);
+            }
             const T rho2_inv_j = tdiv(T(1.0), sqr(rhoj));
             const T s1 = mj * pstar * rho2_inv_i;
             const T s2 = mj * pstar * rho2_inv_j;
