@@ -145,29 +145,31 @@ SPH_DEV void test_range(const float4 *srec, int t0, int t1, const float xf[3], c
 
 // The same test for the radius search (is_ij = 0) on candidates stored
 // relative to the group centre g: record (y, |y|^2) with y = x_j - g (its
-// image nearest g), so that r^2 = |y_i|^2 + |y_j|^2 - 2 y_i . y_j is one add
-// and D fused multiply-adds (m2y = -2 y_i, yyi = |y_i|^2).  The candidates
-// run from last to first and each decision is shifted in at bit 0 with a
-// carry-in add (v_addc_co_u32 d, d + d + vcc), one instruction per mask; km
-// gets r^2 < R2lo, kle gets !(r^2 > R2hi), the band is kle & ~km.
+// image nearest g).  r^2 = |y_i|^2 + |y_j|^2 - 2 y_i . y_j, and the lane's
+// |y_i|^2 moves to the bounds: s = |y_j|^2 - 2 y_i . y_j is D fused
+// multiply-adds (m2y = -2 y_i), tested against Tlo = R2lo - |y_i|^2 and
+// Thi = R2hi - |y_i|^2 (formed once per piece).  The candidates run from
+// last to first and each decision is shifted in at bit 0 with a carry-in add
+// (v_addc_co_u32 d, d + d + vcc), one instruction per mask; km gets
+// s < Tlo, kle gets !(s > Thi), the band is kle & ~km.
 template <int D>
-SPH_DEV void test_range_expand(const float4 *srec, int t0, int t1, const float m2y[3], float yyi, float R2lo,
-                               float R2hi, unsigned &km, unsigned &ku)
+SPH_DEV void test_range_expand(const float4 *srec, int t0, int t1, const float m2y[3], float Tlo, float Thi,
+                               unsigned &km, unsigned &ku)
 {
     unsigned kk = 0, kl = 0;
 #pragma unroll 4
     for (int t = t1 - 1; t >= t0; --t) {
         const float4 c = srec[t];
-        float r2f = yyi + c.w;
-        if (D > 2) r2f = __builtin_fmaf(m2y[2], c.z, r2f);
-        if (D > 1) r2f = __builtin_fmaf(m2y[1], c.y, r2f);
-        r2f = __builtin_fmaf(m2y[0], c.x, r2f);
-        asm("v_cmp_gt_f32 vcc, %[lo], %[r2]\n\t"
+        float sf = c.w;
+        if (D > 2) sf = __builtin_fmaf(m2y[2], c.z, sf);
+        if (D > 1) sf = __builtin_fmaf(m2y[1], c.y, sf);
+        sf = __builtin_fmaf(m2y[0], c.x, sf);
+        asm("v_cmp_gt_f32 vcc, %[lo], %[s]\n\t"
             "v_addc_co_u32 %[kk], vcc, %[kk], %[kk], vcc\n\t"
-            "v_cmp_nlt_f32 vcc, %[hi], %[r2]\n\t"
+            "v_cmp_nlt_f32 vcc, %[hi], %[s]\n\t"
             "v_addc_co_u32 %[kl], vcc, %[kl], %[kl], vcc"
             : [kk] "+v"(kk), [kl] "+v"(kl)
-            : [r2] "v"(r2f), [lo] "v"(R2lo), [hi] "v"(R2hi)
+            : [s] "v"(sf), [lo] "v"(Tlo), [hi] "v"(Thi)
             : "vcc");
     }
     km = kk;
@@ -236,6 +238,8 @@ __global__ __launch_bounds__(256, 4) void k_search(SearchArgs a)
     float m2y[3] = {0, 0, 0}, yyi = 0.f;
     // this piece's band bounds on r^2 (direct or relative form)
     float R2lo = 0.f, R2hi = 0.f;
+    // and the relative form's bounds on r^2 - |y_i|^2
+    float Tlo = 0.f, Thi = 0.f;
 
     // the FP64 predicate (the reference's) for candidate j; the particle's
     // own FP64 position and radius are reloaded here (a rare path) rather
@@ -294,8 +298,8 @@ __global__ __launch_bounds__(256, 4) void k_search(SearchArgs a)
         unsigned km0 = 0, km1 = 0, ku0 = 0, ku1 = 0;
         const int n0 = min(nb, 32);
         if (expand) {
-            test_range_expand<D>(srec, 0, n0, m2y, yyi, R2lo, R2hi, km0, ku0);
-            if (nb > 32) test_range_expand<D>(srec, 32, nb, m2y, yyi, R2lo, R2hi, km1, ku1);
+            test_range_expand<D>(srec, 0, n0, m2y, Tlo, Thi, km0, ku0);
+            if (nb > 32) test_range_expand<D>(srec, 32, nb, m2y, Tlo, Thi, km1, ku1);
         } else if (!a.box.periodic || shift) {
             test_range<D, IS_IJ, true>(srec, 0, n0, xf, Lf, Rf, R2lo, R2hi, delta, z2, km0, ku0);
             if (nb > 32) test_range<D, IS_IJ, true>(srec, 32, nb, xf, Lf, Rf, R2lo, R2hi, delta, z2, km1, ku1);
@@ -419,9 +423,16 @@ __global__ __launch_bounds__(256, 4) void k_search(SearchArgs a)
             const float de = delta + 8.0f * 5.9604644775390625e-08f * (ext * ext) / (Rf * Rf);
             R2lo = Rf * Rf * (1.0f - de);
             R2hi = Rf * Rf * (1.0f + de);
+            // the bounds on s = r^2 - |y_i|^2: their rounding (u (R^2 +
+            // |y_i|^2)) and the FMA chain's (3 u (|y_j|^2 + 2 |y_i| |y_j|))
+            // with those of |y_i|^2 and |y_j|^2 stay below 6 u (|y_i| +
+            // |y_j|)^2 + u R^2, inside the 8 u (2 |y_i| + 2R)^2 above
+            Tlo = R2lo - yy;
+            Thi = R2hi - yy;
             if (!(fast && de < 0.5f)) {
-                R2lo = -1.0f;  // no FP32 decision: every candidate is in the band
-                R2hi = __builtin_inff();
+                // no FP32 decision: every candidate is in the band
+                Tlo = -__builtin_inff();
+                Thi = __builtin_inff();
             }
         } else {
             R2lo = Rf * Rf * (1.0f - delta);
