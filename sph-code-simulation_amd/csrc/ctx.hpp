@@ -184,6 +184,17 @@ struct sphgpu_ctx {
     // the pre-interaction search's overflow flag has not been read yet (the
     // time step reads it together with the force-list flag, one copy)
     bool pre_search_pending = false;
+    // Search radius of the time step's pre-interaction: the reference's
+    // kappa h_guess (kappa = 1.2) scaled by pre_rscale.  Inside the time
+    // step's deferred path the radius is (1 + pre_eps) h_guess; the
+    // pre-interaction flags any particle whose Newton iterates reach it, and
+    // that step's pre-interaction then runs again at the reference's radius.
+    // pre_eps doubles after such a step (up to kappa - 1, the reference's
+    // radius) and halves again after 64 steps without one.
+    double pre_rscale = 1.0;
+    double pre_eps = 0.05;
+    int pre_quiet = 0;
+    long long pre_tight_reruns = 0;
     // set by the time-step loop around its tree build: fields that the step
     // recomputes before reading them are not permuted
     bool in_step = false;
@@ -241,7 +252,7 @@ void lists_free(Lists &l);
 // defer: launch only; the overflow flag is read later (pre_resolve)
 // flag_cleared: the overflow flag was reset by an earlier launch of the step
 int search(sphgpu_ctx *c, Lists &l, int is_ij, const double *radius, bool defer = false,
-           bool flag_cleared = false);
+           bool flag_cleared = false, double rscale = 1.0);
 // settles a pending force list (flist == 1) from the device flag: 0 = use it
 int flist_check(sphgpu_ctx *c);
 

@@ -60,6 +60,13 @@ template <typename T> struct PreArgs {
     int *fl_cnt, *fl_cnt_a;
     int fl_cap;
     int *fl_flag;                 // set when the derived list cannot be used
+    // the candidates were searched within hs_i * hscale; with tight set
+    // (hscale below 1), a particle whose Newton iterates or final h reach
+    // that radius sets bit 1 of pre_flag (the pre-interaction must run again
+    // at the reference's radius hs_i)
+    double hscale;
+    int tight;
+    int *pre_flag;
 };
 
 // neighbour loops handle candidates in groups of this many
@@ -148,11 +155,16 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
 
     // ---- smoothing length: newton_raphson (pre_interaction.cpp:227-283) ----
     T h;
+    // the radius the candidates were searched within
+    const double hst = hs64 * a.hscale;
     if (a.iterative) {
         h = T(hs64 / a.kappa);
         const T b = (TYPE == SPHGPU_DISPH || TYPE == SPHGPU_GDISPH) ? T(a.nn / A) : T(a.mass64[i] * a.nn / A);
         bool conv = false;
         int it = 0;
+        // the largest h any sum of this particle uses (the iterates and the
+        // final h)
+        T hmax = h;
         for (; it < 10;) {
             const T hb = h;
             BlockAcc<T> dn, ddn;  // compensated in blocks of 16 candidates
@@ -231,9 +243,17 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
             const T df = ddens * powh<D>(h) + T(D) * dens * powh_m1<D>(h);
             h -= tdiv(f, df);
             ++it;
+            hmax = h > hmax ? h : hmax;
             if (tabs(h - hb) < (h + hb) * T(1e-4)) { conv = true; break; }
         }
         newton_iters = it;
+        // with the tight radius every candidate r < h of each sum must have
+        // been searched: h stays below hst with room for the pair
+        // precision's rounding of r (FP32 distances: 2^-16 relative)
+        if (a.tight && valid) {
+            const double m = (sizeof(T) == 8) ? 9.094947017729282e-13 : 1.52587890625e-05;
+            if (!((double)hmax < hst * (1.0 - m))) atomicOr(a.pre_flag, 2);
+        }
         if (!conv) {
             h = T(hs64 / a.kappa);
             newton_fail = 1;
@@ -483,8 +503,8 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
             a.fl_cnt_a[i] = min(na, a.fl_cap);
             a.fl_cnt[i] = min(na, a.fl_cap);
             // complete only when every j with r < h_i was a candidate
-            // (r^2 < hs^2): h_i must stay below hs_i with room for rounding
-            const bool beyond = !(h64 < hs64 * (1.0 - 9.094947017729282e-13));
+            // (r^2 < hst^2): h_i must stay below hst with room for rounding
+            const bool beyond = !(h64 < hst * (1.0 - 9.094947017729282e-13));
             const bool other = (!fl_rows && qself < 0) || coincident || na > a.fl_cap;
             if (beyond || other) {
                 atomicOr(a.fl_flag, 1);
@@ -1553,6 +1573,9 @@ template <int D, int K, int TYPE, typename T> void launch_pre(sphgpu_ctx *c)
     a.fl_cnt_a = c->lforce.cnt_a;
     a.fl_cap = c->lforce.cap;
     a.fl_flag = c->dflag + 2;
+    a.hscale = c->pre_rscale;
+    a.tight = c->pre_rscale < 1.0 && c->prm.iterative_sml;
+    a.pre_flag = c->dflag;
     hipLaunchKernelGGL((k_pre<D, K, TYPE, T>), dim3(grid_for(c->n, 256)), dim3(256), 0, c->stream, a);
 }
 
@@ -1730,10 +1753,20 @@ int pre_resolve(sphgpu_ctx *c)
     HIPCHK(c, hipMemcpyAsync(w, c->dflag, sizeof(w), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (w[0]) {
-        // the search overflowed: grow its lists (the reference's capacity is
-        // nn * 1000) and run the pre-interaction again from the same radii
-        int rc = lists_grow(c, c->lpre);
-        if (rc) return rc;
+        // bit 0: the search overflowed: grow its lists (the reference's
+        // capacity is nn * 1000).  Bit 1: a particle's h reached the tight
+        // search radius: the next steps search wider.  Either way the
+        // pre-interaction runs again, at the reference's radius
+        if (w[0] & 2) {
+            c->pre_eps = std::min(2.0 * c->pre_eps, 0.2);
+            c->pre_quiet = 0;
+            ++c->pre_tight_reruns;
+        }
+        if (w[0] & 1) {
+            int rc = lists_grow(c, c->lpre);
+            if (rc) return rc;
+        }
+        c->pre_rscale = 1.0;
         return pre_core(c, false, false);
     }
     if (c->flist == 1) c->flist = w[2] ? 0 : 2;
@@ -1772,8 +1805,23 @@ static int pre_core(sphgpu_ctx *c, bool defer, bool guess_h)
         time_begin(c, "pre_hguess");
         hguess(c, kappa, c->hs, true);
         time_end(c);
+        // The time step's deferred pre-interaction searches within
+        // (1 + pre_eps) h_guess instead of kappa h_guess: a step's h stays
+        // within a few per cent of its guess (the Newton iteration starts
+        // there and mostly stops after one step), and every sum of the
+        // reference runs over the candidates with r < h only, so the results
+        // are the same as long as no h reaches the tight radius (checked per
+        // particle; such a step runs again at the reference's radius).
+        c->pre_rscale = 1.0;
+        if (defer && fl) {
+            if (++c->pre_quiet >= 64) {
+                c->pre_quiet = 0;
+                c->pre_eps = std::max(0.05, 0.5 * c->pre_eps);
+            }
+            if (1.0 + c->pre_eps < kappa) c->pre_rscale = (1.0 + c->pre_eps) / kappa;
+        }
     }
-    if ((rc = search(c, c->lpre, 0, c->hs, defer, guess_h))) return rc;
+    if ((rc = search(c, c->lpre, 0, c->hs, defer, guess_h, c->pre_rscale))) return rc;
     if (!guess_h)
         hipLaunchKernelGGL(k_minbits_init, dim3(1), dim3(kStatCopies), 0, c->stream, c->dmin, fl ? c->dflag + 2 : nullptr);
     time_begin(c, "pre_interaction");

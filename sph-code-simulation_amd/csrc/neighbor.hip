@@ -38,6 +38,7 @@ struct SearchArgs {
     int n_nodes;
     const double *x[3];
     const double *radius;     // hs_i (is_ij = 0) or h_i (is_ij = 1)
+    double rscale;            // the search radius is radius[i] * rscale
     const double *sml;        // h_j for is_ij
     const float4 *posf;       // FP32 (x, y, z, h) records for the prefilter
     const double *root;       // tree root geometry (bounds |x|)
@@ -187,7 +188,7 @@ __global__ __launch_bounds__(256, 4) void k_search(SearchArgs a)
     const int ii = valid ? i : (a.n - 1);
     const float4 pfi = a.posf[ii];
     const float xf[3] = {pfi.x, pfi.y, pfi.z};
-    const float Rf = (float)a.radius[ii];
+    const float Rf = (float)(a.radius[ii] * a.rscale);
     // three times the pair kernels' band: pre-shifting a candidate to the
     // image nearest the group centre adds up to 5 u X of rounding to each
     // component (10 u X + u |r| in all, against the 5 u X + u |r| the band
@@ -253,7 +254,7 @@ __global__ __launch_bounds__(256, 4) void k_search(SearchArgs a)
             xi[k] = a.x[k][ii];
             xj64[k] = a.x[k][j];
         }
-        const double R = a.radius[ii];
+        const double R = a.radius[ii] * a.rscale;
         const double r2 = r2_exact<D>(xi, xj64, a.box);
         if (IS_IJ) {
             const double hj = a.sml[j];
@@ -679,7 +680,7 @@ int flist_check(sphgpu_ctx *c)
     return c->flist == 2 ? 0 : 1;
 }
 
-int search(sphgpu_ctx *c, Lists &l, int is_ij, const double *radius, bool defer, bool flag_cleared)
+int search(sphgpu_ctx *c, Lists &l, int is_ij, const double *radius, bool defer, bool flag_cleared, double rscale)
 {
     Tree &t = c->tree;
     if (!t.valid) return fail(c, SPHGPU_E_STATE, "neighbour search without a tree");
@@ -692,6 +693,7 @@ int search(sphgpu_ctx *c, Lists &l, int is_ij, const double *radius, bool defer,
         a.n_nodes = t.n_nodes;
         for (int k = 0; k < 3; ++k) a.x[k] = c->pos[k < c->dim ? k : 0];
         a.radius = radius;
+        a.rscale = rscale;
         a.sml = c->sc[S_SML];
         a.posf = c->rec32[R_POSH];
         a.root = t.root;

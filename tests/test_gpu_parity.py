@@ -373,10 +373,57 @@ def test_deferred_search_overflow_reruns_pre():
     for r in (ics.F_POS, ics.F_VEL, ics.F_ENE, ics.F_DENS, ics.F_PRES, ics.F_SML, ics.F_ACC, ics.F_NEIGHBOR):
         np.testing.assert_array_equal(ga[r], gb[r], err_msg=str(r))
     sa, sb = a.stats(), b.stats()
-    # both searched with the same growing capacities; the step's
-    # pre-interaction kernel ran twice (clipped lists, then full ones)
-    assert sa["pre_candidates"] == sb["pre_candidates"] > 20 * 5000
+    # the step's pre-interaction kernel ran twice (on clipped lists searched
+    # within the time step's tight radius, then on grown lists at the
+    # reference's radius); the separate entry points searched at the
+    # reference's radius until the lists fitted and ran it once
+    assert sa["pre_candidates"] > 20 * 5000 and sb["pre_candidates"] > 20 * 5000
     assert sa["pre_newton_iters"] > sb["pre_newton_iters"]
+
+
+@pytest.mark.parametrize("prec", [ics.PAIR_FP64, ics.PAIR_FP32])
+def test_tight_search_radius_reruns_pre(prec):
+    """The time step searches the pre-interaction candidates within
+    (1 + eps) h_guess instead of the reference's 1.2 h_guess; a particle
+    whose Newton iterates reach that radius makes the step run its
+    pre-interaction again at the reference's radius.  Densities raised 1.3x
+    on 20 particles shrink their h guess 1.3^(-1/3) = 0.92x, so their h ends
+    ~9% above the guess: past the tight radius, inside the reference's.  The
+    step must equal the one the separate entry points take (they search at
+    the reference's radius), field for field, and must have searched twice;
+    an unperturbed step searches once."""
+    p, f = ics.sedov_3d(n1=24)
+    ctxs = []
+    for bump in (1.3, 1.3, 1.0):
+        c = gpu_ctx(p, f, pair_precision=prec)
+        c.initialize()
+        g = c.download()
+        g[ics.F_DENS, 100:120] *= bump
+        c.upload(g, mask=1 << ics.F_DENS)
+        c.enable_timing(1)
+        ctxs.append(c)
+        if len(ctxs) == 1:
+            dens_raised = g[ics.F_DENS, 100:120].copy()
+    a, b, quiet = ctxs
+    a.integrate(1)
+    quiet.integrate(1)
+    b.timestep()
+    b.predict()
+    b.build_tree()
+    b.pre_interaction()
+    b.fluid_force()
+    b.gravity()
+    b.correct()
+    ga, gb = a.download(), b.download()
+    for r in (ics.F_POS, ics.F_VEL, ics.F_ENE, ics.F_DENS, ics.F_PRES, ics.F_SML, ics.F_ACC, ics.F_NEIGHBOR):
+        np.testing.assert_array_equal(ga[r], gb[r], err_msg=str(r))
+    h_guess = (p["neighbor_number"] * f[ics.F_MASS, 100:120] / (dens_raised * 4.0 * np.pi / 3.0)) ** (1 / 3)
+    ratio = ga[ics.F_SML, 100:120] / h_guess
+    searches = [c.kernel_times()["search_pre"][1] for c in (a, b, quiet)]
+    print(f"prec={prec}: h / h_guess of the raised particles {ratio.min():.4f}..{ratio.max():.4f}; "
+          f"searches: step {searches[0]}, entry points {searches[1]}, unperturbed step {searches[2]}")
+    assert ratio.max() > 1.05
+    assert searches == [2, 1, 1]
 
 
 @pytest.mark.parametrize("n1,fits", [(33, True), (37, False)])
