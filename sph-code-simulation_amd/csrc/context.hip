@@ -256,6 +256,7 @@ int sphgpu_create(const sphgpu_params *p, int n, sphgpu_ctx **out)
     if (p->pair_precision == SPHGPU_PAIR_FP64)
         for (int r = 0; r < R_NREC && ok; ++r) ok = alloc((void **)&c->rec64[r], sizeof(double4) * (size_t)n);
     ok = ok && alloc((void **)&c->hs, dn) && alloc((void **)&c->stage, dn * SPHGPU_NFIELDS);
+    if (p->use_time_dependent_av) ok = ok && alloc((void **)&c->alpha_snap, dn);
     c->red_blocks = 1024;
     ok = ok && alloc((void **)&c->red, sizeof(double) * 8 * c->red_blocks);
     ok = ok && alloc((void **)&c->dscal, sizeof(double) * 16) && alloc((void **)&c->dmin, sizeof(unsigned long long) * kStatCopies) &&
@@ -301,7 +302,7 @@ int sphgpu_destroy(sphgpu_ctx *c)
         hipFree(c->rec32[r]);
         hipFree(c->rec64[r]);
     }
-    hipFree(c->hs); hipFree(c->stage); hipFree(c->red);
+    hipFree(c->hs); hipFree(c->alpha_snap); hipFree(c->stage); hipFree(c->red);
     hipFree(c->dscal); hipFree(c->dmin); hipFree(c->dstats); hipFree(c->dflag);
     if (c->hscal) hipHostFree(c->hscal);
     if (c->hflag) hipHostFree(c->hflag);

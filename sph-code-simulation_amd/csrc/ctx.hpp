@@ -156,6 +156,10 @@ struct sphgpu_ctx {
     double4 *rec64[sphgpu::R_NREC] = {};
     // per-particle scratch
     double *hs = nullptr;        // search radius of the pre-interaction
+    // alpha before the time step's pre-interaction (time-dependent AV): a
+    // repeated pre-interaction of the step starts its alpha update from it
+    double *alpha_snap = nullptr;
+    bool alpha_snap_on = false;
     double *stage = nullptr;     // host-exchange staging, [NFIELDS][n]
     double *red = nullptr;       // reduction partials
     int red_blocks = 0;
@@ -258,8 +262,8 @@ int flist_check(sphgpu_ctx *c);
 
 // hydro.hip
 int pre_interaction(sphgpu_ctx *c);
-// the same with its search's overflow check left to pre_resolve (not with
-// the time-dependent viscosity, whose alpha update cannot be repeated)
+// the same with its search's overflow check left to pre_resolve (with the
+// time-dependent viscosity, a repeated run takes alpha from alpha_snap)
 int pre_interaction_deferred(sphgpu_ctx *c);
 // reads a deferred search's overflow flag (and the force-list flag in the
 // same copy); after an overflow the lists grow and the pre-interaction runs

@@ -67,6 +67,9 @@ template <typename T> struct PreArgs {
     double hscale;
     int tight;
     int *pre_flag;
+    // time-dependent AV: alpha before the step's first pre-interaction (so a
+    // repeated run updates it from the same value), or null: the record's
+    const double *alpha_in;
 };
 
 // neighbour loops handle candidates in groups of this many
@@ -576,7 +579,7 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
     // GDISPH, gd_pre_interaction.cpp:150-197: Balsara sums weighted by mass
     // and divided by density, the AV without Balsara as in DISPH) ----
     const T dt = T(a.dscal[0]);
-    T alpha = a.rv.r[R_AUX][i].z;
+    T alpha = a.alpha_in ? T(a.alpha_in[i]) : a.rv.r[R_AUX][i].z;
     if (do_bal || a.td_av) {
         // sums from the density pass
         T div = bdiv, rot[3] = {brot[0], brot[1], brot[2]};
@@ -1576,6 +1579,7 @@ template <int D, int K, int TYPE, typename T> void launch_pre(sphgpu_ctx *c)
     a.hscale = c->pre_rscale;
     a.tight = c->pre_rscale < 1.0 && c->prm.iterative_sml;
     a.pre_flag = c->dflag;
+    a.alpha_in = c->alpha_snap_on ? c->alpha_snap : nullptr;
     hipLaunchKernelGGL((k_pre<D, K, TYPE, T>), dim3(grid_for(c->n, 256)), dim3(256), 0, c->stream, a);
 }
 
@@ -1743,7 +1747,10 @@ static int pre_core(sphgpu_ctx *c, bool defer, bool guess_h);
 
 int pre_interaction(sphgpu_ctx *c) { return pre_core(c, false, true); }
 
-int pre_interaction_deferred(sphgpu_ctx *c) { return pre_core(c, !c->prm.use_time_dependent_av, true); }
+int pre_interaction_deferred(sphgpu_ctx *c)
+{
+    return pre_core(c, !c->prm.use_time_dependent_av || c->alpha_snap != nullptr, true);
+}
 
 int pre_resolve(sphgpu_ctx *c)
 {
@@ -1812,6 +1819,12 @@ static int pre_core(sphgpu_ctx *c, bool defer, bool guess_h)
         // reference runs over the candidates with r < h only, so the results
         // are the same as long as no h reaches the tight radius (checked per
         // particle; such a step runs again at the reference's radius).
+        // time-dependent AV: a repeated run of this step's pre-interaction
+        // (list overflow, tight radius) starts alpha from the step's value
+        c->alpha_snap_on = defer && c->prm.use_time_dependent_av && c->alpha_snap;
+        if (c->alpha_snap_on)
+            HIPCHK(c, hipMemcpyAsync(c->alpha_snap, c->sc[S_ALPHA], sizeof(double) * c->n, hipMemcpyDeviceToDevice,
+                                     c->stream));
         c->pre_rscale = 1.0;
         if (defer && fl) {
             if (++c->pre_quiet >= 64) {

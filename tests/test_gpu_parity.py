@@ -381,18 +381,25 @@ def test_deferred_search_overflow_reruns_pre():
     assert sa["pre_newton_iters"] > sb["pre_newton_iters"]
 
 
+@pytest.mark.parametrize("case", ["sedov_3d", "evrard_lattice"])
 @pytest.mark.parametrize("prec", [ics.PAIR_FP64, ics.PAIR_FP32])
-def test_tight_search_radius_reruns_pre(prec):
+def test_tight_search_radius_reruns_pre(case, prec):
     """The time step searches the pre-interaction candidates within
     (1 + eps) h_guess instead of the reference's 1.2 h_guess; a particle
     whose Newton iterates reach that radius makes the step run its
     pre-interaction again at the reference's radius.  Densities raised 1.3x
-    on 20 particles shrink their h guess 1.3^(-1/3) = 0.92x, so their h ends
+    on 20 particles shrink their h guess 1.3^(-1/D) = 0.92x, so their h ends
     ~9% above the guess: past the tight radius, inside the reference's.  The
     step must equal the one the separate entry points take (they search at
     the reference's radius), field for field, and must have searched twice;
-    an unperturbed step searches once."""
-    p, f = ics.sedov_3d(n1=24)
+    an unperturbed step searches once.  The Evrard lattice (DISPH with the
+    time-dependent AV) checks that the repeated run starts its alpha update
+    from the step's alpha, not from the first run's result."""
+    if case == "sedov_3d":
+        p, f = ics.sedov_3d(n1=24)
+    else:
+        p, f = ics.evrard_lattice_3d(n_side=24)
+    dim = p["dim"]
     ctxs = []
     for bump in (1.3, 1.3, 1.0):
         c = gpu_ctx(p, f, pair_precision=prec)
@@ -415,12 +422,14 @@ def test_tight_search_radius_reruns_pre(prec):
     b.gravity()
     b.correct()
     ga, gb = a.download(), b.download()
-    for r in (ics.F_POS, ics.F_VEL, ics.F_ENE, ics.F_DENS, ics.F_PRES, ics.F_SML, ics.F_ACC, ics.F_NEIGHBOR):
+    for r in (ics.F_POS, ics.F_VEL, ics.F_ENE, ics.F_DENS, ics.F_PRES, ics.F_SML, ics.F_ACC, ics.F_NEIGHBOR,
+              ics.F_ALPHA, ics.F_BALSARA):
         np.testing.assert_array_equal(ga[r], gb[r], err_msg=str(r))
-    h_guess = (p["neighbor_number"] * f[ics.F_MASS, 100:120] / (dens_raised * 4.0 * np.pi / 3.0)) ** (1 / 3)
+    area = {1: 2.0, 2: np.pi, 3: 4.0 * np.pi / 3.0}[dim]
+    h_guess = (p["neighbor_number"] * f[ics.F_MASS, 100:120] / (dens_raised * area)) ** (1 / dim)
     ratio = ga[ics.F_SML, 100:120] / h_guess
     searches = [c.kernel_times()["search_pre"][1] for c in (a, b, quiet)]
-    print(f"prec={prec}: h / h_guess of the raised particles {ratio.min():.4f}..{ratio.max():.4f}; "
+    print(f"{case} prec={prec}: h / h_guess of the raised particles {ratio.min():.4f}..{ratio.max():.4f}; "
           f"searches: step {searches[0]}, entry points {searches[1]}, unperturbed step {searches[2]}")
     assert ratio.max() > 1.05
     assert searches == [2, 1, 1]
