@@ -103,7 +103,12 @@ struct Tree {
     int *sync_words = nullptr;         // [0] node count, [1] prefix run > leaf
     int *host_words = nullptr;         // pinned host copy of sync_words
     hipEvent_t words_ready = nullptr;  // recorded after that copy
-    bool full_sort = false;            // a partial sort failed its check once
+    // partial sorts order the keys on their top sorted_digits digits; a
+    // failed check sorts that build fully (full_once) and adds 4 digits to
+    // the later builds' partial sorts (full_sort once they cover the key)
+    int sorted_digits = 8;
+    bool full_once = false;
+    bool full_sort = false;
     long long partial_sorts = 0, full_sorts = 0;
     unsigned int *scan_tmp = nullptr;  // tile sums of the multi-block scans
 };

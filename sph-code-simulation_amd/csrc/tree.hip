@@ -1000,18 +1000,17 @@ int tree_build(sphgpu_ctx *c)
     time_end(c);
 
     // stable LSD radix sort over the dim*maxTreeLevel key bits.  Large
-    // 2D/3D sets sort on the top 12 digits only (the low passes are skipped)
-    // and check that no depth-12 cell holds more than leafParticleNumber
-    // particles; otherwise the tree is built again with the full sort (and
-    // every later build sorts fully).
+    // 2D/3D sets sort on the top t.sorted_digits digits only (8 at first;
+    // the low passes are skipped) and check that no cell of that depth holds
+    // more than leafParticleNumber particles; otherwise the tree is built
+    // again with the full sort, and later builds sort 4 digits more.
     time_begin(c, "tree_sort");
     const int key_bits = sh.bits * sh.ml;
     const int passes = (key_bits + 7) / 8;
     const int nb = t.hist_blocks;
-    constexpr int kSortedDigits = 12;
     int skip = 0, cut = 0;
-    if (!t.full_sort && n >= 65536 && c->dim >= 2) {
-        const int keep = (c->dim * kSortedDigits + 7) / 8;
+    if (!t.full_sort && !t.full_once && n >= 65536 && c->dim >= 2) {
+        const int keep = (c->dim * t.sorted_digits + 7) / 8;
         if (keep < passes) {
             skip = passes - keep;
             cut = key_bits - c->dim * ((key_bits - 8 * skip) / c->dim);
@@ -1065,11 +1064,15 @@ int tree_build(sphgpu_ctx *c)
     const int total = t.host_words[0];
     if (skip) {
         if (t.host_words[1]) {
-            // a depth-12 cell holds more than a leaf: sort fully from here on
-            // (the state is already in the partial order; the full build
-            // starts from it)
-            t.full_sort = true;
-            return tree_build(c);
+            // a cell of the sorted depth holds more than a leaf: this build
+            // sorts fully (the state is already in the partial order; the
+            // full build starts from it), later ones on 4 more digits
+            t.sorted_digits += 4;
+            if (t.sorted_digits >= sh.ml) t.full_sort = true;
+            t.full_once = true;
+            rc = tree_build(c);
+            t.full_once = false;
+            return rc;
         }
         ++t.partial_sorts;
     } else {

@@ -78,10 +78,11 @@ def test_tree_topology(case):
 @pytest.mark.parametrize("cluster", [False, True])
 def test_tree_large_partial_sort(cluster):
     """Above 65536 particles in 2D/3D the key sort skips its low radix passes
-    and checks that no depth-12 cell holds more than a leaf; the tree must be
+    (it sorts the top 8 digits, 4 more after each failed check) and checks
+    that no cell of the sorted depth holds more than a leaf; the tree must be
     the reference's either way.  The clustered case packs 40 particles into a
-    cell of edge 1e-7 (more than a leaf below depth 12), so the check fails
-    and the build sorts fully."""
+    cell of edge 1e-7 (more than a leaf below depth 20), so the check fails
+    in both builds and each sorts fully."""
     p, f = ics.sedov_3d(n1=48)
     f = f.copy()
     if cluster:
