@@ -201,7 +201,10 @@ int sphgpu_correct(sphgpu_ctx *ctx);
 /* Solver::initialize tail (src/solver.cpp:941-963): alpha, balsara, sound,
  * tree, pre-interaction, fluid force, gravity at dt = 0. */
 int sphgpu_initialize(sphgpu_ctx *ctx);
-/* nsteps x Solver::integrate + Simulation::update_time (src/solver.cpp:966-978) */
+/* nsteps x Solver::integrate + Simulation::update_time (src/solver.cpp:966-978).
+ * The step's PreInteraction searches its candidates within 1.05 h_guess and
+ * repeats itself at the reference's 1.2 h_guess when some h reaches that
+ * radius: the fields are the reference's either way. */
 int sphgpu_integrate(sphgpu_ctx *ctx, int nsteps);
 /* out[0..6] = kinetic, thermal, potential, total, P_x, P_y, P_z (FP64) */
 int sphgpu_energy_momentum(sphgpu_ctx *ctx, double out[7]);
