@@ -194,7 +194,7 @@ __global__ __launch_bounds__(256) void k_gravity(GravArgs<T> a)
         const bool open = act && ((g.l2 > a.theta2 * d2) || (d2 < g.l2));
         if (act && !open) {
             // monopole (src/bhtree.cpp:387-392)
-            const T r_inv = tdiv(T(1.0), tsqrt(T(d2)));
+            const T r_inv = (sizeof(T) == 4) ? trsqrt(T(d2)) : tdiv(T(1.0), tsqrt(T(d2)));
             const T gm = G * T(g.mass);
             phi.add(-(gm * r_inv));
             const T s = gm * (r_inv * r_inv * r_inv);
@@ -247,11 +247,14 @@ __global__ __launch_bounds__(256) void k_gravity(GravArgs<T> a)
                         }
 #pragma unroll
                         for (int k = 0; k < D; ++k) r2 = (k == 0) ? rij[k] * rij[k] : r2 + rij[k] * rij[k];
-                        const T r = tsqrt(r2);
                         const T gm = st.gm[t];
                         T fsum, gsum;
                         if constexpr (sizeof(T) == 4) {
-                            const float inv_r = tdiv(1.0f, r);
+                            // 1/r from one reciprocal square root, r = r^2 / r
+                            // (r = 0 for coincident particles: the polynomial
+                            // branch u < 1 takes no 1/r)
+                            const float inv_r = trsqrt(r2);
+                            const float r = r2 > 0.f ? r2 * inv_r : 0.f;
                             // beyond both softening lengths (u >= 2 on both
                             // sides, the usual case in an opened leaf) the
                             // terms are 1/r and 1/r^3; the polynomials run
@@ -268,6 +271,7 @@ __global__ __launch_bounds__(256) void k_gravity(GravArgs<T> a)
                                 gsum = 2.0f * (inv_r * inv_r * inv_r);
                             }
                         } else {
+                            const T r = tsqrt(r2);
                             fsum = hk_f(r, hi) + hk_f(r, q.w);
                             gsum = hk_g(r, hi) + hk_g(r, q.w);
                         }
