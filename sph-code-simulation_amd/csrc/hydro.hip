@@ -1812,6 +1812,12 @@ static int pre_core(sphgpu_ctx *c, bool defer, bool guess_h)
         time_begin(c, "pre_hguess");
         hguess(c, kappa, c->hs, true);
         time_end(c);
+        // time-dependent AV: a repeated run of this step's pre-interaction
+        // (list overflow, tight radius) starts alpha from the step's value
+        c->alpha_snap_on = defer && c->prm.use_time_dependent_av && c->alpha_snap;
+        if (c->alpha_snap_on)
+            HIPCHK(c, hipMemcpyAsync(c->alpha_snap, c->sc[S_ALPHA], sizeof(double) * c->n, hipMemcpyDeviceToDevice,
+                                     c->stream));
         // The time step's deferred pre-interaction searches within
         // (1 + pre_eps) h_guess instead of kappa h_guess: a step's h stays
         // within a few per cent of its guess (the Newton iteration starts
@@ -1819,12 +1825,6 @@ static int pre_core(sphgpu_ctx *c, bool defer, bool guess_h)
         // reference runs over the candidates with r < h only, so the results
         // are the same as long as no h reaches the tight radius (checked per
         // particle; such a step runs again at the reference's radius).
-        // time-dependent AV: a repeated run of this step's pre-interaction
-        // (list overflow, tight radius) starts alpha from the step's value
-        c->alpha_snap_on = defer && c->prm.use_time_dependent_av && c->alpha_snap;
-        if (c->alpha_snap_on)
-            HIPCHK(c, hipMemcpyAsync(c->alpha_snap, c->sc[S_ALPHA], sizeof(double) * c->n, hipMemcpyDeviceToDevice,
-                                     c->stream));
         c->pre_rscale = 1.0;
         if (defer && fl) {
             if (++c->pre_quiet >= 64) {
