@@ -203,7 +203,6 @@ struct sphgpu_ctx {
     double pre_rscale = 1.0;
     double pre_eps = 0.05;
     int pre_quiet = 0;
-    long long pre_tight_reruns = 0;
     // set by the time-step loop around its tree build: fields that the step
     // recomputes before reading them are not permuted
     bool in_step = false;

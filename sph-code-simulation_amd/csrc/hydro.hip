@@ -1767,7 +1767,6 @@ int pre_resolve(sphgpu_ctx *c)
         if (w[0] & 2) {
             c->pre_eps = std::min(2.0 * c->pre_eps, 0.2);
             c->pre_quiet = 0;
-            ++c->pre_tight_reruns;
         }
         if (w[0] & 1) {
             int rc = lists_grow(c, c->lpre);
