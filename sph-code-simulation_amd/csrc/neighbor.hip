@@ -231,6 +231,9 @@ __global__ __launch_bounds__(256, 4) void k_search(SearchArgs a)
     // per-piece state used by flush: group centre, and whether candidates can
     // be pre-shifted to the minimum image seen from the group centre
     float g[3] = {0, 0, 0};
+    // and the piece's half extents and largest radius (the staging cull)
+    float hb[3] = {0, 0, 0};
+    float Rg = 0.f;
     int shift = 1;
     // radius search with candidates relative to g (EXPAND): this lane's
     // -2 (x_i - g), |x_i - g|^2 and its band widened for that form
@@ -266,35 +269,50 @@ __global__ __launch_bounds__(256, 4) void k_search(SearchArgs a)
 
     // test the nb pending candidates against this lane's particle
     auto flush = [&](bool mine) {
-        tests += nb;
-        // each loading lane stores its candidate (x, y, z, h) in LDS; with
-        // `shift` the position is moved to its image nearest the group centre,
-        // which for every pair closer than the search radius is the minimum
-        // image seen from the group's particles
-        if (lane < nb && expand) {
+        // each loading lane prepares its candidate's record (x, y, z, h);
+        // with `shift` the position is moved to its image nearest the group
+        // centre, which for every pair closer than the search radius is the
+        // minimum image seen from the group's particles; the relative form
+        // stores (y, |y|^2) with y that image minus the centre.  A candidate
+        // outside the piece's box widened by the piece's largest radius (the
+        // node reach test's bound, with its padding) is within no lane's
+        // radius and is dropped; the kept ones are compacted in candidate
+        // order (the leaves' members beyond the group's reach are most of
+        // what the leaf granularity adds)
+        float4 rec = make_float4(0.f, 0.f, 0.f, 0.f);
+        bool keepc = false;
+        if (lane < nb) {
             const float4 pj = a.posf[cand];
-            float q[3] = {pj.x, pj.y, pj.z}, y[3] = {0.f, 0.f, 0.f};
+            const float q[3] = {pj.x, pj.y, pj.z};
+            float dd[3] = {0.f, 0.f, 0.f};
 #pragma unroll
             for (int k = 0; k < D; ++k) {
                 const float d = q[k] - g[k];
-                y[k] = !a.box.periodic ? d : (fabsf(d) <= 0.5f * Lf[k] ? d : (d > 0.f ? d - Lf[k] : d + Lf[k]));
+                dd[k] = !a.box.periodic ? d : (fabsf(d) <= 0.5f * Lf[k] ? d : (d > 0.f ? d - Lf[k] : d + Lf[k]));
             }
-            srec[lane] = make_float4(y[0], y[1], y[2], __builtin_fmaf(y[0], y[0], __builtin_fmaf(y[1], y[1], y[2] * y[2])));
-            sidx[lane] = cand;
-        } else if (lane < nb) {
-            float4 pj = a.posf[cand];
-            if (a.box.periodic && shift) {
-                float q[3] = {pj.x, pj.y, pj.z};
-#pragma unroll
-                for (int k = 0; k < D; ++k) {
-                    const float d = q[k] - g[k];
-                    q[k] = g[k] + (fabsf(d) <= 0.5f * Lf[k] ? d : (d > 0.f ? d - Lf[k] : d + Lf[k]));
+            if (expand) {
+                rec = make_float4(dd[0], dd[1], dd[2], __builtin_fmaf(dd[0], dd[0], __builtin_fmaf(dd[1], dd[1], dd[2] * dd[2])));
+            } else {
+                rec = pj;
+                if (a.box.periodic && shift) {
+                    rec.x = g[0] + dd[0];
+                    if (D > 1) rec.y = g[1] + dd[1];
+                    if (D > 2) rec.z = g[2] + dd[2];
                 }
-                pj.x = q[0]; pj.y = q[1]; pj.z = q[2];
             }
-            srec[lane] = pj;
-            sidx[lane] = cand;
+            const float rc = (IS_IJ ? fmaxf(Rg, pj.w) : Rg) * kReachRel + padX;
+            keepc = true;
+#pragma unroll
+            for (int k = 0; k < D; ++k) keepc = keepc && (fabsf(dd[k]) - hb[k] <= rc);
         }
+        const unsigned long long kb = __ballot(keepc);
+        if (keepc) {
+            const int pos = __popcll(kb & ((1ull << lane) - 1ull));
+            srec[pos] = rec;
+            sidx[pos] = cand;
+        }
+        nb = __popcll(kb);
+        tests += nb;
         wave_lds_fence();
         unsigned km0 = 0, km1 = 0, ku0 = 0, ku1 = 0;
         const int n0 = min(nb, 32);
@@ -372,7 +390,6 @@ __global__ __launch_bounds__(256, 4) void k_search(SearchArgs a)
         const int sub_end = min(p_end, lc.y);
         const bool mine = valid && i >= sub && i < sub_end;
         // FP32 bounding box and radius of this piece
-        float hb[3] = {0, 0, 0};
 #pragma unroll
         for (int k = 0; k < D; ++k) {
             float lo = mine ? xf[k] : 3.0e38f, hi = mine ? xf[k] : -3.0e38f;
@@ -383,7 +400,7 @@ __global__ __launch_bounds__(256, 4) void k_search(SearchArgs a)
             g[k] = 0.5f * (lo + hi);
             hb[k] = 0.5f * (hi - lo);
         }
-        float Rg = mine ? Rf : 0.f;
+        Rg = mine ? Rf : 0.f;
         for (int off = kWave / 2; off > 0; off >>= 1) Rg = fmaxf(Rg, __shfl_xor(Rg, off, kWave));
         // pre-shifting candidates is exact for pairs closer than R when
         // R + (group half extent) < L/2 on every axis (with room for rounding);
