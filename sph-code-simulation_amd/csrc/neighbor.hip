@@ -274,11 +274,11 @@ __global__ __launch_bounds__(256, 4) void k_search(SearchArgs a)
         // centre, which for every pair closer than the search radius is the
         // minimum image seen from the group's particles; the relative form
         // stores (y, |y|^2) with y that image minus the centre.  A candidate
-        // outside the piece's box widened by the piece's largest radius (the
-        // node reach test's bound, with its padding) is within no lane's
-        // radius and is dropped; the kept ones are compacted in candidate
-        // order (the leaves' members beyond the group's reach are most of
-        // what the leaf granularity adds)
+        // farther from the piece's box than the piece's largest radius (with
+        // the node reach test's padding) is within no lane's radius and is
+        // dropped; the kept ones are compacted in candidate order (the
+        // leaves' members beyond the group's reach are most of what the leaf
+        // granularity adds)
         float4 rec = make_float4(0.f, 0.f, 0.f, 0.f);
         bool keepc = false;
         if (lane < nb) {
@@ -301,9 +301,13 @@ __global__ __launch_bounds__(256, 4) void k_search(SearchArgs a)
                 }
             }
             const float rc = (IS_IJ ? fmaxf(Rg, pj.w) : Rg) * kReachRel + padX;
-            keepc = true;
+            float e2 = 0.f;  // squared distance to the piece's box
 #pragma unroll
-            for (int k = 0; k < D; ++k) keepc = keepc && (fabsf(dd[k]) - hb[k] <= rc);
+            for (int k = 0; k < D; ++k) {
+                const float e = fmaxf(fabsf(dd[k]) - hb[k], 0.f);
+                e2 = __builtin_fmaf(e, e, e2);
+            }
+            keepc = e2 <= rc * rc;
         }
         const unsigned long long kb = __ballot(keepc);
         if (keepc) {
