@@ -339,6 +339,14 @@ int sphgpu_upload(sphgpu_ctx *c, const double *fields, uint64_t mask)
         if (mask & SPHGPU_MASK(f))
             HIPCHK(c, hipMemcpyAsync(c->stage + (size_t)f * c->n, fields + (size_t)f * c->n, dn,
                                      hipMemcpyHostToDevice, c->stream));
+    if (mask & SPHGPU_MASK(SPHGPU_F_MASS)) {
+        // masses are inputs only: whether they are all equal is decided here
+        const double *m = fields + (size_t)SPHGPU_F_MASS * c->n;
+        bool same = c->n > 0;
+        for (int p = 1; same && p < c->n; ++p) same = (m[p] == m[0]);
+        c->mass_uniform = same;
+        c->mass_u = c->n > 0 ? m[0] : 0.0;
+    }
     FieldPtrs fp = field_ptrs(c);
     hipLaunchKernelGGL(k_gather_fields, dim3(grid_for(c->n, 256)), dim3(256), 0, c->stream, fp,
                        (const double *)c->stage, (const int *)c->id, c->n, (unsigned long long)mask);

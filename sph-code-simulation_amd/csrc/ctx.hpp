@@ -165,6 +165,10 @@ struct sphgpu_ctx {
     // repeated pre-interaction of the step starts its alpha update from it
     double *alpha_snap = nullptr;
     bool alpha_snap_on = false;
+    // every particle has the same mass (checked when masses are uploaded;
+    // SPH steps never change them): the density sums then skip the mass loads
+    bool mass_uniform = false;
+    double mass_u = 0.0;
     double *stage = nullptr;     // host-exchange staging, [NFIELDS][n]
     double *red = nullptr;       // reduction partials
     int red_blocks = 0;

@@ -44,6 +44,8 @@ template <typename T> struct PreArgs {
     const double *x64[3];
     const double *hs;
     const double *mass64;
+    double mass_u;     // the common mass when mass_uniform
+    int mass_uniform;
     double *sml, *dens, *pres, *gradh, *balsara, *alpha;
     int *neighbor;
     double *grad[kNGrad];
@@ -179,6 +181,10 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
             // loaded one group ahead as well, its indices two ahead)
             constexpr bool kAhead = sizeof(T) == 4;
             constexpr bool kMassW = !(TYPE == SPHGPU_DISPH || TYPE == SPHGPU_GDISPH);
+            // equal masses: m_j is the common value (the record holds the same
+            // rounded number), one record load per candidate instead of two
+            const bool mload = kMassW && !a.mass_uniform;
+            const T mu = kMassW ? T(a.mass_u) : T(1);
             int jn[kGroup];
             V4 pn[kGroup];
             T wn[kGroup];
@@ -186,8 +192,13 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
             if (kAhead && nc > 0) {
 #pragma unroll
                 for (int u = 0; u < kGroup; ++u) pn[u] = RP[jn[u]];
+                if (mload) {
 #pragma unroll
-                for (int u = 0; u < kGroup; ++u) wn[u] = kMassW ? rec_mass(RV, jn[u]) : T(1);
+                    for (int u = 0; u < kGroup; ++u) wn[u] = rec_mass(RV, jn[u]);
+                } else {
+#pragma unroll
+                    for (int u = 0; u < kGroup; ++u) wn[u] = mu;
+                }
                 if (kGroup < nc) list_group(a.idx, i, kGroup, nc, a.cap, jn);
             }
             for (int q0 = 0; q0 < nc; q0 += kGroup) {
@@ -206,8 +217,13 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
                     if (q0 + kGroup < nc) {
 #pragma unroll
                         for (int u = 0; u < kGroup; ++u) pn[u] = RP[jn[u]];
+                        if (mload) {
 #pragma unroll
-                        for (int u = 0; u < kGroup; ++u) wn[u] = kMassW ? rec_mass(RV, jn[u]) : T(1);
+                            for (int u = 0; u < kGroup; ++u) wn[u] = rec_mass(RV, jn[u]);
+                        } else {
+#pragma unroll
+                            for (int u = 0; u < kGroup; ++u) wn[u] = mu;
+                        }
                         if (q0 + 2 * kGroup < nc) list_group(a.idx, i, q0 + 2 * kGroup, nc, a.cap, jn);
                     }
                 } else {
@@ -217,8 +233,13 @@ __global__ __launch_bounds__(256) void k_pre(PreArgs<T> a)
 #pragma unroll
                     for (int u = 0; u < kGroup; ++u) pg[u] = RP[jg[u]];
                     if (q0 + kGroup < nc) list_group(a.idx, i, q0 + kGroup, nc, a.cap, jn);
+                    if (mload) {
 #pragma unroll
-                    for (int u = 0; u < kGroup; ++u) wg[u] = kMassW ? rec_mass(RV, jg[u]) : T(1);
+                        for (int u = 0; u < kGroup; ++u) wg[u] = rec_mass(RV, jg[u]);
+                    } else {
+#pragma unroll
+                        for (int u = 0; u < kGroup; ++u) wg[u] = mu;
+                    }
                 }
 #pragma unroll
                 for (int u = 0; u < kGroup; ++u) {
@@ -1549,6 +1570,8 @@ template <int D, int K, int TYPE, typename T> void launch_pre(sphgpu_ctx *c)
     for (int k = 0; k < 3; ++k) a.x64[k] = c->pos[k < c->dim ? k : 0];
     a.hs = c->hs;
     a.mass64 = c->sc[S_MASS];
+    a.mass_u = c->mass_u;
+    a.mass_uniform = c->mass_uniform ? 1 : 0;
     a.sml = c->sc[S_SML]; a.dens = c->sc[S_DENS]; a.pres = c->sc[S_PRES]; a.gradh = c->sc[S_GRADH];
     a.balsara = c->sc[S_BALSARA]; a.alpha = c->sc[S_ALPHA];
     a.neighbor = c->neighbor;

@@ -280,6 +280,29 @@ def test_short_run(case, prec):
     assert abs(drift_g - drift_o) < (1e-10 if prec == ics.PAIR_FP64 else 1e-5)
 
 
+@pytest.mark.parametrize("prec", [ics.PAIR_FP64, ics.PAIR_FP32])
+def test_short_run_unequal_masses(prec):
+    """The density sums read each candidate's mass unless every uploaded
+    mass is equal; a Sedov run with masses spread by +-5% (seeded) takes
+    the per-candidate path in the GSPH PreInteraction and must still match
+    the oracle."""
+    p, f = SMALL["sedov_3d"]()
+    rng = np.random.default_rng(7)
+    f = f.copy()
+    f[ics.F_MASS] *= 1.0 + 0.05 * rng.uniform(-1.0, 1.0, f.shape[1])
+    assert np.unique(f[ics.F_MASS]).size > 1
+    steps = 3
+    o = Oracle(p, f)
+    o.initialize()
+    o.integrate(steps)
+    of = o.get_fields()
+    g = gpu_ctx(p, f, pair_precision=prec)
+    g.initialize()
+    g.integrate(steps)
+    gf = g.download()
+    compare_fields("sedov_3d_unequal_masses", p, gf, of, prec)
+
+
 def test_riemann_batch_matches_oracle():
     from checkers import oracle_riemann
     rng = np.random.default_rng(3)
