@@ -175,6 +175,9 @@ int         sphgpu_get_params(const sphgpu_ctx *ctx, sphgpu_params *p, int32_t *
 int         sphgpu_device_info(char *name, int name_len, int *cu_count, double *hbm_gb);
 
 /* ---- state exchange (fields: [SPHGPU_NFIELDS][n] FP64, index = id) ------- */
+/* An upload that includes the mass row also records, from a host scan of
+ * that row, whether every mass is equal; the density sums then take the
+ * common mass instead of loading each neighbour's (same results). */
 int sphgpu_upload(sphgpu_ctx *ctx, const double *fields, uint64_t mask);
 int sphgpu_download(sphgpu_ctx *ctx, double *fields, uint64_t mask);
 int sphgpu_set_dt(sphgpu_ctx *ctx, double dt);
